@@ -1,0 +1,154 @@
+#!/usr/bin/env python3
+"""Benchmark of the gfx950 BMFR frame chain on one MI355X.
+
+Workload (the headline metric's configuration): plain single-stream BMFR at
+1280x720, the seeded procedural plane + 24 spheres scene with a moving
+camera, FP32 G-buffers and FP16 tmp_data (the reference defaults).  Per
+frame the renderer stand-in writes the frame's G-buffers straight into the
+stream's device buffers (not timed, like the reference's uploads,
+opencl/bmfr.cpp:415-416); the five stages are then timed with HIP events on
+the stream's own HIP stream.  ms/frame is the reference's "Total time in all
+kernels": first stage's start to last stage's end (opencl/bmfr.cpp:497-502),
+averaged over the timed frames.
+
+Prints one JSON line.  `--gpus N` with N > 1 exits: every GPU box here has a
+single GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+W, H = 1280, 720
+SPHERES = 24
+SEED = 0xB3F40000 + 1
+
+
+def stage_bytes(P, Q, nB, s=4, t=2):
+    """Algorithmic HBM bytes per launch of each stage (unique bytes: every
+    input texel read once, every output texel written once)."""
+    return {
+        "accumulate_noisy_data": P * (21 * s + 11) + 13 * t * Q,
+        "fitter": 13 * t * Q + 168 * nB,
+        "weighted_sum": P * (6 * s + 12) + 168 * nB,
+        "accumulate_filtered_data": P * (46 + 6 * s),
+        "taa": P * (20 + 6 * s),
+    }
+
+
+STAGES = ["accumulate_noisy_data", "fitter", "weighted_sum", "accumulate_filtered_data", "taa"]
+
+
+def run_plain(args):
+    import bmfr_amd
+    p = bmfr_amd.default_params(W, H)
+    st = bmfr_amd.Stream(p)
+    if args.fitter_config:
+        st.set_fitter_config(*[int(x) for x in args.fitter_config.split(",")])
+    sc = bmfr_amd.Scene(SEED, SPHERES, 1e-3, W, H)
+    st.set_timing(True)
+    n_frames = args.warmup + args.steps
+    cams = [sc.frame_args(f) for f in range(n_frames)]
+    per_frame = []
+    wall0 = None
+    for f in range(n_frames):
+        if f == args.warmup:
+            st.synchronize()
+            wall0 = time.perf_counter()
+        sc.render_to_stream(f, st)
+        vp, off = cams[f]
+        st.run_frame(f, vp, off)
+        if f >= args.warmup:
+            per_frame.append(st.stage_times())
+    st.synchronize()
+    wall = time.perf_counter() - wall0
+    t = np.array(per_frame, np.float64)  # [frames, 6]
+    s = st.sizes
+    st.close()
+    sc.close()
+    return t, wall, s
+
+
+def cpu_baseline(frames=8):
+    """The CPU oracle (a scalar C restatement, OpenMP over rows/blocks) on the
+    host cores: 720p frames of the same scene; ms/frame over frames 1..N-1."""
+    import bmfr_amd
+    from oracle import bind
+    sc = bmfr_amd.Scene(SEED, SPHERES, 1e-3, W, H)
+    o = bind.OracleStream(bind.params(W, H, half_tmp=True))
+    times = []
+    for f in range(frames):
+        g = sc.render_host(f)
+        o.upload(g["albedo"], g["normals"], g["positions"], g["noisy"])
+        vp, off = sc.frame_args(f)
+        t0 = time.perf_counter()
+        o.run_frame(f, vp, off)
+        times.append(time.perf_counter() - t0)
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": 1000.0 * float(np.mean(times[1:])), "unit": "ms/frame", "cores": cores,
+            "kind": "port",
+            "sample": f"CPU oracle (oracle/bmfr_oracle.c, OpenMP) on {frames} frames of the same "
+                      f"1280x720 scene; mean of frames 1..{frames - 1}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--fitter-config", default="")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the path shards by whole render streams, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    t, wall, s = run_plain(args)
+    ms_frame = float(t[:, 5].mean())
+    stage_ms = {name: float(t[:, i].mean()) for i, name in enumerate(STAGES)}
+    P, Q, nB = s.pixels, s.padded_pixels, s.blocks
+    sb = stage_bytes(P, Q, nB)
+    stage_gbs = {k: sb[k] / (stage_ms[k] * 1e-3) / 1e9 for k in STAGES}
+    dom = max(STAGES, key=lambda k: stage_ms[k])
+    roof = {"kernel": dom, "bound": "hbm", "achieved": round(stage_gbs[dom], 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(stage_gbs[dom] / HBM_PEAK_GBS, 4),
+            "traffic": None, "algorithmic_bytes": int(sb[dom])}
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline()
+    out = {
+        "metric": "720p ms/frame plain vs batched; p99 ms vs streams; fitter % LDS/HBM peak",
+        "value": round(ms_frame, 4),
+        "unit": "ms/frame",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_frame, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded procedural plane + 24 spheres, moving camera, rendered on the GPU",
+        "config": {"workload": "plain BMFR single stream 1280x720 (984 blocks/frame), FP32 "
+                               "G-buffers, FP16 tmp_data", "width": W, "height": H,
+                   "blocks": nB, "frames": args.warmup + args.steps},
+        "mpixel_per_s": round(P / (ms_frame * 1e-3) / 1e6, 1),
+        "frames_per_s_wall": round(args.steps / wall, 1),
+        "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
+        "stage_gbs": {k: round(v, 1) for k, v in stage_gbs.items()},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,264 @@
+"""Host-side mirror of the reference's frame API over the gfx950 C ABI.
+
+The reference drives five OpenCL kernels per frame from C++
+(opencl/bmfr.cpp:417-485): accumulate_noisy_data, fitter, weighted_sum,
+accumulate_filtered_data, taa, then swaps its double buffers.  ``Stream``
+exposes the same five calls (same names, order and argument meaning) on a
+device-resident stream from ``libbmfr_hip.so``; every call goes to a
+hand-written HIP kernel.  Failures raise ``BmfrError`` carrying the C
+status code.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import BmfrParams, BmfrSizes
+
+# enum bmfr_status
+OK = 0
+ERR_INVALID_ARGUMENT = 1
+ERR_OUT_OF_BUDGET = 2
+ERR_HIP = 3
+ERR_NO_DEVICE = 4
+ERR_STATE = 5
+
+STORAGE_F32 = 0
+STORAGE_F16 = 1
+QR_FP32 = 0
+QR_KAHAN = 1
+QR_KAHAN_FP64 = 2
+
+# enum bmfr_buffer: id -> (name, element dtype key, elements per pixel kind)
+BUFFERS = {
+    "albedo": 0, "normals_cur": 1, "normals_prev": 2, "positions_cur": 3,
+    "positions_prev": 4, "noisy_in": 5, "noisy_acc_cur": 6, "noisy_acc_prev": 7,
+    "spp_cur": 8, "spp_prev": 9, "prev_pixel": 10, "accept": 11, "tmp_data": 12,
+    "weights": 13, "mins_maxs": 14, "filtered": 15, "tone_mapped": 16,
+    "out_cur": 17, "out_prev": 18, "result_cur": 19, "result_prev": 20,
+}
+_STORAGE_BUFFERS = {"normals_cur", "normals_prev", "positions_cur", "positions_prev",
+                    "noisy_in", "noisy_acc_cur", "noisy_acc_prev", "out_cur", "out_prev",
+                    "result_cur", "result_prev"}
+_U8_BUFFERS = {"spp_cur", "spp_prev", "accept"}
+
+
+class BmfrError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"bmfr status {status}: {message}")
+        self.status = status
+
+
+def _check(status):
+    if status != OK:
+        lib = _lib.load()
+        raise BmfrError(status, lib.bmfr_last_error().decode(errors="replace"))
+
+
+def _fp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _f32(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} floats, got {a.size}")
+    return a
+
+
+def device_count():
+    return _lib.load().bmfr_device_count()
+
+
+def default_params(width, height, **overrides):
+    """Reference defaults (opencl/bmfr.cpp:58-62, 88) with optional overrides."""
+    lib = _lib.load()
+    p = BmfrParams()
+    st = lib.bmfr_default_params(width, height, ctypes.byref(p))
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    if overrides or st:
+        s = BmfrSizes()
+        st = lib.bmfr_compute_sizes(ctypes.byref(p), ctypes.byref(s))
+    _check(st)
+    return p
+
+
+def compute_sizes(params):
+    lib = _lib.load()
+    s = BmfrSizes()
+    _check(lib.bmfr_compute_sizes(ctypes.byref(params), ctypes.byref(s)))
+    return s
+
+
+class Stream:
+    """One render stream with device-resident buffers and history."""
+
+    def __init__(self, params):
+        self._lib = _lib.load()
+        self.params = params
+        h = ctypes.c_void_p()
+        _check(self._lib.bmfr_stream_open(ctypes.byref(params), ctypes.byref(h)))
+        self._h = h
+        s = BmfrSizes()
+        _check(self._lib.bmfr_stream_sizes(h, ctypes.byref(s)))
+        self.sizes = s
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            _check(self._lib.bmfr_stream_close(self._h))
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- inputs ---------------------------------------------------------------
+    def upload_frame(self, albedo, normals, positions, noisy):
+        n = self.params.width * self.params.height * 3
+        arrs = [_f32(x, n) for x in (albedo, normals, positions, noisy)]
+        _check(self._lib.bmfr_upload_frame(self._h, *[_fp(a) for a in arrs]))
+
+    # -- the five stages (opencl/bmfr.cpp:429-476) --------------------------------
+    def accumulate_noisy_data(self, frame, prev_frame_camera_matrix, pixel_offset):
+        m = _f32(prev_frame_camera_matrix, 16)
+        o = _f32(pixel_offset, 2)
+        _check(self._lib.bmfr_accumulate_noisy_data(self._h, frame, _fp(m), _fp(o)))
+
+    def fitter(self, frame):
+        _check(self._lib.bmfr_fitter(self._h, frame))
+
+    def weighted_sum(self, frame):
+        _check(self._lib.bmfr_weighted_sum(self._h, frame))
+
+    def accumulate_filtered_data(self, frame):
+        _check(self._lib.bmfr_accumulate_filtered_data(self._h, frame))
+
+    def taa(self, frame):
+        _check(self._lib.bmfr_taa(self._h, frame))
+
+    def end_frame(self):
+        _check(self._lib.bmfr_end_frame(self._h))
+
+    def run_frame(self, frame, prev_frame_camera_matrix, pixel_offset):
+        m = _f32(prev_frame_camera_matrix, 16)
+        o = _f32(pixel_offset, 2)
+        _check(self._lib.bmfr_run_frame(self._h, frame, _fp(m), _fp(o)))
+
+    def set_fitter_config(self, rows_per_lane, blocks_per_group):
+        _check(self._lib.bmfr_set_fitter_config(self._h, rows_per_lane, blocks_per_group))
+
+    def synchronize(self):
+        _check(self._lib.bmfr_synchronize(self._h))
+
+    def reset(self):
+        _check(self._lib.bmfr_stream_reset(self._h))
+
+    # -- outputs ----------------------------------------------------------------
+    def download_result(self):
+        out = np.empty(self.params.width * self.params.height * 3, np.float32)
+        _check(self._lib.bmfr_download_result(self._h, _fp(out)))
+        return out
+
+    def buffer_dtype(self, name):
+        if name in _U8_BUFFERS:
+            return np.uint8
+        if name == "tmp_data":
+            return np.float16 if self.params.tmp_storage == STORAGE_F16 else np.float32
+        if name in _STORAGE_BUFFERS:
+            return np.float16 if self.params.storage == STORAGE_F16 else np.float32
+        return np.float32
+
+    def buffer_bytes(self, name):
+        n = ctypes.c_size_t()
+        _check(self._lib.bmfr_buffer_bytes(self._h, BUFFERS[name], ctypes.byref(n)))
+        return n.value
+
+    def read_buffer(self, name):
+        dt = np.dtype(self.buffer_dtype(name))
+        nbytes = self.buffer_bytes(name)
+        out = np.empty(nbytes // dt.itemsize, dt)
+        _check(self._lib.bmfr_read_buffer(self._h, BUFFERS[name], out.ctypes.data, nbytes))
+        return out
+
+    def write_buffer(self, name, array):
+        dt = np.dtype(self.buffer_dtype(name))
+        a = np.ascontiguousarray(array, dtype=dt)
+        nbytes = self.buffer_bytes(name)
+        if a.nbytes != nbytes:
+            raise ValueError(f"{name} holds {nbytes} bytes, got {a.nbytes}")
+        _check(self._lib.bmfr_write_buffer(self._h, BUFFERS[name], a.ctypes.data, nbytes))
+
+    def buffer_device_ptr(self, name):
+        p = ctypes.c_void_p()
+        _check(self._lib.bmfr_buffer_device_ptr(self._h, BUFFERS[name], ctypes.byref(p)))
+        return p.value
+
+    # -- timing -------------------------------------------------------------------
+    def set_timing(self, enable=True):
+        _check(self._lib.bmfr_set_timing(self._h, 1 if enable else 0))
+
+    def stage_times(self):
+        """ms of the last frame's five stages plus the first-start to
+        last-end total (opencl/bmfr.cpp:497-502)."""
+        out = np.empty(6, np.float32)
+        _check(self._lib.bmfr_stage_times(self._h, _fp(out)))
+        return out
+
+
+class Scene:
+    """Seeded procedural plane+spheres scene (the input producer)."""
+
+    def __init__(self, seed, spheres, firefly_probability, width, height, static_camera=False):
+        self._lib = _lib.load()
+        h = ctypes.c_void_p()
+        _check(self._lib.bmfr_scene_create(seed, spheres, firefly_probability, width, height,
+                                           1 if static_camera else 0, ctypes.byref(h)))
+        self._h = h
+        self.width, self.height = width, height
+
+    def close(self):
+        if self._h:
+            self._lib.bmfr_scene_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def camera(self, frame):
+        vp = np.empty(16, np.float32)
+        off = np.empty(2, np.float32)
+        _check(self._lib.bmfr_scene_camera(self._h, frame, _fp(vp), _fp(off)))
+        return vp, off
+
+    def render_host(self, frame):
+        n = self.width * self.height * 3
+        bufs = [np.empty(n, np.float32) for _ in range(4)]
+        _check(self._lib.bmfr_scene_render_host(self._h, frame, *[_fp(b) for b in bufs]))
+        return dict(zip(("albedo", "normals", "positions", "noisy"), bufs))
+
+    def render_to_stream(self, frame, stream):
+        _check(self._lib.bmfr_scene_render_to_stream(self._h, frame, stream.handle))
+
+    def frame_args(self, frame):
+        """(prev_frame_camera_matrix, pixel_offset) of `frame` as the reference
+        binds them: camera_matrices[max(frame-1, 0)], pixel_offsets[frame]
+        (opencl/bmfr.cpp:440-444)."""
+        vp, _ = self.camera(max(frame - 1, 0))
+        _, off = self.camera(frame)
+        return vp, off

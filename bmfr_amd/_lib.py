@@ -1,0 +1,106 @@
+"""ctypes binding of libbmfr_hip.so (the C ABI declared in include/bmfr_hip.h).
+
+The shared library is built in-tree by ``bmfr_amd/csrc/Makefile``.  Loading
+fails loudly when it is missing: there is no CPU fallback for any stage.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbmfr_hip.so")
+
+c_int32 = ctypes.c_int32
+c_float_p = ctypes.POINTER(ctypes.c_float)
+
+
+class BmfrParams(ctypes.Structure):
+    """struct bmfr_params (include/bmfr_hip.h); the reference's -D options
+    of opencl/bmfr.cpp:205-232 as runtime fields."""
+    _fields_ = [
+        ("width", ctypes.c_int32),
+        ("height", ctypes.c_int32),
+        ("position_limit_squared", ctypes.c_float),
+        ("normal_limit_squared", ctypes.c_float),
+        ("blend_alpha", ctypes.c_float),
+        ("second_blend_alpha", ctypes.c_float),
+        ("taa_blend_alpha", ctypes.c_float),
+        ("noise_amount", ctypes.c_double),
+        ("storage", ctypes.c_int32),
+        ("tmp_storage", ctypes.c_int32),
+        ("qr_mode", ctypes.c_int32),
+        ("refactor_threshold", ctypes.c_float),
+    ]
+
+
+class BmfrSizes(ctypes.Structure):
+    _fields_ = [
+        ("workset_width", ctypes.c_int32),
+        ("workset_height", ctypes.c_int32),
+        ("margin_width", ctypes.c_int32),
+        ("margin_height", ctypes.c_int32),
+        ("blocks_x", ctypes.c_int32),
+        ("blocks_y", ctypes.c_int32),
+        ("blocks", ctypes.c_int32),
+        ("pixels", ctypes.c_int64),
+        ("padded_pixels", ctypes.c_int64),
+        ("device_bytes", ctypes.c_int64),
+    ]
+
+
+# (name, restype, argtypes) of every exported entry point.
+SIGNATURES = [
+    ("bmfr_api_version", c_int32, []),
+    ("bmfr_status_string", ctypes.c_char_p, [c_int32]),
+    ("bmfr_last_error", ctypes.c_char_p, []),
+    ("bmfr_device_count", c_int32, []),
+    ("bmfr_default_params", c_int32, [c_int32, c_int32, ctypes.POINTER(BmfrParams)]),
+    ("bmfr_compute_sizes", c_int32, [ctypes.POINTER(BmfrParams), ctypes.POINTER(BmfrSizes)]),
+    ("bmfr_stream_open", c_int32, [ctypes.POINTER(BmfrParams), ctypes.POINTER(ctypes.c_void_p)]),
+    ("bmfr_stream_close", c_int32, [ctypes.c_void_p]),
+    ("bmfr_stream_sizes", c_int32, [ctypes.c_void_p, ctypes.POINTER(BmfrSizes)]),
+    ("bmfr_stream_reset", c_int32, [ctypes.c_void_p]),
+    ("bmfr_set_fitter_config", c_int32, [ctypes.c_void_p, c_int32, c_int32]),
+    ("bmfr_upload_frame", c_int32, [ctypes.c_void_p, c_float_p, c_float_p, c_float_p, c_float_p]),
+    ("bmfr_accumulate_noisy_data", c_int32, [ctypes.c_void_p, c_int32, c_float_p, c_float_p]),
+    ("bmfr_fitter", c_int32, [ctypes.c_void_p, c_int32]),
+    ("bmfr_weighted_sum", c_int32, [ctypes.c_void_p, c_int32]),
+    ("bmfr_accumulate_filtered_data", c_int32, [ctypes.c_void_p, c_int32]),
+    ("bmfr_taa", c_int32, [ctypes.c_void_p, c_int32]),
+    ("bmfr_end_frame", c_int32, [ctypes.c_void_p]),
+    ("bmfr_run_frame", c_int32, [ctypes.c_void_p, c_int32, c_float_p, c_float_p]),
+    ("bmfr_download_result", c_int32, [ctypes.c_void_p, c_float_p]),
+    ("bmfr_synchronize", c_int32, [ctypes.c_void_p]),
+    ("bmfr_buffer_bytes", c_int32, [ctypes.c_void_p, c_int32, ctypes.POINTER(ctypes.c_size_t)]),
+    ("bmfr_read_buffer", c_int32, [ctypes.c_void_p, c_int32, ctypes.c_void_p, ctypes.c_size_t]),
+    ("bmfr_write_buffer", c_int32, [ctypes.c_void_p, c_int32, ctypes.c_void_p, ctypes.c_size_t]),
+    ("bmfr_buffer_device_ptr", c_int32, [ctypes.c_void_p, c_int32, ctypes.POINTER(ctypes.c_void_p)]),
+    ("bmfr_set_timing", c_int32, [ctypes.c_void_p, c_int32]),
+    ("bmfr_stage_times", c_int32, [ctypes.c_void_p, c_float_p]),
+    ("bmfr_scene_create", c_int32, [ctypes.c_uint64, c_int32, ctypes.c_float, c_int32, c_int32,
+                                    c_int32, ctypes.POINTER(ctypes.c_void_p)]),
+    ("bmfr_scene_destroy", c_int32, [ctypes.c_void_p]),
+    ("bmfr_scene_camera", c_int32, [ctypes.c_void_p, c_int32, c_float_p, c_float_p]),
+    ("bmfr_scene_render_host", c_int32, [ctypes.c_void_p, c_int32, c_float_p, c_float_p,
+                                         c_float_p, c_float_p]),
+    ("bmfr_scene_render_to_stream", c_int32, [ctypes.c_void_p, c_int32, ctypes.c_void_p]),
+]
+
+_lib = None
+
+
+def load():
+    """Load libbmfr_hip.so once; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `make -C bmfr_amd/csrc` "
+            "(or __graft_entry__.build()); there is no CPU fallback")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib

@@ -1,0 +1,643 @@
+// C-ABI host layer of the gfx950 BMFR library (include/bmfr_hip.h).
+//
+// A bmfr_stream owns one render stream's device buffers (the reference's
+// cl::Buffer set, opencl/bmfr.cpp:315-347, sized at run time from W and H)
+// in a single allocation, its own HIP stream (the reference's in-order
+// queue, bmfr.cpp:191) and the double-buffer parity (Double_buffer,
+// bmfr.cpp:120-135).  Stage calls resolve the current/previous buffers and
+// launch the matching kernel.  Nothing here throws or exits: failures come
+// back as bmfr_status codes with a thread-local message.
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "bmfr_hip.h"
+#include "kernels.h"
+#include "scene.h"
+#include "scene_launch.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int32_t fail(int32_t code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(BMFR_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));       \
+    } while (0)
+
+constexpr int kStages = 5;
+
+// Physical buffers; double buffers take two slots.
+enum Slot {
+    S_ALBEDO,
+    S_NORMALS_A, S_NORMALS_B,
+    S_POSITIONS_A, S_POSITIONS_B,
+    S_NOISY_IN,
+    S_NOISY_ACC_A, S_NOISY_ACC_B,
+    S_SPP_A, S_SPP_B,
+    S_PREV_PIXEL, S_ACCEPT, S_TMP, S_WEIGHTS, S_MINS_MAXS, S_FILTERED, S_TONE,
+    S_OUT_A, S_OUT_B,
+    S_RESULT_A, S_RESULT_B,
+    S_COUNT
+};
+
+int32_t validate(const bmfr_params *p) {
+    if (!p) return fail(BMFR_ERR_INVALID_ARGUMENT, "params is null");
+    // Mirroring at the padded-grid margins is valid only when both image
+    // dimensions exceed the 32-pixel block (opencl/bmfr.cl:312-313).
+    if (p->width <= 32 || p->height <= 32 || p->width > 16384 || p->height > 16384)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "image size %dx%d outside 33..16384", p->width,
+                    p->height);
+    if (p->storage != BMFR_STORAGE_F32 && p->storage != BMFR_STORAGE_F16)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "unknown storage %d", p->storage);
+    if (p->tmp_storage != BMFR_STORAGE_F32 && p->tmp_storage != BMFR_STORAGE_F16)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "unknown tmp_storage %d", p->tmp_storage);
+    if (p->qr_mode < BMFR_QR_FP32 || p->qr_mode > BMFR_QR_KAHAN_FP64)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "unknown qr_mode %d", p->qr_mode);
+    if (!(p->blend_alpha >= 0.f && p->blend_alpha <= 1.f) ||
+        !(p->second_blend_alpha >= 0.f && p->second_blend_alpha <= 1.f) ||
+        !(p->taa_blend_alpha >= 0.f && p->taa_blend_alpha <= 1.f))
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "blend factors must lie in [0, 1]");
+    return BMFR_OK;
+}
+
+void sizes_of(const bmfr_params &p, bmfr_sizes &s, size_t *slot_bytes) {
+    // opencl/bmfr.cpp:107-118
+    s.workset_width = 32 * ((p.width + 31) / 32);
+    s.workset_height = 32 * ((p.height + 31) / 32);
+    s.margin_width = s.workset_width + 32;
+    s.margin_height = s.workset_height + 32;
+    s.blocks_x = s.margin_width / 32;
+    s.blocks_y = s.margin_height / 32;
+    s.blocks = s.blocks_x * s.blocks_y;
+    s.pixels = (int64_t)p.width * p.height;
+    s.padded_pixels = (int64_t)s.margin_width * s.margin_height;
+    const size_t P = (size_t)s.pixels;
+    const size_t g = p.storage == BMFR_STORAGE_F16 ? 2 : 4;
+    const size_t t = p.tmp_storage == BMFR_STORAGE_F16 ? 2 : 4;
+    size_t b[S_COUNT];
+    b[S_ALBEDO] = 12 * P;
+    b[S_NORMALS_A] = b[S_NORMALS_B] = 3 * g * P;
+    b[S_POSITIONS_A] = b[S_POSITIONS_B] = 3 * g * P;
+    b[S_NOISY_IN] = 3 * g * P;
+    b[S_NOISY_ACC_A] = b[S_NOISY_ACC_B] = 3 * g * P;
+    b[S_SPP_A] = b[S_SPP_B] = P;
+    b[S_PREV_PIXEL] = 8 * P;
+    b[S_ACCEPT] = P;
+    b[S_TMP] = 13 * t * (size_t)s.padded_pixels;
+    b[S_WEIGHTS] = 30 * 4 * (size_t)s.blocks;
+    b[S_MINS_MAXS] = 12 * 4 * (size_t)s.blocks;
+    b[S_FILTERED] = 12 * P;
+    b[S_TONE] = 12 * P;
+    b[S_OUT_A] = b[S_OUT_B] = 3 * g * P;
+    b[S_RESULT_A] = b[S_RESULT_B] = 3 * g * P;
+    size_t total = 0;
+    for (int i = 0; i < S_COUNT; ++i) {
+        if (slot_bytes) slot_bytes[i] = b[i];
+        total += (b[i] + 255) & ~(size_t)255;
+    }
+    s.device_bytes = (int64_t)total;
+}
+
+}  // namespace
+
+struct bmfr_stream_s {
+    bmfr_params p{};
+    bmfr_sizes sz{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    char *base = nullptr;
+    void *slot[S_COUNT]{};
+    size_t slot_bytes[S_COUNT]{};
+    bool swapped = false;   // Double_buffer::swapped
+    int last_result = -1;   // slot holding the latest TAA output
+    bool timing = false;
+    hipEvent_t ev_start[kStages]{}, ev_end[kStages]{};
+    bool recorded[kStages]{};
+    bmfr::FitterConfig fitter{8, 1};
+
+    int cur(int a_slot) const { return swapped ? a_slot : a_slot + 1; }
+    int prev(int a_slot) const { return swapped ? a_slot + 1 : a_slot; }
+
+    int resolve(int id) const {
+        switch (id) {
+            case BMFR_BUF_ALBEDO: return S_ALBEDO;
+            case BMFR_BUF_NORMALS_CUR: return cur(S_NORMALS_A);
+            case BMFR_BUF_NORMALS_PREV: return prev(S_NORMALS_A);
+            case BMFR_BUF_POSITIONS_CUR: return cur(S_POSITIONS_A);
+            case BMFR_BUF_POSITIONS_PREV: return prev(S_POSITIONS_A);
+            case BMFR_BUF_NOISY_IN: return S_NOISY_IN;
+            case BMFR_BUF_NOISY_ACC_CUR: return cur(S_NOISY_ACC_A);
+            case BMFR_BUF_NOISY_ACC_PREV: return prev(S_NOISY_ACC_A);
+            case BMFR_BUF_SPP_CUR: return cur(S_SPP_A);
+            case BMFR_BUF_SPP_PREV: return prev(S_SPP_A);
+            case BMFR_BUF_PREV_PIXEL: return S_PREV_PIXEL;
+            case BMFR_BUF_ACCEPT: return S_ACCEPT;
+            case BMFR_BUF_TMP_DATA: return S_TMP;
+            case BMFR_BUF_WEIGHTS: return S_WEIGHTS;
+            case BMFR_BUF_MINS_MAXS: return S_MINS_MAXS;
+            case BMFR_BUF_FILTERED: return S_FILTERED;
+            case BMFR_BUF_TONE_MAPPED: return S_TONE;
+            case BMFR_BUF_OUT_CUR: return cur(S_OUT_A);
+            case BMFR_BUF_OUT_PREV: return prev(S_OUT_A);
+            case BMFR_BUF_RESULT_CUR: return cur(S_RESULT_A);
+            case BMFR_BUF_RESULT_PREV: return prev(S_RESULT_A);
+            default: return -1;
+        }
+    }
+
+    bmfr::StageBuffers buffers() const {
+        bmfr::StageBuffers b;
+        b.albedo = slot[S_ALBEDO];
+        b.normals_cur = slot[cur(S_NORMALS_A)];
+        b.normals_prev = slot[prev(S_NORMALS_A)];
+        b.positions_cur = slot[cur(S_POSITIONS_A)];
+        b.positions_prev = slot[prev(S_POSITIONS_A)];
+        b.noisy_in = slot[S_NOISY_IN];
+        b.noisy_acc_cur = slot[cur(S_NOISY_ACC_A)];
+        b.noisy_acc_prev = slot[prev(S_NOISY_ACC_A)];
+        b.spp_cur = slot[cur(S_SPP_A)];
+        b.spp_prev = slot[prev(S_SPP_A)];
+        b.prev_pixel = slot[S_PREV_PIXEL];
+        b.accept = slot[S_ACCEPT];
+        b.tmp_data = slot[S_TMP];
+        b.weights = slot[S_WEIGHTS];
+        b.mins_maxs = slot[S_MINS_MAXS];
+        b.filtered = slot[S_FILTERED];
+        b.tone_mapped = slot[S_TONE];
+        b.out_cur = slot[cur(S_OUT_A)];
+        b.out_prev = slot[prev(S_OUT_A)];
+        b.result_cur = slot[cur(S_RESULT_A)];
+        b.result_prev = slot[prev(S_RESULT_A)];
+        return b;
+    }
+
+    bmfr::StageArgs args(int frame) const {
+        bmfr::StageArgs a{};
+        a.width = p.width;
+        a.height = p.height;
+        a.margin_width = sz.margin_width;
+        a.margin_height = sz.margin_height;
+        a.blocks_x = sz.blocks_x;
+        a.blocks = sz.blocks;
+        a.position_limit_squared = p.position_limit_squared;
+        a.normal_limit_squared = p.normal_limit_squared;
+        a.blend_alpha = p.blend_alpha;
+        a.second_blend_alpha = p.second_blend_alpha;
+        a.taa_blend_alpha = p.taa_blend_alpha;
+        a.noise_amount = p.noise_amount;
+        a.frame = frame;
+        a.storage_f16 = p.storage == BMFR_STORAGE_F16;
+        a.tmp_f16 = p.tmp_storage == BMFR_STORAGE_F16;
+        return a;
+    }
+};
+
+struct bmfr_scene_s {
+    bmfr::SceneData data{};
+    bmfr::SceneData *device_copy = nullptr;
+};
+
+namespace {
+
+int32_t check_stream(bmfr_stream s) {
+    if (!s) return fail(BMFR_ERR_INVALID_ARGUMENT, "stream is null");
+    return BMFR_OK;
+}
+
+int32_t check_frame(int32_t frame) {
+    if (frame < 0) return fail(BMFR_ERR_INVALID_ARGUMENT, "frame_number %d < 0", frame);
+    return BMFR_OK;
+}
+
+template <typename F>
+int32_t timed_stage(bmfr_stream s, int stage, F &&launch) {
+    HIP_TRY(hipSetDevice(s->device));
+    if (s->timing) HIP_TRY(hipEventRecord(s->ev_start[stage], s->stream));
+    hipError_t e = launch();
+    if (e != hipSuccess) return fail(BMFR_ERR_HIP, "stage %d launch failed: %s", stage, hipGetErrorString(e));
+    if (s->timing) {
+        HIP_TRY(hipEventRecord(s->ev_end[stage], s->stream));
+        s->recorded[stage] = true;
+    }
+    return BMFR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t bmfr_api_version(void) { return BMFR_API_VERSION; }
+
+const char *bmfr_status_string(int32_t status) {
+    switch (status) {
+        case BMFR_OK: return "ok";
+        case BMFR_ERR_INVALID_ARGUMENT: return "invalid argument";
+        case BMFR_ERR_OUT_OF_BUDGET: return "out of device-memory budget";
+        case BMFR_ERR_HIP: return "HIP runtime error";
+        case BMFR_ERR_NO_DEVICE: return "no usable gfx950 device";
+        case BMFR_ERR_STATE: return "invalid state";
+        default: return "unknown status";
+    }
+}
+
+const char *bmfr_last_error(void) { return g_last_error.c_str(); }
+
+int32_t bmfr_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int32_t bmfr_default_params(int32_t width, int32_t height, bmfr_params *out) {
+    if (!out) return fail(BMFR_ERR_INVALID_ARGUMENT, "out is null");
+    // opencl/bmfr.cpp:58-62, 85-88; the position/normal limits come from the
+    // dataset header in the reference (bmfr.cpp:226-227) and are chosen here.
+    out->width = width;
+    out->height = height;
+    out->position_limit_squared = 0.01f;
+    out->normal_limit_squared = 0.1f;
+    out->blend_alpha = 0.2f;
+    out->second_blend_alpha = 0.1f;
+    out->taa_blend_alpha = 0.2f;
+    out->noise_amount = 1e-2;
+    out->storage = BMFR_STORAGE_F32;
+    out->tmp_storage = BMFR_STORAGE_F16;
+    out->qr_mode = BMFR_QR_FP32;
+    out->refactor_threshold = 1e-6f;
+    return validate(out);
+}
+
+int32_t bmfr_compute_sizes(const bmfr_params *params, bmfr_sizes *out) {
+    int32_t st = validate(params);
+    if (st) return st;
+    if (!out) return fail(BMFR_ERR_INVALID_ARGUMENT, "out is null");
+    sizes_of(*params, *out, nullptr);
+    return BMFR_OK;
+}
+
+int32_t bmfr_stream_open(const bmfr_params *params, bmfr_stream *out) {
+    int32_t st = validate(params);
+    if (st) return st;
+    if (!out) return fail(BMFR_ERR_INVALID_ARGUMENT, "out is null");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0)
+        return fail(BMFR_ERR_NO_DEVICE, "no HIP device visible");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(BMFR_ERR_NO_DEVICE, "device %d is %s; this library is built for gfx950", dev,
+                    prop.gcnArchName);
+    bmfr_stream s = new (std::nothrow) bmfr_stream_s;
+    if (!s) return fail(BMFR_ERR_INVALID_ARGUMENT, "host allocation failed");
+    s->p = *params;
+    s->device = dev;
+    sizes_of(s->p, s->sz, s->slot_bytes);
+    hipError_t e = hipMalloc((void **)&s->base, (size_t)s->sz.device_bytes);
+    if (e != hipSuccess) {
+        delete s;
+        return fail(BMFR_ERR_OUT_OF_BUDGET, "hipMalloc of %lld bytes failed: %s",
+                    (long long)s->sz.device_bytes, hipGetErrorString(e));
+    }
+    size_t off = 0;
+    for (int i = 0; i < S_COUNT; ++i) {
+        s->slot[i] = s->base + off;
+        off += (s->slot_bytes[i] + 255) & ~(size_t)255;
+    }
+    e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMemsetAsync(s->base, 0, (size_t)s->sz.device_bytes, s->stream);
+    for (int i = 0; i < kStages && e == hipSuccess; ++i) {
+        e = hipEventCreate(&s->ev_start[i]);
+        if (e == hipSuccess) e = hipEventCreate(&s->ev_end[i]);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
+    if (e != hipSuccess) {
+        bmfr_stream_close(s);
+        return fail(BMFR_ERR_HIP, "stream set-up failed: %s", hipGetErrorString(e));
+    }
+    *out = s;
+    return BMFR_OK;
+}
+
+int32_t bmfr_stream_close(bmfr_stream s) {
+    if (!s) return BMFR_OK;
+    hipSetDevice(s->device);
+    if (s->stream) hipStreamSynchronize(s->stream);
+    for (int i = 0; i < kStages; ++i) {
+        if (s->ev_start[i]) hipEventDestroy(s->ev_start[i]);
+        if (s->ev_end[i]) hipEventDestroy(s->ev_end[i]);
+    }
+    if (s->stream) hipStreamDestroy(s->stream);
+    if (s->base) hipFree(s->base);
+    delete s;
+    return BMFR_OK;
+}
+
+int32_t bmfr_stream_sizes(bmfr_stream s, bmfr_sizes *out) {
+    if (int32_t st = check_stream(s)) return st;
+    if (!out) return fail(BMFR_ERR_INVALID_ARGUMENT, "out is null");
+    *out = s->sz;
+    return BMFR_OK;
+}
+
+int32_t bmfr_stream_reset(bmfr_stream s) {
+    if (int32_t st = check_stream(s)) return st;
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipMemsetAsync(s->base, 0, (size_t)s->sz.device_bytes, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    s->swapped = false;
+    s->last_result = -1;
+    return BMFR_OK;
+}
+
+int32_t bmfr_set_fitter_config(bmfr_stream s, int32_t rows_per_lane, int32_t blocks_per_group) {
+    if (int32_t st = check_stream(s)) return st;
+    const int key = rows_per_lane * 10 + blocks_per_group;
+    if (key != 41 && key != 42 && key != 81 && key != 82 && key != 161 && key != 162 && key != 164)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "unsupported fitter config rows_per_lane=%d blocks_per_group=%d",
+                    rows_per_lane, blocks_per_group);
+    s->fitter.rows_per_lane = rows_per_lane;
+    s->fitter.blocks_per_group = blocks_per_group;
+    return BMFR_OK;
+}
+
+int32_t bmfr_upload_frame(bmfr_stream s, const float *albedo, const float *normals,
+                          const float *positions, const float *noisy) {
+    if (int32_t st = check_stream(s)) return st;
+    if (!albedo || !normals || !positions || !noisy)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "null input array");
+    HIP_TRY(hipSetDevice(s->device));
+    const size_t n = (size_t)s->sz.pixels * 3;
+    HIP_TRY(hipMemcpyAsync(s->slot[S_ALBEDO], albedo, n * 4, hipMemcpyHostToDevice, s->stream));
+    const float *src[3] = {normals, positions, noisy};
+    const int dst[3] = {s->cur(S_NORMALS_A), s->cur(S_POSITIONS_A), S_NOISY_IN};
+    if (s->p.storage == BMFR_STORAGE_F32) {
+        for (int i = 0; i < 3; ++i)
+            HIP_TRY(hipMemcpyAsync(s->slot[dst[i]], src[i], n * 4, hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+    } else {
+        std::vector<__half> h(n);
+        for (int i = 0; i < 3; ++i) {
+            for (size_t k = 0; k < n; ++k) h[k] = __float2half_rn(src[i][k]);
+            HIP_TRY(hipMemcpyAsync(s->slot[dst[i]], h.data(), n * 2, hipMemcpyHostToDevice, s->stream));
+            HIP_TRY(hipStreamSynchronize(s->stream));
+        }
+    }
+    return BMFR_OK;
+}
+
+int32_t bmfr_accumulate_noisy_data(bmfr_stream s, int32_t frame, const float *matrix,
+                                   const float *pixel_offset) {
+    if (int32_t st = check_stream(s)) return st;
+    if (int32_t st = check_frame(frame)) return st;
+    if (!matrix || !pixel_offset) return fail(BMFR_ERR_INVALID_ARGUMENT, "null matrix or offset");
+    bmfr::StageArgs a = s->args(frame);
+    std::memcpy(a.matrix, matrix, sizeof(a.matrix));
+    a.pixel_offset_x = pixel_offset[0];
+    a.pixel_offset_y = pixel_offset[1];
+    const bmfr::StageBuffers b = s->buffers();
+    return timed_stage(s, 0, [&] { return bmfr::launch_accumulate_noisy_data(a, b, s->stream); });
+}
+
+int32_t bmfr_fitter(bmfr_stream s, int32_t frame) {
+    if (int32_t st = check_stream(s)) return st;
+    if (int32_t st = check_frame(frame)) return st;
+    const bmfr::StageArgs a = s->args(frame);
+    const bmfr::StageBuffers b = s->buffers();
+    return timed_stage(s, 1, [&] { return bmfr::launch_fitter(a, b, s->fitter, s->p.qr_mode, s->stream); });
+}
+
+int32_t bmfr_weighted_sum(bmfr_stream s, int32_t frame) {
+    if (int32_t st = check_stream(s)) return st;
+    if (int32_t st = check_frame(frame)) return st;
+    const bmfr::StageArgs a = s->args(frame);
+    const bmfr::StageBuffers b = s->buffers();
+    return timed_stage(s, 2, [&] { return bmfr::launch_weighted_sum(a, b, s->stream); });
+}
+
+int32_t bmfr_accumulate_filtered_data(bmfr_stream s, int32_t frame) {
+    if (int32_t st = check_stream(s)) return st;
+    if (int32_t st = check_frame(frame)) return st;
+    const bmfr::StageArgs a = s->args(frame);
+    const bmfr::StageBuffers b = s->buffers();
+    return timed_stage(s, 3, [&] { return bmfr::launch_accumulate_filtered_data(a, b, s->stream); });
+}
+
+int32_t bmfr_taa(bmfr_stream s, int32_t frame) {
+    if (int32_t st = check_stream(s)) return st;
+    if (int32_t st = check_frame(frame)) return st;
+    const bmfr::StageArgs a = s->args(frame);
+    const bmfr::StageBuffers b = s->buffers();
+    int32_t st = timed_stage(s, 4, [&] { return bmfr::launch_taa(a, b, s->stream); });
+    if (st == BMFR_OK) s->last_result = s->cur(S_RESULT_A);
+    return st;
+}
+
+int32_t bmfr_end_frame(bmfr_stream s) {
+    if (int32_t st = check_stream(s)) return st;
+    s->swapped = !s->swapped;  // opencl/bmfr.cpp:482-484
+    return BMFR_OK;
+}
+
+int32_t bmfr_run_frame(bmfr_stream s, int32_t frame, const float *matrix, const float *pixel_offset) {
+    int32_t st = bmfr_accumulate_noisy_data(s, frame, matrix, pixel_offset);
+    if (!st) st = bmfr_fitter(s, frame);
+    if (!st) st = bmfr_weighted_sum(s, frame);
+    if (!st) st = bmfr_accumulate_filtered_data(s, frame);
+    if (!st) st = bmfr_taa(s, frame);
+    if (!st) st = bmfr_end_frame(s);
+    return st;
+}
+
+int32_t bmfr_synchronize(bmfr_stream s) {
+    if (int32_t st = check_stream(s)) return st;
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return BMFR_OK;
+}
+
+int32_t bmfr_download_result(bmfr_stream s, float *out) {
+    if (int32_t st = check_stream(s)) return st;
+    if (!out) return fail(BMFR_ERR_INVALID_ARGUMENT, "out is null");
+    if (s->last_result < 0) return fail(BMFR_ERR_STATE, "no frame has completed taa yet");
+    HIP_TRY(hipSetDevice(s->device));
+    const size_t n = (size_t)s->sz.pixels * 3;
+    if (s->p.storage == BMFR_STORAGE_F32) {
+        HIP_TRY(hipMemcpyAsync(out, s->slot[s->last_result], n * 4, hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+    } else {
+        std::vector<__half> h(n);
+        HIP_TRY(hipMemcpyAsync(h.data(), s->slot[s->last_result], n * 2, hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        for (size_t k = 0; k < n; ++k) out[k] = __half2float(h[k]);
+    }
+    return BMFR_OK;
+}
+
+int32_t bmfr_buffer_bytes(bmfr_stream s, int32_t buffer, size_t *bytes) {
+    if (int32_t st = check_stream(s)) return st;
+    const int sl = s->resolve(buffer);
+    if (sl < 0) return fail(BMFR_ERR_INVALID_ARGUMENT, "unknown buffer %d", buffer);
+    if (!bytes) return fail(BMFR_ERR_INVALID_ARGUMENT, "bytes is null");
+    *bytes = s->slot_bytes[sl];
+    return BMFR_OK;
+}
+
+int32_t bmfr_read_buffer(bmfr_stream s, int32_t buffer, void *host, size_t bytes) {
+    if (int32_t st = check_stream(s)) return st;
+    const int sl = s->resolve(buffer);
+    if (sl < 0) return fail(BMFR_ERR_INVALID_ARGUMENT, "unknown buffer %d", buffer);
+    if (!host || bytes != s->slot_bytes[sl])
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "buffer %d holds %zu bytes, %zu requested", buffer,
+                    s->slot_bytes[sl], bytes);
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipMemcpyAsync(host, s->slot[sl], bytes, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return BMFR_OK;
+}
+
+int32_t bmfr_write_buffer(bmfr_stream s, int32_t buffer, const void *host, size_t bytes) {
+    if (int32_t st = check_stream(s)) return st;
+    const int sl = s->resolve(buffer);
+    if (sl < 0) return fail(BMFR_ERR_INVALID_ARGUMENT, "unknown buffer %d", buffer);
+    if (!host || bytes != s->slot_bytes[sl])
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "buffer %d holds %zu bytes, %zu given", buffer,
+                    s->slot_bytes[sl], bytes);
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipMemcpyAsync(s->slot[sl], host, bytes, hipMemcpyHostToDevice, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return BMFR_OK;
+}
+
+int32_t bmfr_buffer_device_ptr(bmfr_stream s, int32_t buffer, void **dptr) {
+    if (int32_t st = check_stream(s)) return st;
+    const int sl = s->resolve(buffer);
+    if (sl < 0) return fail(BMFR_ERR_INVALID_ARGUMENT, "unknown buffer %d", buffer);
+    if (!dptr) return fail(BMFR_ERR_INVALID_ARGUMENT, "dptr is null");
+    *dptr = s->slot[sl];
+    return BMFR_OK;
+}
+
+int32_t bmfr_set_timing(bmfr_stream s, int32_t enable) {
+    if (int32_t st = check_stream(s)) return st;
+    s->timing = enable != 0;
+    for (int i = 0; i < kStages; ++i) s->recorded[i] = false;
+    return BMFR_OK;
+}
+
+int32_t bmfr_stage_times(bmfr_stream s, float *ms6) {
+    if (int32_t st = check_stream(s)) return st;
+    if (!ms6) return fail(BMFR_ERR_INVALID_ARGUMENT, "ms6 is null");
+    for (int i = 0; i < kStages; ++i)
+        if (!s->recorded[i]) return fail(BMFR_ERR_STATE, "stage %d has no timed launch", i);
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipEventSynchronize(s->ev_end[kStages - 1]));
+    for (int i = 0; i < kStages; ++i) HIP_TRY(hipEventElapsedTime(&ms6[i], s->ev_start[i], s->ev_end[i]));
+    HIP_TRY(hipEventElapsedTime(&ms6[5], s->ev_start[0], s->ev_end[kStages - 1]));
+    return BMFR_OK;
+}
+
+// ---- scene -------------------------------------------------------------------
+int32_t bmfr_scene_create(uint64_t seed, int32_t spheres, float firefly, int32_t width, int32_t height,
+                          int32_t static_camera, bmfr_scene *out) {
+    if (!out) return fail(BMFR_ERR_INVALID_ARGUMENT, "out is null");
+    if (spheres < 0 || spheres > bmfr::kMaxSpheres)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "spheres %d outside 0..%d", spheres, bmfr::kMaxSpheres);
+    if (width <= 0 || height <= 0) return fail(BMFR_ERR_INVALID_ARGUMENT, "bad scene size");
+    bmfr_scene sc = new (std::nothrow) bmfr_scene_s;
+    if (!sc) return fail(BMFR_ERR_INVALID_ARGUMENT, "host allocation failed");
+    bmfr::scene_init(sc->data, seed, spheres, firefly, width, height, static_camera);
+    *out = sc;
+    return BMFR_OK;
+}
+
+int32_t bmfr_scene_destroy(bmfr_scene sc) {
+    if (!sc) return BMFR_OK;
+    if (sc->device_copy) hipFree(sc->device_copy);
+    delete sc;
+    return BMFR_OK;
+}
+
+int32_t bmfr_scene_camera(bmfr_scene sc, int32_t frame, float *vp, float *offset) {
+    if (!sc || !vp || !offset) return fail(BMFR_ERR_INVALID_ARGUMENT, "null argument");
+    bmfr::CameraFrame c;
+    bmfr::scene_camera(sc->data, frame, c);
+    std::memcpy(vp, c.vp, sizeof(c.vp));
+    offset[0] = c.offset[0];
+    offset[1] = c.offset[1];
+    return BMFR_OK;
+}
+
+int32_t bmfr_scene_render_host(bmfr_scene sc, int32_t frame, float *albedo, float *normals,
+                               float *positions, float *noisy) {
+    if (!sc) return fail(BMFR_ERR_INVALID_ARGUMENT, "scene is null");
+    bmfr::CameraFrame c;
+    bmfr::scene_camera(sc->data, frame, c);
+    const int W = sc->data.width, H = sc->data.height;
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; ++t) {
+        th.emplace_back([&, t] {
+            float px[12];
+            for (int y = (int)t; y < H; y += (int)nt)
+                for (int x = 0; x < W; ++x) {
+                    bmfr::scene_pixel(sc->data, c, frame, x, y, px);
+                    const size_t lp = (size_t)y * W + x;
+                    for (int k = 0; k < 3; ++k) {
+                        if (albedo) albedo[lp * 3 + k] = px[k];
+                        if (normals) normals[lp * 3 + k] = px[3 + k];
+                        if (positions) positions[lp * 3 + k] = px[6 + k];
+                        if (noisy) noisy[lp * 3 + k] = px[9 + k];
+                    }
+                }
+        });
+    }
+    for (auto &t : th) t.join();
+    return BMFR_OK;
+}
+
+int32_t bmfr_scene_render_to_stream(bmfr_scene sc, int32_t frame, bmfr_stream s) {
+    if (!sc) return fail(BMFR_ERR_INVALID_ARGUMENT, "scene is null");
+    if (int32_t st = check_stream(s)) return st;
+    if (sc->data.width != s->p.width || sc->data.height != s->p.height)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "scene %dx%d does not match stream %dx%d", sc->data.width,
+                    sc->data.height, s->p.width, s->p.height);
+    HIP_TRY(hipSetDevice(s->device));
+    if (!sc->device_copy) {
+        HIP_TRY(hipMalloc((void **)&sc->device_copy, sizeof(bmfr::SceneData)));
+        HIP_TRY(hipMemcpy(sc->device_copy, &sc->data, sizeof(bmfr::SceneData), hipMemcpyHostToDevice));
+    }
+    bmfr::CameraFrame c;
+    bmfr::scene_camera(sc->data, frame, c);
+    hipError_t e = bmfr::launch_render(sc->device_copy, sc->data.width, sc->data.height, c, frame,
+                                       s->p.storage == BMFR_STORAGE_F16, s->slot[S_ALBEDO],
+                                       s->slot[s->cur(S_NORMALS_A)], s->slot[s->cur(S_POSITIONS_A)],
+                                       s->slot[S_NOISY_IN], s->stream);
+    if (e != hipSuccess) return fail(BMFR_ERR_HIP, "scene render launch failed: %s", hipGetErrorString(e));
+    return BMFR_OK;
+}
+
+}  // extern "C"

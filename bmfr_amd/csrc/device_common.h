@@ -1,0 +1,121 @@
+// Device-side helpers shared by the gfx950 BMFR kernels.
+//
+// The whole library is compiled with -ffp-contract=off: every float
+// operation in the stage kernels is rounded separately, in the order of the
+// reference kernels (opencl/bmfr.cl), so that integer decisions derived from
+// float math (reprojection floor, accept bits, spp rounding) are
+// reproducible bit for bit.  Kernels that want fused multiply-adds ask for
+// them explicitly with __builtin_fmaf.
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bmfr {
+
+constexpr int kBlock = 32;               // BLOCK_EDGE_LENGTH
+constexpr int kBlockPixels = 1024;       // BLOCK_PIXELS
+constexpr int kBufferCount = 13;         // BUFFER_COUNT: 10 features + RGB
+constexpr int kFeatures = 10;
+constexpr int kNotScaled = 4;            // FEATURES_NOT_SCALED: 1, n.xyz
+constexpr int kScaled = 6;               // FEATURES_SCALED: p.xyz, p.xyz^2
+constexpr int kREdge = 11;               // R_EDGE
+
+// Per-frame jitter of the block grid, opencl/bmfr.cl:267-285.  A table of
+// constants that has to be identical for the results to match.
+static __constant__ int2 kBlockOffsets[16] = {
+    {-14, -14}, {4, -6},  {-8, 14},  {8, 0},   {-10, -8}, {2, 12},
+    {12, -12},  {-10, 0}, {12, 14},  {-8, -16}, {6, 6},   {-2, -2},
+    {6, -14},   {-16, 12}, {14, -4}, {-6, 4}};
+
+__device__ __forceinline__ int2 block_offset(int frame) {
+    int k = frame % 16;
+    if (k < 0) k += 16;
+    return kBlockOffsets[k];
+}
+
+// opencl/bmfr.cl:209-216; valid when |index| is less than one size out.
+__device__ __forceinline__ int mirror(int index, int size) {
+    if (index < 0) return -index - 1;
+    if (index >= size) return 2 * size - index - 1;
+    return index;
+}
+
+// Storage-type access for G-buffers and histories (float or half).
+template <typename T> struct Store;
+template <> struct Store<float> {
+    __device__ __forceinline__ static float ld(const float *p, int64_t i) { return p[i]; }
+    __device__ __forceinline__ static void st(float *p, int64_t i, float v) { p[i] = v; }
+};
+template <> struct Store<__half> {
+    __device__ __forceinline__ static float ld(const __half *p, int64_t i) {
+        return __half2float(p[i]);
+    }
+    __device__ __forceinline__ static void st(__half *p, int64_t i, float v) {
+        p[i] = __float2half_rn(v);
+    }
+};
+
+template <typename T>
+__device__ __forceinline__ float3 load3(const T *p, int64_t pixel) {
+    return make_float3(Store<T>::ld(p, pixel * 3 + 0), Store<T>::ld(p, pixel * 3 + 1),
+                       Store<T>::ld(p, pixel * 3 + 2));
+}
+template <typename T>
+__device__ __forceinline__ void store3(T *p, int64_t pixel, float3 v) {
+    Store<T>::st(p, pixel * 3 + 0, v.x);
+    Store<T>::st(p, pixel * 3 + 1, v.y);
+    Store<T>::st(p, pixel * 3 + 2, v.z);
+}
+
+// convert_int_rtn (opencl/bmfr.cl:356, 794, 879) with an explicitly
+// saturating conversion, NaN -> 0 (what v_cvt_i32_f32 does); written out so
+// that no out-of-range float->int conversion is left undefined.
+__device__ __forceinline__ int cvt_rtn(float f) {
+    if (f != f) return 0;
+    const float fl = floorf(f);
+    if (fl >= 2147483648.0f) return 2147483647;
+    if (fl < -2147483648.0f) return -2147483647 - 1;
+    return (int)fl;
+}
+
+// Integer sums of reprojected coordinates wrap (two's complement).
+__device__ __forceinline__ int wrap_add(int a, int b) {
+    return (int)((unsigned)a + (unsigned)b);
+}
+__device__ __forceinline__ int wrap_mad(int a, int b, int c) {
+    return (int)((unsigned)a * (unsigned)b + (unsigned)c);
+}
+
+// opencl/bmfr.cl:162-171 (hash) and 173-182 (feature noise, evaluated in
+// double because NOISE_AMOUNT is a double literal).  The hash argument
+// id + sub_vector*256 equals the row index within the block.
+__device__ __forceinline__ uint32_t hash32(uint32_t a) {
+    a = (a + 0x7ed55d16u) + (a << 12);
+    a = (a ^ 0xc761c23cu) ^ (a >> 19);
+    a = (a + 0x165667b1u) + (a << 5);
+    a = (a + 0xd3a2646cu) ^ (a << 9);
+    a = (a + 0xfd7046c5u) + (a << 3);
+    a = (a ^ 0xb55a4f09u) ^ (a >> 16);
+    return a;
+}
+__device__ __forceinline__ float add_noise(float value, int row, int feature, int frame,
+                                           double noise_amount) {
+    const uint32_t arg = (uint32_t)row + (uint32_t)feature * kBlockPixels +
+                         (uint32_t)frame * (uint32_t)(kBufferCount * kBlockPixels);
+    const float r = (float)hash32(arg) / 4294967296.0f - 0.5f;
+    return (float)((double)value + noise_amount * 2.0 * (double)r);
+}
+
+// opencl/bmfr.cl:200-205
+__device__ __forceinline__ float scale_feature(float v, float mn, float mx) {
+    if (fabsf(mx - mn) > 1.0f) return (v - mn) / (mx - mn);
+    return v - mn;
+}
+
+__device__ __forceinline__ float round_half(float v) {
+    return __half2float(__float2half_rn(v));
+}
+
+}  // namespace bmfr

@@ -1,0 +1,370 @@
+// Stage 2: fitter, opencl/bmfr.cl:487-700, for gfx950.
+//
+// Method kept from the reference: per-block min/max scaling of the six
+// world-position features, feature noise on the first pass, Householder QR
+// of the 1024x13 block matrix with the reference's sign choice
+// (u_c = x_c - ||x||), and the reference's back-substitution sequence.
+//
+// Layout: the whole 1024x13 block matrix lives in VGPRs.  A block is
+// handled by W = 16/RPL waves; each lane owns RPL rows (13*RPL registers),
+// so the QR never touches memory after the initial load.  Column sums are
+// 64-lane DPP reductions (quad_perm, row_half_mirror, row_mirror,
+// row_bcast15, row_bcast31, then v_readlane of lane 63); with W > 1 the W
+// wave partials meet in a few LDS words behind one barrier.  All trailing
+// dot products of a column step are reduced together, so a block needs two
+// reduction rounds per pivot column.
+//
+// Columns 10-12 (the RGB right-hand sides) are never pivots: the reference's
+// steps col = 10..12 only reflect rows >= 10 of the colour columns, which
+// the back substitution never reads (it uses rows 0-9 of column 10 of R),
+// so skipping them leaves mins_maxs and weights unchanged.  The factorised
+// matrix is not written back to tmp_data: nothing reads it after the fitter
+// (opencl/bmfr.cpp:449-476).
+#include "device_common.h"
+#include "kernels.h"
+
+namespace bmfr {
+
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ float dpp(float old, float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), CTRL,
+                                                      ROW_MASK, 0xf, false));
+}
+
+__device__ __forceinline__ float readlane(float v, int lane) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+struct OpAdd {
+    static constexpr float identity = 0.f;
+    __device__ __forceinline__ static float f(float a, float b) { return a + b; }
+};
+struct OpMin {
+    static constexpr float identity = __builtin_huge_valf();
+    __device__ __forceinline__ static float f(float a, float b) { return fminf(a, b); }
+};
+
+// 64-lane reduction; the result is wave-uniform.
+template <typename Op>
+__device__ __forceinline__ float wave_reduce(float v) {
+    v = Op::f(v, dpp<0xB1>(v, v));    // quad_perm [1,0,3,2]
+    v = Op::f(v, dpp<0x4E>(v, v));    // quad_perm [2,3,0,1]
+    v = Op::f(v, dpp<0x141>(v, v));   // row_half_mirror
+    v = Op::f(v, dpp<0x140>(v, v));   // row_mirror
+    v = Op::f(v, dpp<0x142, 0xA>(Op::identity, v));  // row_bcast15 -> rows 1, 3
+    v = Op::f(v, dpp<0x143, 0xC>(Op::identity, v));  // row_bcast31 -> rows 2, 3
+    return readlane(v, 63);
+}
+
+// Reduce NV values over the W waves of one block.  `red` points at this
+// block's [W][16] slot of the round's LDS buffer.
+template <int W, int NV, typename Op>
+__device__ __forceinline__ void block_reduce(float (&v)[NV], float *red, int wave, int lane) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = wave_reduce<Op>(v[i]);
+    if constexpr (W > 1) {
+        if (lane == 0) {
+#pragma unroll
+            for (int i = 0; i < NV; ++i) red[wave * 16 + i] = v[i];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            float acc = red[i];
+#pragma unroll
+            for (int w = 1; w < W; ++w) acc = Op::f(acc, red[w * 16 + i]);
+            v[i] = acc;
+        }
+    }
+}
+
+template <typename TT, int V>
+__device__ __forceinline__ void load_rows(const TT *p, float *out) {
+    if constexpr (sizeof(TT) == 4) {
+        static_assert(V == 4, "fp32 rows load as float4");
+        const float4 x = *reinterpret_cast<const float4 *>(p);
+        out[0] = x.x; out[1] = x.y; out[2] = x.z; out[3] = x.w;
+    } else if constexpr (V == 4) {
+        const uint2 x = *reinterpret_cast<const uint2 *>(p);
+        const __half2 h0 = *reinterpret_cast<const __half2 *>(&x.x);
+        const __half2 h1 = *reinterpret_cast<const __half2 *>(&x.y);
+        out[0] = __low2float(h0); out[1] = __high2float(h0);
+        out[2] = __low2float(h1); out[3] = __high2float(h1);
+    } else {
+        static_assert(V == 8, "fp16 rows load as 4 or 8 halves");
+        const uint4 x = *reinterpret_cast<const uint4 *>(p);
+        const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const __half2 h = *reinterpret_cast<const __half2 *>(&w[i]);
+            out[2 * i] = __low2float(h);
+            out[2 * i + 1] = __high2float(h);
+        }
+    }
+}
+
+struct FitterParams {
+    const void *tmp;
+    float *mins_maxs;
+    float *weights;
+    int32_t blocks;
+    int32_t frame;
+    double noise_amount;
+};
+
+// RPL rows per lane, BPG blocks per work-group, TT tmp element type,
+// HALF_ROUND: round every stored element to binary16 like the reference's
+// USE_HALF_PRECISION_IN_TMP_DATA=1 (bmfr.cl:255-258, 540, 651).
+template <int RPL, int BPG, typename TT, bool HALF_ROUND>
+__global__ __launch_bounds__(64 * (16 / RPL) * BPG) void k_fitter(const FitterParams p) {
+    constexpr int W = 16 / RPL;            // waves per block
+    constexpr int T = 64 * W;              // lanes per block
+    constexpr int V = (RPL < (int)(16 / sizeof(TT))) ? RPL : (int)(16 / sizeof(TT));
+    constexpr int Q = RPL / V;             // vector loads per feature
+    __shared__ float red[2][BPG][W][16];
+
+    const int tid = threadIdx.x;
+    const int sub = tid / T;
+    const int t = tid - sub * T;
+    const int lane = tid & 63;
+    const int wave = t >> 6;
+    const int blk_raw = blockIdx.x * BPG + sub;
+    const bool active = blk_raw < p.blocks;
+    const int blk = active ? blk_raw : p.blocks - 1;
+    const TT *src = reinterpret_cast<const TT *>(p.tmp) + (int64_t)blk * (kBufferCount * kBlockPixels);
+
+    // Row held in element k of this lane: rows t*V..t*V+V-1 of each T*V strip.
+    auto row_of = [&](int k) { return (k / V) * (T * V) + t * V + (k % V); };
+
+    float a[kBufferCount][RPL];
+#pragma unroll
+    for (int f = 0; f < kBufferCount; ++f)
+#pragma unroll
+        for (int q = 0; q < Q; ++q) load_rows<TT, V>(src + f * kBlockPixels + q * (T * V) + t * V, &a[f][q * V]);
+
+    int round = 0;
+    // ---- scaling, bmfr.cl:511-542 (max taken as -min(-x): exact) ----
+    float bmin[kScaled], bmax[kScaled];
+    {
+        float v[2 * kScaled];
+#pragma unroll
+        for (int i = 0; i < kScaled; ++i) {
+            float mn = INFINITY, nmx = INFINITY;
+#pragma unroll
+            for (int k = 0; k < RPL; ++k) {
+                mn = fminf(mn, a[kNotScaled + i][k]);
+                nmx = fminf(nmx, -a[kNotScaled + i][k]);
+            }
+            v[i] = mn;
+            v[kScaled + i] = nmx;
+        }
+        block_reduce<W, 2 * kScaled, OpMin>(v, &red[round & 1][sub][0][0], wave, lane);
+        ++round;
+#pragma unroll
+        for (int i = 0; i < kScaled; ++i) {
+            bmin[i] = v[i];
+            bmax[i] = -v[kScaled + i];
+        }
+    }
+    if (active && t == 0) {
+        float *mm = p.mins_maxs + (int64_t)blk * (kScaled * 2);
+#pragma unroll
+        for (int i = 0; i < kScaled; ++i) {
+            mm[2 * i + 0] = bmin[i];
+            mm[2 * i + 1] = bmax[i];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < kScaled; ++i)
+#pragma unroll
+        for (int k = 0; k < RPL; ++k) {
+            float s = scale_feature(a[kNotScaled + i][k], bmin[i], bmax[i]);
+            a[kNotScaled + i][k] = HALF_ROUND ? round_half(s) : s;
+        }
+    // ---- feature noise of the first pass, bmfr.cl:623-627 ----
+#pragma unroll
+    for (int f = 1; f < kFeatures; ++f)
+#pragma unroll
+        for (int k = 0; k < RPL; ++k) a[f][k] = add_noise(a[f][k], row_of(k), f, p.frame, p.noise_amount);
+
+    // ---- Householder QR over the 10 feature columns, bmfr.cl:549-656 ----
+    float norms[kFeatures];
+#pragma unroll
+    for (int col = 0; col < kFeatures; ++col) {
+        // Pivot row `col` sits in lane col/V (wave 0), element col%V.
+        constexpr int dummy = 0;
+        (void)dummy;
+        float r1[2];
+        {
+            float s = 0.f;
+#pragma unroll
+            for (int k = 0; k < RPL; ++k) {
+                const float x = a[col][k];
+                const float sq = x * x;
+                s = s + (row_of(k) > col ? sq : 0.f);
+            }
+            r1[0] = s;
+        }
+        float xc = readlane(a[col][col % V], col / V);
+        if constexpr (W > 1) {
+            r1[0] = wave_reduce<OpAdd>(r1[0]);
+            float *rd = &red[round & 1][sub][0][0];
+            if (lane == 0) {
+                rd[wave * 16 + 0] = r1[0];
+                if (wave == 0) rd[1] = xc;
+            }
+            __syncthreads();
+            float acc = rd[0];
+#pragma unroll
+            for (int w = 1; w < W; ++w) acc = acc + rd[w * 16];
+            r1[0] = acc;
+            xc = rd[1];
+        } else {
+            r1[0] = wave_reduce<OpAdd>(r1[0]);
+        }
+        ++round;
+        const float sigma = r1[0];
+        const float norm = sqrtf(sigma + xc * xc);
+        const float uc = xc - norm;
+        const float uu = sigma + uc * uc;
+        norms[col] = norm;
+
+        float u[RPL];
+#pragma unroll
+        for (int k = 0; k < RPL; ++k) {
+            const int row = row_of(k);
+            u[k] = row > col ? a[col][k] : (row == col ? uc : 0.f);
+        }
+        constexpr int kMaxTrail = kBufferCount - 1;
+        float d[kMaxTrail];
+#pragma unroll
+        for (int j = 0; j < kMaxTrail; ++j) d[j] = 0.f;
+#pragma unroll
+        for (int j = col + 1; j < kBufferCount; ++j) {
+            float s = 0.f;
+#pragma unroll
+            for (int k = 0; k < RPL; ++k) s = __builtin_fmaf(u[k], a[j][k], s);
+            d[j - col - 1] = s;
+        }
+        // reduce only the live trailing columns
+        {
+            float *rd = &red[round & 1][sub][0][0];
+            const int nv = kBufferCount - 1 - col;
+#pragma unroll
+            for (int i = 0; i < kMaxTrail; ++i)
+                if (i < nv) d[i] = wave_reduce<OpAdd>(d[i]);
+            if constexpr (W > 1) {
+                if (lane == 0) {
+#pragma unroll
+                    for (int i = 0; i < kMaxTrail; ++i)
+                        if (i < nv) rd[wave * 16 + i] = d[i];
+                }
+                __syncthreads();
+#pragma unroll
+                for (int i = 0; i < kMaxTrail; ++i) {
+                    if (i < nv) {
+                        float acc = rd[i];
+#pragma unroll
+                        for (int w = 1; w < W; ++w) acc = acc + rd[w * 16 + i];
+                        d[i] = acc;
+                    }
+                }
+            }
+            ++round;
+        }
+#pragma unroll
+        for (int j = col + 1; j < kBufferCount; ++j) {
+            const float fct = (2.f * d[j - col - 1]) / uu;
+#pragma unroll
+            for (int k = 0; k < RPL; ++k) {
+                const float v = a[j][k] - u[k] * fct;
+                a[j][k] = HALF_ROUND ? round_half(v) : v;
+            }
+        }
+    }
+
+    // ---- back substitution, bmfr.cl:658-699, in wave 0 ----
+    // Lane L = 3*c + ch holds column c (0..10) of R for colour channel ch:
+    // R[r][c] = a[c] at row r (r < c), R[c][c] = norms[c]; column 10 is
+    // Q^T b = rows 0..9 of colour column 10+ch.
+    if (wave == 0) {
+        const int cl = lane / 3;
+        const int ch = lane - 3 * cl;
+        float rc[kFeatures];
+#pragma unroll
+        for (int r = 0; r < kFeatures; ++r) rc[r] = 0.f;
+#pragma unroll
+        for (int c = 0; c < kFeatures; ++c) {
+#pragma unroll
+            for (int r = 0; r < kFeatures; ++r) {
+                float val;
+                if (r < c) val = readlane(a[c][r % V], r / V);
+                else if (r == c) val = norms[c];
+                else val = 0.f;
+                if (cl == c) rc[r] = val;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kFeatures; ++r) {
+            const float v0 = readlane(a[10][r % V], r / V);
+            const float v1 = readlane(a[11][r % V], r / V);
+            const float v2 = readlane(a[12][r % V], r / V);
+            if (cl == 10) rc[r] = ch == 0 ? v0 : (ch == 1 ? v1 : v2);
+        }
+#pragma unroll
+        for (int i = kREdge - 2; i >= 0; --i) {
+            const float divider = __shfl(rc[i], 3 * i + ch);
+            if (cl >= i) rc[i] = rc[i] / divider;
+            float acc = __shfl(rc[i], 30 + ch);
+#pragma unroll
+            for (int j = i + 1; j < kREdge - 1; ++j) acc = acc - __shfl(rc[i], 3 * j + ch);
+            if (cl == 10) rc[i] = acc;
+            const float wi = __shfl(rc[i], 30 + ch);
+            if (cl == i) {
+#pragma unroll
+                for (int r = 0; r <= i; ++r) rc[r] = rc[r] * wi;
+            }
+        }
+        if (active && cl == 10 && ch < 3) {
+            float *w = p.weights + (int64_t)blk * (kFeatures * 3);
+#pragma unroll
+            for (int r = 0; r < kFeatures; ++r) w[r * 3 + ch] = rc[r];
+        }
+    }
+}
+
+template <int RPL, int BPG>
+static hipError_t launch_rpl(const FitterParams &fp, bool tmp_f16, hipStream_t s) {
+    constexpr int threads = 64 * (16 / RPL) * BPG;
+    const dim3 grid((fp.blocks + BPG - 1) / BPG);
+    if (tmp_f16)
+        hipLaunchKernelGGL((k_fitter<RPL, BPG, __half, true>), grid, dim3(threads), 0, s, fp);
+    else
+        hipLaunchKernelGGL((k_fitter<RPL, BPG, float, false>), grid, dim3(threads), 0, s, fp);
+    return hipGetLastError();
+}
+
+hipError_t launch_fitter(const StageArgs &a, const StageBuffers &b, const FitterConfig &cfg,
+                         int32_t qr_mode, hipStream_t s) {
+    (void)qr_mode;
+    FitterParams fp;
+    fp.tmp = b.tmp_data;
+    fp.mins_maxs = (float *)b.mins_maxs;
+    fp.weights = (float *)b.weights;
+    fp.blocks = a.blocks;
+    fp.frame = a.frame;
+    fp.noise_amount = a.noise_amount;
+    const bool h = a.tmp_f16 != 0;
+    switch (cfg.rows_per_lane * 10 + cfg.blocks_per_group) {
+        case 41: return launch_rpl<4, 1>(fp, h, s);
+        case 42: return launch_rpl<4, 2>(fp, h, s);
+        case 81: return launch_rpl<8, 1>(fp, h, s);
+        case 82: return launch_rpl<8, 2>(fp, h, s);
+        case 161: return launch_rpl<16, 1>(fp, h, s);
+        case 162: return launch_rpl<16, 2>(fp, h, s);
+        case 164: return launch_rpl<16, 4>(fp, h, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace bmfr

@@ -1,0 +1,62 @@
+// Interface between the C-ABI host layer and the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bmfr {
+
+// Per-launch scalars of one stream (the reference's -D values plus the
+// per-frame kernel arguments of opencl/bmfr.cpp:429-476).
+struct StageArgs {
+    int32_t width, height;
+    int32_t margin_width, margin_height;
+    int32_t blocks_x, blocks;
+    float position_limit_squared, normal_limit_squared;
+    float blend_alpha, second_blend_alpha, taa_blend_alpha;
+    double noise_amount;
+    int32_t frame;
+    int32_t storage_f16;   // G-buffers and histories stored as half
+    int32_t tmp_f16;       // tmp_data stored as half (reference default)
+    float matrix[16];      // previous frame's view-projection, row-vector
+    float pixel_offset_x, pixel_offset_y;
+};
+
+// Device pointers of one stream's buffer set for the current frame (the
+// double buffers already resolved to current/previous).
+struct StageBuffers {
+    void *albedo;
+    void *normals_cur, *normals_prev;
+    void *positions_cur, *positions_prev;
+    void *noisy_in, *noisy_acc_cur, *noisy_acc_prev;
+    void *spp_cur, *spp_prev;
+    void *prev_pixel, *accept;
+    void *tmp_data, *weights, *mins_maxs;
+    void *filtered, *tone_mapped;
+    void *out_cur, *out_prev;
+    void *result_cur, *result_prev;
+};
+
+// Fitter launch shape, chosen per stream (or by the autotuner).
+struct FitterConfig {
+    int32_t rows_per_lane;    // 4, 8 or 16: 1024 rows over 256, 128 or 64 lanes
+    int32_t blocks_per_group; // 32x32 blocks handled by one work-group
+};
+
+// One fitter work item: a 32x32 block of some stream's frame.
+struct FitterTask {
+    const void *tmp_block;    // 13*1024 elements of that block
+    float *mins_maxs;         // 12 floats
+    float *weights;           // 30 floats
+    int32_t frame;
+    int32_t flags;            // bit 0: tmp is half; bit 1: round to half
+};
+
+hipError_t launch_accumulate_noisy_data(const StageArgs &a, const StageBuffers &b, hipStream_t s);
+hipError_t launch_fitter(const StageArgs &a, const StageBuffers &b, const FitterConfig &cfg,
+                         int32_t qr_mode, hipStream_t s);
+hipError_t launch_weighted_sum(const StageArgs &a, const StageBuffers &b, hipStream_t s);
+hipError_t launch_accumulate_filtered_data(const StageArgs &a, const StageBuffers &b, hipStream_t s);
+hipError_t launch_taa(const StageArgs &a, const StageBuffers &b, hipStream_t s);
+
+}  // namespace bmfr

@@ -1,0 +1,409 @@
+// Stages 1, 3, 4 and 5 of the BMFR frame chain for gfx950: per-pixel,
+// HBM-streaming kernels.  Each follows the operation order of the
+// corresponding reference kernel so that, with contraction off, its results
+// are reproducible bit for bit (the tone map's powf aside).
+//
+// Launch shapes: stage 1 runs over the padded grid in 32x8 tiles so that a
+// wave writes whole 32-pixel rows of one block of tmp_data (block-major,
+// [by][bx][feature][y][x]); stages 3-5 run over the image in 64x4 tiles so
+// that a wave reads 64 consecutive AoS pixels (768 contiguous bytes).
+#include "device_common.h"
+#include "kernels.h"
+
+namespace bmfr {
+
+// ---------------------------------------------------------------------------
+// Stage 1: accumulate_noisy_data, opencl/bmfr.cl:287-485.
+template <typename GT, typename TT>
+__global__ __launch_bounds__(256) void k_accumulate_noisy_data(
+    const StageArgs a, const GT *__restrict__ cur_normals, const GT *__restrict__ prev_normals,
+    const GT *__restrict__ cur_positions, const GT *__restrict__ prev_positions,
+    const GT *__restrict__ cur_noisy, const GT *__restrict__ prev_noisy,
+    const uint8_t *__restrict__ prev_spp, GT *__restrict__ acc_noisy,
+    float2 *__restrict__ out_prev_pixel, uint8_t *__restrict__ accept_bools,
+    uint8_t *__restrict__ cur_spp, TT *__restrict__ tmp_data) {
+    const int gx = blockIdx.x * 32 + (threadIdx.x & 31);
+    const int gy = blockIdx.y * 8 + (threadIdx.x >> 5);
+    const int W = a.width, H = a.height;
+    const int2 off = block_offset(a.frame);
+    const int pwx = gx - kBlock / 2 + off.x;
+    const int pwy = gy - kBlock / 2 + off.y;
+    const int px = mirror(pwx, W), py = mirror(pwy, H);
+    const int64_t lp = (int64_t)py * W + px;
+
+    const float3 wp = load3(cur_positions, lp);
+    const float3 nrm = load3(cur_normals, lp);
+    const float3 cur = load3(cur_noisy, lp);
+
+    float ppf_x = (float)px, ppf_y = (float)py;
+    uint32_t store_accept = 0;
+    float blend_alpha = 1.f;
+    float3 prev_col = make_float3(0.f, 0.f, 0.f);
+    float sample_spp = 0.f;
+    if (a.frame > 0) {
+        const float *m = a.matrix;
+        // row-vector matrix, dot() left to right, w = 1 (bmfr.cl:343-356)
+        float uvx = ((m[0] * wp.x + m[4] * wp.y) + m[8] * wp.z) + m[12] * 1.f;
+        float uvy = ((m[1] * wp.x + m[5] * wp.y) + m[9] * wp.z) + m[13] * 1.f;
+        const float ww = ((m[3] * wp.x + m[7] * wp.y) + m[11] * wp.z) + m[15] * 1.f;
+        uvx = uvx / ww;
+        uvy = uvy / ww;
+        uvx = uvx + 1.f;
+        uvy = uvy + 1.f;
+        uvx = uvx / 2.f;
+        uvy = uvy / 2.f;
+        ppf_x = uvx * (float)W;
+        ppf_y = uvy * (float)H;
+        ppf_x = ppf_x - a.pixel_offset_x;
+        ppf_y = ppf_y - (1.f - a.pixel_offset_y);
+        const int ipx = cvt_rtn(ppf_x), ipy = cvt_rtn(ppf_y);
+        const float fx = ppf_x - (float)ipx, fy = ppf_y - (float)ipy;
+        const float omx = 1.f - fx, omy = 1.f - fy;
+        const float wts[4] = {omx * omy, fx * omy, omx * fy, fx * fy};
+        float total_weight = 0.f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {  // bmfr.cl:374-419
+            const int sx = wrap_add(ipx, i & 1);
+            const int sy = wrap_add(ipy, i >> 1);
+            if (sx >= 0 && sy >= 0 && sx < W && sy < H) {
+                const int64_t ls = (int64_t)wrap_mad(sy, W, sx);
+                const float3 pp = load3(prev_positions, ls);
+                const float dx = pp.x - wp.x, dy = pp.y - wp.y, dz = pp.z - wp.z;
+                const float pd = (dx * dx + dy * dy) + dz * dz;
+                if (pd < a.position_limit_squared) {
+                    const float3 pn = load3(prev_normals, ls);
+                    const float nx = pn.x - nrm.x, ny = pn.y - nrm.y, nz = pn.z - nrm.z;
+                    const float nd = (nx * nx + ny * ny) + nz * nz;
+                    if (nd < a.normal_limit_squared) {
+                        store_accept |= 1u << i;
+                        sample_spp = sample_spp + wts[i] * (float)prev_spp[ls];
+                        const float3 pc = load3(prev_noisy, ls);
+                        prev_col.x = prev_col.x + wts[i] * pc.x;
+                        prev_col.y = prev_col.y + wts[i] * pc.y;
+                        prev_col.z = prev_col.z + wts[i] * pc.z;
+                        total_weight = total_weight + wts[i];
+                    }
+                }
+            }
+        }
+        if (total_weight > 0.f) {  // bmfr.cl:421-429
+            prev_col.x = prev_col.x / total_weight;
+            prev_col.y = prev_col.y / total_weight;
+            prev_col.z = prev_col.z / total_weight;
+            sample_spp = sample_spp / total_weight;
+            blend_alpha = 1.f / (sample_spp + 1.f);
+            blend_alpha = fmaxf(blend_alpha, a.blend_alpha);
+        }
+    }
+    // bmfr.cl:433-442
+    uint32_t new_spp = 1;
+    if (blend_alpha < 1.f) {
+        if (sample_spp > 254.f) {
+            new_spp = 255;
+        } else {
+            float r = __builtin_rintf(sample_spp);
+            r = fminf(fmaxf(r, 0.f), 255.f);
+            new_spp = (uint32_t)r + 1u;
+        }
+    }
+    const float omb = 1.f - blend_alpha;
+    float3 nc;
+    nc.x = blend_alpha * cur.x + omb * prev_col.x;
+    nc.y = blend_alpha * cur.y + omb * prev_col.y;
+    nc.z = blend_alpha * cur.z + omb * prev_col.z;
+
+    // bmfr.cl:448-476: 13 features, block-major
+    const float feat[kBufferCount] = {1.f,
+                                      nrm.x,
+                                      nrm.y,
+                                      nrm.z,
+                                      wp.x,
+                                      wp.y,
+                                      wp.z,
+                                      wp.x * wp.x,
+                                      wp.y * wp.y,
+                                      wp.z * wp.z,
+                                      nc.x,
+                                      nc.y,
+                                      nc.z};
+    const int xin = gx & (kBlock - 1), yin = gy & (kBlock - 1);
+    const int64_t blk = (int64_t)(gy / kBlock) * a.blocks_x + gx / kBlock;
+    TT *dst = tmp_data + blk * (kBufferCount * kBlockPixels) + yin * kBlock + xin;
+#pragma unroll
+    for (int f = 0; f < kBufferCount; ++f) {
+        float v = feat[f];
+        if (v != v) v = 0.f;
+        if constexpr (sizeof(TT) == 2) {
+            v = fmaxf(fminf(v, 65504.f), -65504.f);
+            dst[f * kBlockPixels] = __float2half_rn(v);
+        } else {
+            dst[f * kBlockPixels] = v;
+        }
+    }
+    // bmfr.cl:478-484 (and the spp of bmfr.cl:442, which mirrored
+    // work-items would only rewrite with the same value)
+    if (pwx >= 0 && pwx < W && pwy >= 0 && pwy < H) {
+        store3(acc_noisy, lp, nc);
+        out_prev_pixel[lp] = make_float2(ppf_x, ppf_y);
+        accept_bools[lp] = (uint8_t)store_accept;
+        cur_spp[lp] = (uint8_t)new_spp;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Stage 3: weighted_sum, opencl/bmfr.cl:703-758.
+template <typename GT>
+__global__ __launch_bounds__(256) void k_weighted_sum(
+    const StageArgs a, const float *__restrict__ weights, const float *__restrict__ mins_maxs,
+    float *__restrict__ output, const GT *__restrict__ cur_normals,
+    const GT *__restrict__ cur_positions) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= a.width || y >= a.height) return;
+    const int64_t lp = (int64_t)y * a.width + x;
+    const int2 off = block_offset(a.frame);
+    const int ox = x + kBlock / 2 - off.x, oy = y + kBlock / 2 - off.y;
+    const int64_t g = (int64_t)(ox / kBlock) + (int64_t)(oy / kBlock) * a.blocks_x;
+    const float3 wp = load3(cur_positions, lp);
+    const float3 n = load3(cur_normals, lp);
+    const float feat[kFeatures] = {1.f,  n.x,  n.y,         n.z,         wp.x,
+                                   wp.y, wp.z, wp.x * wp.x, wp.y * wp.y, wp.z * wp.z};
+    const float *w = weights + g * (kFeatures * 3);
+    const float *mm = mins_maxs + g * (kScaled * 2);
+    float cx = 0.f, cy = 0.f, cz = 0.f;
+#pragma unroll
+    for (int f = 0; f < kFeatures; ++f) {
+        float v = feat[f];
+        if (f >= kNotScaled) v = scale_feature(v, mm[(f - kNotScaled) * 2], mm[(f - kNotScaled) * 2 + 1]);
+        cx = cx + w[f * 3 + 0] * v;
+        cy = cy + w[f * 3 + 1] * v;
+        cz = cz + w[f * 3 + 2] * v;
+    }
+    output[lp * 3 + 0] = cx < 0.f ? 0.f : cx;
+    output[lp * 3 + 1] = cy < 0.f ? 0.f : cy;
+    output[lp * 3 + 2] = cz < 0.f ? 0.f : cz;
+}
+
+// ---------------------------------------------------------------------------
+// Stage 4: accumulate_filtered_data, opencl/bmfr.cl:761-857.
+template <typename GT>
+__global__ __launch_bounds__(256) void k_accumulate_filtered_data(
+    const StageArgs a, const float *__restrict__ filtered, const float2 *__restrict__ prev_pixel,
+    const uint8_t *__restrict__ accept_bools, const float *__restrict__ albedo,
+    float *__restrict__ tone_mapped, const uint8_t *__restrict__ cur_spp,
+    const GT *__restrict__ acc_prev, GT *__restrict__ acc_out) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int W = a.width;
+    if (x >= W || y >= a.height) return;
+    const int64_t lp = (int64_t)y * W + x;
+    const float3 fc = make_float3(filtered[lp * 3 + 0], filtered[lp * 3 + 1], filtered[lp * 3 + 2]);
+    float3 pc = make_float3(0.f, 0.f, 0.f);
+    float blend_alpha = 1.f;
+    if (a.frame > 0) {
+        const uint32_t acc = accept_bools[lp];
+        if (acc > 0) {
+            const float2 pf = prev_pixel[lp];
+            const int ix = cvt_rtn(pf.x), iy = cvt_rtn(pf.y);
+            const float fx = pf.x - (float)ix, fy = pf.y - (float)iy;
+            const float omx = 1.f - fx, omy = 1.f - fy;
+            const float wts[4] = {omx * omy, fx * omy, omx * fy, fx * fy};
+            float total_weight = 0.f;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (acc & (1u << i)) {
+                    total_weight = total_weight + wts[i];
+                    const int64_t ls = (int64_t)wrap_add(wrap_mad(wrap_add(iy, i >> 1), W, ix), i & 1);
+                    const float3 hv = load3(acc_prev, ls);
+                    pc.x = pc.x + wts[i] * hv.x;
+                    pc.y = pc.y + wts[i] * hv.y;
+                    pc.z = pc.z + wts[i] * hv.z;
+                }
+            }
+            if (total_weight > 0.f) {
+                blend_alpha = 1.f / (float)cur_spp[lp];
+                blend_alpha = fmaxf(blend_alpha, a.second_blend_alpha);
+                pc.x = pc.x / total_weight;
+                pc.y = pc.y / total_weight;
+                pc.z = pc.z / total_weight;
+            }
+        }
+    }
+    const float omb = 1.f - blend_alpha;
+    float3 ac;
+    ac.x = blend_alpha * fc.x + omb * pc.x;
+    ac.y = blend_alpha * fc.y + omb * pc.y;
+    ac.z = blend_alpha * fc.z + omb * pc.z;
+    store3(acc_out, lp, ac);
+    const float ar = albedo[lp * 3 + 0], ag = albedo[lp * 3 + 1], ab = albedo[lp * 3 + 2];
+    tone_mapped[lp * 3 + 0] = fminf(fmaxf(powf(fmaxf(0.f, ar * ac.x), 0.454545f), 0.f), 1.f);
+    tone_mapped[lp * 3 + 1] = fminf(fmaxf(powf(fmaxf(0.f, ag * ac.y), 0.454545f), 0.f), 1.f);
+    tone_mapped[lp * 3 + 2] = fminf(fmaxf(powf(fmaxf(0.f, ab * ac.z), 0.454545f), 0.f), 1.f);
+}
+
+// ---------------------------------------------------------------------------
+// Stage 5: taa, opencl/bmfr.cl:860-974.
+__device__ __forceinline__ float3 rgb_to_ycocg(float3 c) {
+    return make_float3((c.x * 1.f + c.y * 2.f) + c.z * 1.f, (c.x * 2.f + c.y * 0.f) + c.z * -2.f,
+                       (c.x * -1.f + c.y * 2.f) + c.z * -1.f);
+}
+__device__ __forceinline__ float3 ycocg_to_rgb(float3 c) {
+    return make_float3((c.x * 0.25f + c.y * 0.25f) + c.z * -0.25f,
+                       (c.x * 0.25f + c.y * 0.f) + c.z * 0.25f,
+                       (c.x * 0.25f + c.y * -0.25f) + c.z * -0.25f);
+}
+
+template <typename GT>
+__global__ __launch_bounds__(256) void k_taa(const StageArgs a,
+                                             const float2 *__restrict__ prev_pixel,
+                                             const float *__restrict__ new_frame,
+                                             GT *__restrict__ result, const GT *__restrict__ prev_result) {
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int W = a.width, H = a.height;
+    if (x >= W || y >= H) return;
+    const int64_t lp = (int64_t)y * W + x;
+    const float3 mine = make_float3(new_frame[lp * 3 + 0], new_frame[lp * 3 + 1], new_frame[lp * 3 + 2]);
+    const float2 pf = prev_pixel[lp];
+    const int ix = cvt_rtn(pf.x), iy = cvt_rtn(pf.y);
+    if (a.frame == 0 || ix < -1 || iy < -1 || ix >= W || iy >= H) {
+        store3(result, lp, mine);
+        return;
+    }
+    float3 mnb = make_float3(INFINITY, INFINITY, INFINITY), mnc = mnb;
+    float3 mxb = make_float3(-INFINITY, -INFINITY, -INFINITY), mxc = mxb;
+#pragma unroll
+    for (int dy = -1; dy < 2; ++dy) {
+#pragma unroll
+        for (int dx = -1; dx < 2; ++dx) {
+            const int sx = x + dx, sy = y + dy;
+            if (sx >= 0 && sy >= 0 && sx < W && sy < H) {
+                float3 sc;
+                if (dx == 0 && dy == 0) {
+                    sc = mine;
+                } else {
+                    const int64_t ls = (int64_t)sy * W + sx;
+                    sc = make_float3(new_frame[ls * 3 + 0], new_frame[ls * 3 + 1], new_frame[ls * 3 + 2]);
+                }
+                const float3 yc = rgb_to_ycocg(sc);
+                if (dx == 0 || dy == 0) {
+                    mnc = make_float3(fminf(mnc.x, yc.x), fminf(mnc.y, yc.y), fminf(mnc.z, yc.z));
+                    mxc = make_float3(fmaxf(mxc.x, yc.x), fmaxf(mxc.y, yc.y), fmaxf(mxc.z, yc.z));
+                }
+                mnb = make_float3(fminf(mnb.x, yc.x), fminf(mnb.y, yc.y), fminf(mnb.z, yc.z));
+                mxb = make_float3(fmaxf(mxb.x, yc.x), fmaxf(mxb.y, yc.y), fmaxf(mxb.z, yc.z));
+            }
+        }
+    }
+    float3 pc = make_float3(0.f, 0.f, 0.f);
+    float tw = 0.f;
+    const float fx = pf.x - (float)ix, fy = pf.y - (float)iy;
+    const float omx = 1.f - fx, omy = 1.f - fy;
+    const int64_t r0 = (int64_t)iy * W, r1 = (int64_t)(iy + 1) * W;
+    if (iy >= 0) {
+        if (ix >= 0) {
+            const float w = omx * omy;
+            const float3 v = load3(prev_result, r0 + ix);
+            pc.x = pc.x + w * v.x; pc.y = pc.y + w * v.y; pc.z = pc.z + w * v.z;
+            tw = tw + w;
+        }
+        if (ix < W - 1) {
+            const float w = fx * omy;
+            const float3 v = load3(prev_result, r0 + ix + 1);
+            pc.x = pc.x + w * v.x; pc.y = pc.y + w * v.y; pc.z = pc.z + w * v.z;
+            tw = tw + w;
+        }
+    }
+    if (iy < H - 1) {
+        if (ix >= 0) {
+            const float w = omx * fy;
+            const float3 v = load3(prev_result, r1 + ix);
+            pc.x = pc.x + w * v.x; pc.y = pc.y + w * v.y; pc.z = pc.z + w * v.z;
+            tw = tw + w;
+        }
+        if (ix < W - 1) {
+            const float w = fx * fy;
+            const float3 v = load3(prev_result, r1 + ix + 1);
+            pc.x = pc.x + w * v.x; pc.y = pc.y + w * v.y; pc.z = pc.z + w * v.z;
+            tw = tw + w;
+        }
+    }
+    pc.x = pc.x / tw;  // unguarded like bmfr.cl:962
+    pc.y = pc.y / tw;
+    pc.z = pc.z / tw;
+    const float3 pyc = rgb_to_ycocg(pc);
+    float3 cl;
+    cl.x = fminf(fmaxf(pyc.x, (mnb.x + mnc.x) / 2.f), (mxb.x + mxc.x) / 2.f);
+    cl.y = fminf(fmaxf(pyc.y, (mnb.y + mnc.y) / 2.f), (mxb.y + mxc.y) / 2.f);
+    cl.z = fminf(fmaxf(pyc.z, (mnb.z + mnc.z) / 2.f), (mxb.z + mxc.z) / 2.f);
+    const float3 prgb = ycocg_to_rgb(cl);
+    const float ta = a.taa_blend_alpha, omt = 1.f - ta;
+    float3 r;
+    r.x = ta * mine.x + omt * prgb.x;
+    r.y = ta * mine.y + omt * prgb.y;
+    r.z = ta * mine.z + omt * prgb.z;
+    store3(result, lp, r);
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launchers.
+template <typename GT, typename TT>
+static hipError_t launch_s1(const StageArgs &a, const StageBuffers &b, hipStream_t s) {
+    dim3 grid(a.margin_width / 32, a.margin_height / 8);
+    hipLaunchKernelGGL((k_accumulate_noisy_data<GT, TT>), grid, dim3(256), 0, s, a,
+                       (const GT *)b.normals_cur, (const GT *)b.normals_prev,
+                       (const GT *)b.positions_cur, (const GT *)b.positions_prev,
+                       (const GT *)b.noisy_in, (const GT *)b.noisy_acc_prev,
+                       (const uint8_t *)b.spp_prev, (GT *)b.noisy_acc_cur, (float2 *)b.prev_pixel,
+                       (uint8_t *)b.accept, (uint8_t *)b.spp_cur, (TT *)b.tmp_data);
+    return hipGetLastError();
+}
+
+hipError_t launch_accumulate_noisy_data(const StageArgs &a, const StageBuffers &b, hipStream_t s) {
+    if (a.storage_f16) {
+        return a.tmp_f16 ? launch_s1<__half, __half>(a, b, s) : launch_s1<__half, float>(a, b, s);
+    }
+    return a.tmp_f16 ? launch_s1<float, __half>(a, b, s) : launch_s1<float, float>(a, b, s);
+}
+
+static inline dim3 image_grid(const StageArgs &a) {
+    return dim3((a.width + 63) / 64, (a.height + 3) / 4);
+}
+
+hipError_t launch_weighted_sum(const StageArgs &a, const StageBuffers &b, hipStream_t s) {
+    if (a.storage_f16)
+        hipLaunchKernelGGL((k_weighted_sum<__half>), image_grid(a), dim3(256), 0, s, a,
+                           (const float *)b.weights, (const float *)b.mins_maxs, (float *)b.filtered,
+                           (const __half *)b.normals_cur, (const __half *)b.positions_cur);
+    else
+        hipLaunchKernelGGL((k_weighted_sum<float>), image_grid(a), dim3(256), 0, s, a,
+                           (const float *)b.weights, (const float *)b.mins_maxs, (float *)b.filtered,
+                           (const float *)b.normals_cur, (const float *)b.positions_cur);
+    return hipGetLastError();
+}
+
+template <typename GT>
+static hipError_t launch_s4(const StageArgs &a, const StageBuffers &b, hipStream_t s) {
+    hipLaunchKernelGGL((k_accumulate_filtered_data<GT>), image_grid(a), dim3(256), 0, s, a,
+                       (const float *)b.filtered, (const float2 *)b.prev_pixel,
+                       (const uint8_t *)b.accept, (const float *)b.albedo, (float *)b.tone_mapped,
+                       (const uint8_t *)b.spp_cur, (const GT *)b.out_prev, (GT *)b.out_cur);
+    return hipGetLastError();
+}
+
+hipError_t launch_accumulate_filtered_data(const StageArgs &a, const StageBuffers &b, hipStream_t s) {
+    return a.storage_f16 ? launch_s4<__half>(a, b, s) : launch_s4<float>(a, b, s);
+}
+
+template <typename GT>
+static hipError_t launch_s5(const StageArgs &a, const StageBuffers &b, hipStream_t s) {
+    hipLaunchKernelGGL((k_taa<GT>), image_grid(a), dim3(256), 0, s, a, (const float2 *)b.prev_pixel,
+                       (const float *)b.tone_mapped, (GT *)b.result_cur, (const GT *)b.result_prev);
+    return hipGetLastError();
+}
+
+hipError_t launch_taa(const StageArgs &a, const StageBuffers &b, hipStream_t s) {
+    return a.storage_f16 ? launch_s5<__half>(a, b, s) : launch_s5<float>(a, b, s);
+}
+
+}  // namespace bmfr

@@ -1,0 +1,204 @@
+/*
+ * bmfr_hip.h -- C ABI of the MI355X (gfx950) BMFR denoiser.
+ *
+ * The five stage calls replace the five OpenCL kernels of the reference and
+ * keep their names, order and arguments' meaning (opencl/bmfr.cl:290, 490,
+ * 703, 761, 860, enqueued per frame at opencl/bmfr.cpp:429-476).  The
+ * reference's compile-time -D configuration (opencl/bmfr.cpp:32-118,
+ * 205-232) becomes the runtime struct bmfr_params, and the reference's
+ * cl::Buffer set with its Double_buffer swaps (opencl/bmfr.cpp:120-135,
+ * 315-347, 482-484) becomes a bmfr_stream that owns its device buffers.
+ *
+ * Conventions: plain C types only; every call returns a bmfr_status (0 = OK)
+ * and never throws or exits; bmfr_last_error() gives a message for the last
+ * failure on the calling thread.  All image buffers are AoS float3 (RGB or
+ * XYZ) with row stride `width`, matching opencl/bmfr.cl:224-241.  Stage
+ * calls are asynchronous on the stream's HIP stream and ordered like the
+ * reference's in-order queue (opencl/bmfr.cpp:191).  A stream handle is
+ * driven by one host thread at a time.
+ */
+#ifndef BMFR_HIP_H
+#define BMFR_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BMFR_API_VERSION 1
+
+typedef enum bmfr_status {
+    BMFR_OK = 0,
+    BMFR_ERR_INVALID_ARGUMENT = 1, /* bad size, null pointer, unknown enum */
+    BMFR_ERR_OUT_OF_BUDGET = 2,    /* admission refused by the HBM budget */
+    BMFR_ERR_HIP = 3,              /* a HIP runtime call failed */
+    BMFR_ERR_NO_DEVICE = 4,        /* no usable gfx950 device */
+    BMFR_ERR_STATE = 5             /* call not valid in the current state */
+} bmfr_status;
+
+/* Storage precision of G-buffers and temporal histories. */
+typedef enum bmfr_storage { BMFR_STORAGE_F32 = 0, BMFR_STORAGE_F16 = 1 } bmfr_storage;
+
+/* Fitter arithmetic. */
+typedef enum bmfr_qr_mode {
+    BMFR_QR_FP32 = 0,         /* plain FP32 Householder, like bmfr.cl:549-656 */
+    BMFR_QR_KAHAN = 1,        /* FP32 with compensated norm and dot sums */
+    BMFR_QR_KAHAN_FP64 = 2    /* as KAHAN, plus FP64 re-factorisation of blocks
+                                 whose min |R_ii| < refactor_threshold*max |R_ii| */
+} bmfr_qr_mode;
+
+/* Runtime form of the reference's -D options (opencl/bmfr.cpp:56-118, 205-232). */
+typedef struct bmfr_params {
+    int32_t width;                  /* IMAGE_WIDTH */
+    int32_t height;                 /* IMAGE_HEIGHT */
+    float position_limit_squared;   /* POSITION_LIMIT_SQUARED */
+    float normal_limit_squared;     /* NORMAL_LIMIT_SQUARED */
+    float blend_alpha;              /* BLEND_ALPHA (0.2) */
+    float second_blend_alpha;       /* SECOND_BLEND_ALPHA (0.1) */
+    float taa_blend_alpha;          /* TAA_BLEND_ALPHA (0.2) */
+    double noise_amount;            /* NOISE_AMOUNT (1e-2, applied in double) */
+    int32_t storage;                /* bmfr_storage of normals, positions, noisy
+                                       colour and the accumulation/TAA histories */
+    int32_t tmp_storage;            /* bmfr_storage of the fitter input tmp_data;
+                                       F16 is USE_HALF_PRECISION_IN_TMP_DATA=1 */
+    int32_t qr_mode;                /* bmfr_qr_mode */
+    float refactor_threshold;       /* used by BMFR_QR_KAHAN_FP64 (1e-6) */
+} bmfr_params;
+
+/* Buffer ids for the debug/parity accessors.  Element types: F = float or
+ * half per `storage`, T = float or half per `tmp_storage`, u8 = uint8. */
+typedef enum bmfr_buffer {
+    BMFR_BUF_ALBEDO = 0,          /* float[3P]   albedo_buffer */
+    BMFR_BUF_NORMALS_CUR = 1,     /* F[3P]       normals_buffer.current() */
+    BMFR_BUF_NORMALS_PREV = 2,    /* F[3P]       normals_buffer.previous() */
+    BMFR_BUF_POSITIONS_CUR = 3,   /* F[3P] */
+    BMFR_BUF_POSITIONS_PREV = 4,  /* F[3P] */
+    BMFR_BUF_NOISY_IN = 5,        /* F[3P]       uploaded noisy colour */
+    BMFR_BUF_NOISY_ACC_CUR = 6,   /* F[3P]       accumulated noisy colour */
+    BMFR_BUF_NOISY_ACC_PREV = 7,  /* F[3P]       previous_noisy of stage 1 */
+    BMFR_BUF_SPP_CUR = 8,         /* u8[P] */
+    BMFR_BUF_SPP_PREV = 9,        /* u8[P] */
+    BMFR_BUF_PREV_PIXEL = 10,     /* float2[P]   prev_pixels_buffer */
+    BMFR_BUF_ACCEPT = 11,         /* u8[P]       accept_buffer */
+    BMFR_BUF_TMP_DATA = 12,       /* T[13Q]      in_buffer (block-major) */
+    BMFR_BUF_WEIGHTS = 13,        /* float[30 nB] */
+    BMFR_BUF_MINS_MAXS = 14,      /* float[12 nB] */
+    BMFR_BUF_FILTERED = 15,       /* float[3P] */
+    BMFR_BUF_TONE_MAPPED = 16,    /* float[3P] */
+    BMFR_BUF_OUT_CUR = 17,        /* F[3P]       out_buffer.current() */
+    BMFR_BUF_OUT_PREV = 18,       /* F[3P] */
+    BMFR_BUF_RESULT_CUR = 19,     /* F[3P]       result_buffer.current() */
+    BMFR_BUF_RESULT_PREV = 20,    /* F[3P] */
+    BMFR_BUF_COUNT = 21
+} bmfr_buffer;
+
+typedef struct bmfr_stream_s *bmfr_stream;
+
+/* Sizes derived exactly as opencl/bmfr.cpp:104-118. */
+typedef struct bmfr_sizes {
+    int32_t workset_width, workset_height;   /* WORKSET_WIDTH/HEIGHT */
+    int32_t margin_width, margin_height;     /* WORKSET_WITH_MARGINS_* */
+    int32_t blocks_x, blocks_y, blocks;      /* fitter work-groups */
+    int64_t pixels;                          /* P = W*H */
+    int64_t padded_pixels;                   /* Q = margin_width*margin_height */
+    int64_t device_bytes;                    /* per-stream device footprint */
+} bmfr_sizes;
+
+/* ---- library ------------------------------------------------------------ */
+int32_t bmfr_api_version(void);
+const char *bmfr_status_string(int32_t status);
+const char *bmfr_last_error(void);
+/* Number of visible HIP devices (0 without a GPU); never fails. */
+int32_t bmfr_device_count(void);
+
+/* Reference defaults (opencl/bmfr.cpp:58-62, 88) for a W x H stream:
+ * F32 storage, F16 tmp_data, plain FP32 QR. */
+int32_t bmfr_default_params(int32_t width, int32_t height, bmfr_params *out);
+/* Validates params (W, H > 32, opencl/bmfr.cl:312-313) and fills sizes. */
+int32_t bmfr_compute_sizes(const bmfr_params *params, bmfr_sizes *out);
+
+/* ---- stream lifecycle (replaces tasks() set-up, opencl/bmfr.cpp:179-384) */
+int32_t bmfr_stream_open(const bmfr_params *params, bmfr_stream *out);
+int32_t bmfr_stream_close(bmfr_stream stream);
+int32_t bmfr_stream_sizes(bmfr_stream stream, bmfr_sizes *out);
+/* Clears all temporal history so the next frame behaves like frame 0. */
+int32_t bmfr_stream_reset(bmfr_stream stream);
+
+/* Fitter launch shape: rows_per_lane 4, 8 or 16 (a 32x32 block spread over
+ * 4, 2 or 1 wavefronts) and blocks_per_group 1, 2 or 4 (see fitter.hip for
+ * the supported pairs).  No counterpart in the reference, whose fitter is
+ * always one 256-wide work-group per block (opencl/bmfr.cpp:117-118, 249). */
+int32_t bmfr_set_fitter_config(bmfr_stream stream, int32_t rows_per_lane, int32_t blocks_per_group);
+
+/* ---- per-frame inputs (opencl/bmfr.cpp:420-427) -------------------------- */
+/* Host arrays of W*H*3 floats; converted to the stream's storage.  The call
+ * returns once the data is on the device. */
+int32_t bmfr_upload_frame(bmfr_stream stream, const float *albedo, const float *normals,
+                          const float *positions, const float *noisy);
+
+/* ---- the five stages (opencl/bmfr.cpp:429-476) ---------------------------- */
+/* prev_frame_camera_matrix: 16 floats, the previous frame's view-projection
+ * in the reference's row-vector layout (camera_matrices[max(f-1,0)],
+ * opencl/bmfr.cpp:440-442); pixel_offset: 2 floats (pixel_offsets[f]). */
+int32_t bmfr_accumulate_noisy_data(bmfr_stream stream, int32_t frame_number,
+                                   const float *prev_frame_camera_matrix,
+                                   const float *pixel_offset);
+int32_t bmfr_fitter(bmfr_stream stream, int32_t frame_number);
+int32_t bmfr_weighted_sum(bmfr_stream stream, int32_t frame_number);
+int32_t bmfr_accumulate_filtered_data(bmfr_stream stream, int32_t frame_number);
+int32_t bmfr_taa(bmfr_stream stream, int32_t frame_number);
+/* Swaps the double buffers (opencl/bmfr.cpp:482-484). */
+int32_t bmfr_end_frame(bmfr_stream stream);
+/* All five stages in order, then bmfr_end_frame. */
+int32_t bmfr_run_frame(bmfr_stream stream, int32_t frame_number,
+                       const float *prev_frame_camera_matrix, const float *pixel_offset);
+
+/* ---- outputs ---------------------------------------------------------------- */
+/* The last completed frame's TAA result (result_buffer of the frame before
+ * the most recent bmfr_end_frame), W*H*3 floats (opencl/bmfr.cpp:479-480). */
+int32_t bmfr_download_result(bmfr_stream stream, float *out);
+int32_t bmfr_synchronize(bmfr_stream stream);
+
+/* Debug/parity access to every device buffer, in its native element type. */
+int32_t bmfr_buffer_bytes(bmfr_stream stream, int32_t buffer, size_t *bytes);
+int32_t bmfr_read_buffer(bmfr_stream stream, int32_t buffer, void *host, size_t bytes);
+int32_t bmfr_write_buffer(bmfr_stream stream, int32_t buffer, const void *host, size_t bytes);
+/* Device address of a buffer, for zero-copy producers (a renderer writing the
+ * current G-buffers directly).  Valid until the next bmfr_end_frame. */
+int32_t bmfr_buffer_device_ptr(bmfr_stream stream, int32_t buffer, void **dptr);
+
+/* ---- timing (opencl/bmfr.cpp:386-412, 488-517) -------------------------------- */
+/* When enabled, HIP events bracket every stage on the stream's HIP stream.
+ * bmfr_stage_times waits for the last frame and returns 6 values in ms:
+ * accumulate_noisy_data, fitter, weighted_sum, accumulate_filtered_data,
+ * taa, and the total from the first stage's start to the last stage's end
+ * (the reference's "Total time in all kernels"). */
+int32_t bmfr_set_timing(bmfr_stream stream, int32_t enable);
+int32_t bmfr_stage_times(bmfr_stream stream, float *ms6);
+
+/* ---- procedural scene (replaces the EXR dataset, opencl/bmfr.cpp:145-172) ---- */
+typedef struct bmfr_scene_s *bmfr_scene;
+/* Plane y=0 (|x|,|z| <= 64) with `spheres` spheres, seeded; a pinhole camera
+ * (60 deg vertical FOV) at (0,2,-10) looking at the origin that moves 0.05 in
+ * x and yaws 0.3 deg per frame.  `static_camera` != 0 freezes camera and
+ * pixel offset (for reprojection invariants). */
+int32_t bmfr_scene_create(uint64_t seed, int32_t spheres, float firefly_probability,
+                          int32_t width, int32_t height, int32_t static_camera,
+                          bmfr_scene *out);
+int32_t bmfr_scene_destroy(bmfr_scene scene);
+/* Row-vector view-projection of `frame` (16 floats) and its pixel offset. */
+int32_t bmfr_scene_camera(bmfr_scene scene, int32_t frame, float *view_projection,
+                          float *pixel_offset);
+/* CPU render of one frame into host arrays of W*H*3 floats (any may be null). */
+int32_t bmfr_scene_render_host(bmfr_scene scene, int32_t frame, float *albedo,
+                               float *normals, float *positions, float *noisy);
+/* GPU render of one frame straight into the stream's current input buffers
+ * (albedo, normals, positions, noisy colour), enqueued on the stream. */
+int32_t bmfr_scene_render_to_stream(bmfr_scene scene, int32_t frame, bmfr_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BMFR_HIP_H */

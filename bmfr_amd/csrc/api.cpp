@@ -46,6 +46,7 @@ int32_t fail(int32_t code, const char *fmt, ...) {
     } while (0)
 
 constexpr int kStages = 5;
+constexpr int kMinImageSize = 47;
 
 // Physical buffers; double buffers take two slots.
 enum Slot {
@@ -63,11 +64,15 @@ enum Slot {
 
 int32_t validate(const bmfr_params *p) {
     if (!p) return fail(BMFR_ERR_INVALID_ARGUMENT, "params is null");
-    // Mirroring at the padded-grid margins is valid only when both image
-    // dimensions exceed the 32-pixel block (opencl/bmfr.cl:312-313).
-    if (p->width <= 32 || p->height <= 32 || p->width > 16384 || p->height > 16384)
-        return fail(BMFR_ERR_INVALID_ARGUMENT, "image size %dx%d outside 33..16384", p->width,
-                    p->height);
+    // Stage 1 mirrors padded-grid coordinates up to workset + 29 back into
+    // the image (opencl/bmfr.cl:209-216, 312-316); that stays inside the
+    // image only when workset + 30 <= 2 * size, i.e. size >= 47.  (The
+    // reference's comment says "bigger than BLOCK_EDGE_LENGTH"; for sizes
+    // 33..46 it reads outside the image.)
+    if (p->width < kMinImageSize || p->height < kMinImageSize || p->width > 16384 ||
+        p->height > 16384)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "image size %dx%d outside %d..16384", p->width,
+                    p->height, kMinImageSize);
     if (p->storage != BMFR_STORAGE_F32 && p->storage != BMFR_STORAGE_F16)
         return fail(BMFR_ERR_INVALID_ARGUMENT, "unknown storage %d", p->storage);
     if (p->tmp_storage != BMFR_STORAGE_F32 && p->tmp_storage != BMFR_STORAGE_F16)

@@ -211,9 +211,12 @@ __global__ __launch_bounds__(256) void k_accumulate_filtered_data(
             float total_weight = 0.f;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                if (acc & (1u << i)) {
+                const int64_t ls = (int64_t)wrap_add(wrap_mad(wrap_add(iy, i >> 1), W, ix), i & 1);
+                // Stage 1 sets a bit only for an in-image tap at the same
+                // floor(); the range check keeps inconsistent inputs (e.g.
+                // written through the debug API) from reading outside.
+                if ((acc & (1u << i)) && ls >= 0 && ls < (int64_t)W * a.height) {
                     total_weight = total_weight + wts[i];
-                    const int64_t ls = (int64_t)wrap_add(wrap_mad(wrap_add(iy, i >> 1), W, ix), i & 1);
                     const float3 hv = load3(acc_prev, ls);
                     pc.x = pc.x + wts[i] * hv.x;
                     pc.y = pc.y + wts[i] * hv.y;

@@ -116,7 +116,9 @@ int32_t bmfr_device_count(void);
 /* Reference defaults (opencl/bmfr.cpp:58-62, 88) for a W x H stream:
  * F32 storage, F16 tmp_data, plain FP32 QR. */
 int32_t bmfr_default_params(int32_t width, int32_t height, bmfr_params *out);
-/* Validates params (W, H > 32, opencl/bmfr.cl:312-313) and fills sizes. */
+/* Validates params and fills sizes.  W and H must be >= 47: stage 1 mirrors
+ * padded-grid coordinates up to WORKSET+29 back into the image
+ * (opencl/bmfr.cl:209-216, 312-316), which needs WORKSET+30 <= 2*size. */
 int32_t bmfr_compute_sizes(const bmfr_params *params, bmfr_sizes *out);
 
 /* ---- stream lifecycle (replaces tasks() set-up, opencl/bmfr.cpp:179-384) */

@@ -71,7 +71,7 @@ def load():
         "orc_tree_sum": (ctypes.c_float, [_P]),
         "orc_tree_min": (ctypes.c_float, [_P]),
         "orc_tree_max": (ctypes.c_float, [_P]),
-        "orc_accumulate_noisy_data": (None, [P] + [_P] * 12 + [_P, _P, ctypes.c_int32]),
+        "orc_accumulate_noisy_data": (ctypes.c_int32, [P] + [_P] * 12 + [_P, _P, ctypes.c_int32]),
         "orc_fitter": (None, [P, _P, _P, _P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
         "orc_fitter_inputs": (None, [P, _P, ctypes.c_int32, ctypes.c_int32, _P, _P, _P]),
         "orc_lstsq_fp64": (ctypes.c_double, [_P, _P, ctypes.c_int32, ctypes.c_int32,
@@ -153,12 +153,14 @@ class OracleStream:
         c, pv = self.cur(), self.prev()
         m = np.ascontiguousarray(matrix, np.float32)
         o = np.ascontiguousarray(offset, np.float32)
-        load().orc_accumulate_noisy_data(
+        rc = load().orc_accumulate_noisy_data(
             ctypes.byref(self.p), _ptr(self.normals[c]), _ptr(self.normals[pv]),
             _ptr(self.positions[c]), _ptr(self.positions[pv]), _ptr(self.noisy_in),
             _ptr(self.noisy_acc[pv]), _ptr(self.spp[pv]), _ptr(self.noisy_acc[c]),
             _ptr(self.prev_pixel), _ptr(self.accept), _ptr(self.spp[c]), _ptr(self.tmp),
             _ptr(m), _ptr(o), frame)
+        if rc != 0:
+            raise ValueError(f"image {self.p.width}x{self.p.height} below the 47-pixel minimum")
 
     def fitter(self, frame, first=0, count=None):
         if count is None:

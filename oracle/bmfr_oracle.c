@@ -241,7 +241,7 @@ static inline int32_t wrap_mad(int32_t a, int32_t b, int32_t c) {
     return (int32_t)((uint32_t)a * (uint32_t)b + (uint32_t)c);
 }
 
-void orc_accumulate_noisy_data(
+int32_t orc_accumulate_noisy_data(
     const orc_params *p, const float *current_normals, const float *previous_normals,
     const float *current_positions, const float *previous_positions,
     const float *current_noisy, const float *previous_noisy, const uint8_t *previous_spp,
@@ -251,6 +251,7 @@ void orc_accumulate_noisy_data(
     orc_sizes s;
     orc_sizes_of(p->width, p->height, &s);
     const int32_t W = p->width, H = p->height;
+    if (W < ORC_MIN_SIZE || H < ORC_MIN_SIZE) return -1;
     int32_t off[2];
     orc_block_offset(frame_number, off);
     const float pos_lim = p->position_limit_squared;
@@ -380,6 +381,7 @@ void orc_accumulate_noisy_data(
             }
         }
     }
+    return 0;
 }
 
 /* LOAD/STORE of bmfr.cl:255-265: in half mode every store rounds to binary16. */
@@ -654,10 +656,12 @@ void orc_accumulate_filtered_data(const orc_params *p, const float *filtered,
                     float total_weight = 0.f;
                     const float wts[4] = {omx * omy, fx * omy, omx * fy, fx * fy};
                     for (int i = 0; i < 4; ++i) {
-                        if (a & (1u << i)) {
+                        const int64_t ls =
+                            (int64_t)wrap_add(wrap_mad(wrap_add(iy, i >> 1), W, ix), i & 1);
+                        /* stage 1 only sets bits of in-image taps; the range
+                         * check covers inconsistent inputs */
+                        if ((a & (1u << i)) && ls >= 0 && ls < (int64_t)W * H) {
                             total_weight = total_weight + wts[i];
-                            const int64_t ls =
-                                (int64_t)wrap_add(wrap_mad(wrap_add(iy, i >> 1), W, ix), i & 1);
                             pc[0] = pc[0] + wts[i] * acc_prev[ls * 3 + 0];
                             pc[1] = pc[1] + wts[i] * acc_prev[ls * 3 + 1];
                             pc[2] = pc[2] + wts[i] * acc_prev[ls * 3 + 2];

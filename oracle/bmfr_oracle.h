@@ -87,11 +87,16 @@ float orc_tree_sum(float *sum_vec);
 float orc_tree_min(float *sum_vec);
 float orc_tree_max(float *sum_vec);
 
-/* Stage 1, bmfr.cl:287-485.  Buffers are AoS float3 with row stride W over
+/* Smallest W, H for which stage 1's mirroring (bmfr.cl:209-216, 312-316)
+ * maps every padded-grid coordinate (up to WORKSET+29) into the image. */
+#define ORC_MIN_SIZE 47
+
+/* Stage 1, bmfr.cl:287-485.  Returns 0, or -1 (and does nothing) when W or
+ * H is below ORC_MIN_SIZE.  Buffers are AoS float3 with row stride W over
  * the first P texels; tmp_data has 13*Q floats in block-major order
  * [by][bx][feature][y_in][x_in] (bmfr.cl:455-464).  In half mode the values
  * written to tmp_data are binary16-representable floats. */
-void orc_accumulate_noisy_data(
+int32_t orc_accumulate_noisy_data(
     const orc_params *p,
     const float *current_normals, const float *previous_normals,
     const float *current_positions, const float *previous_positions,

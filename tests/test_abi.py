@@ -45,7 +45,7 @@ def test_api_version_and_status_strings():
     (1920, 1080, (1920, 1088), (1952, 1120), 2135),
     (640, 360, (640, 384), (672, 416), 273),
     (2560, 1440, (2560, 1440), (2592, 1472), 3726),
-    (33, 33, (64, 64), (96, 96), 9),
+    (47, 47, (64, 64), (96, 96), 9),
 ])
 def test_sizes_match_reference_formulas(w, h, ws, hs, blocks):
     p = bmfr_amd.default_params(w, h)
@@ -68,7 +68,7 @@ def test_reference_defaults():
 
 
 @pytest.mark.parametrize("kw", [
-    dict(width=32), dict(height=16), dict(storage=7), dict(tmp_storage=-1), dict(qr_mode=9),
+    dict(width=32), dict(height=46), dict(width=16), dict(storage=7), dict(tmp_storage=-1), dict(qr_mode=9),
     dict(blend_alpha=1.5), dict(taa_blend_alpha=float("nan")),
 ])
 def test_invalid_params_rejected(kw):

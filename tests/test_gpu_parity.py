@@ -21,7 +21,7 @@ from tests.harness import Lockstep, f16_round, psnr, rel_err, scene
 
 pytestmark = pytest.mark.gpu
 
-SIZES = [(70, 45), (256, 256)]
+SIZES = [(70, 48), (256, 256)]
 
 
 def _stage1_frames(ls, frames, check=True):
@@ -166,7 +166,7 @@ def test_stages_345_bit_exact_given_same_inputs(gpu, w, h):
         ls.close()
 
 
-@pytest.mark.parametrize("w,h,frames", [(70, 45, 8), (256, 256, 8)])
+@pytest.mark.parametrize("w,h,frames", [(70, 48, 8), (256, 256, 8)])
 @pytest.mark.parametrize("tmp_f16", [False, True])
 def test_end_to_end_final_frame(gpu, w, h, frames, tmp_f16):
     """Independent GPU and oracle chains over several frames (moving camera,

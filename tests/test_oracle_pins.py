@@ -105,7 +105,7 @@ def test_mirror_closed_form():
             assert 0 <= lib.orc_mirror(i, n) < n
 
 
-@pytest.mark.parametrize("w,h", [(33, 40), (64, 64), (70, 45)])
+@pytest.mark.parametrize("w,h", [(47, 50), (64, 64), (70, 48)])
 def test_padded_grid_covers_each_pixel_once_and_stage3_maps_back(w, h):
     """Every image pixel is written by exactly one non-mirrored stage-1
     work-item, and stage 3's block index of that pixel is that work-item's
@@ -183,7 +183,7 @@ def _scene(w, h, static=False, seed=0xB3F40000, spheres=8, firefly=1e-3):
 
 
 def test_stage1_frame0_pass_through_and_layout():
-    w, h = 70, 45
+    w, h = 70, 48
     sc = _scene(w, h)
     f = sc.render_host(0)
     p = bind.params(w, h, half_tmp=False)
@@ -248,7 +248,7 @@ def test_stage1_static_camera_closed_form():
 
 
 # ---- stage 2: the fitter against FP64 least squares (bmfr.cl:487-700) --------
-def _fitter_case(w=70, h=45, frame=3, half=False, seed=0xB3F40000):
+def _fitter_case(w=70, h=48, frame=3, half=False, seed=0xB3F40000):
     sc = _scene(w, h, seed=seed)
     p = bind.params(w, h, half_tmp=half)
     o = bind.OracleStream(p)
@@ -320,7 +320,7 @@ def test_fitter_householder_sign_is_reference_choice():
     """R_00 = +||column 0|| = +32 (all-ones column), i.e. u_0 = 1 - 32: with
     a single block whose colour equals 2.5 everywhere, weight 0 is 2.5 and
     the rest vanish."""
-    w, h = 33, 33
+    w, h = 47, 47
     p = bind.params(w, h, half_tmp=False)
     o = bind.OracleStream(p)
     s = o.s
@@ -339,7 +339,7 @@ def test_fitter_householder_sign_is_reference_choice():
 
 # ---- stages 3-5 (bmfr.cl:703-974) -----------------------------------------------
 def test_weighted_sum_unit_weights_pick_block_and_feature():
-    w, h = 70, 45
+    w, h = 70, 48
     sc = _scene(w, h)
     f = sc.render_host(0)
     p = bind.params(w, h)
@@ -366,7 +366,7 @@ def test_weighted_sum_unit_weights_pick_block_and_feature():
 
 
 def test_accumulate_filtered_frame0_and_static_history():
-    w, h = 40, 36
+    w, h = 48, 48
     P = w * h
     p = bind.params(w, h)
     o = bind.OracleStream(p)
@@ -393,7 +393,7 @@ def test_accumulate_filtered_frame0_and_static_history():
 
 
 def test_taa_pass_through_constant_image_and_edges():
-    w, h = 40, 36
+    w, h = 48, 48
     P = w * h
     p = bind.params(w, h)
     o = bind.OracleStream(p)
@@ -419,3 +419,17 @@ def test_taa_pass_through_constant_image_and_edges():
     o.result[o.prev()][:] = 0.0
     o.taa(1)
     assert np.allclose(o.result[o.cur()], 0.5, atol=1e-6)
+
+
+def test_minimum_image_size_is_the_mirror_validity_bound():
+    """47 is the smallest size whose padded grid (up to WORKSET+29 after the
+    largest jitter) mirrors back inside the image; the oracle refuses smaller
+    images, as the C ABI does."""
+    def valid(n):
+        ws = 32 * ((n + 31) // 32)
+        lo, hi = 0 - 16 - 16, ws + 31 - 16 + 14
+        return all(0 <= lib.orc_mirror(i, n) < n for i in (lo, hi))
+    assert [n for n in range(33, 200) if not valid(n)] == list(range(33, 47))
+    o = bind.OracleStream(bind.params(64, 46))
+    with pytest.raises(ValueError):
+        o.accumulate_noisy_data(0, np.zeros(16), np.zeros(2))

@@ -76,6 +76,26 @@ def run_plain(args):
     return t, wall, s
 
 
+def copy_peak(reps=20):
+    """Measured HBM copy bandwidth of the library's 16-byte copy kernel
+    (four 11 MB buffers per launch), the practical streaming ceiling."""
+    import bmfr_amd
+    p = bmfr_amd.default_params(W, H)
+    a, b = bmfr_amd.Stream(p), bmfr_amd.Stream(p)
+    src = [a.buffer_device_ptr(n) for n in ("albedo", "normals_cur", "positions_cur", "noisy_in")]
+    nbytes = sum(a.buffer_bytes(n) for n in ("albedo", "normals_cur", "positions_cur", "noisy_in"))
+    for _ in range(3):
+        b.copy_frame_inputs_device(*src)
+    b.record_marker(0)
+    for _ in range(reps):
+        b.copy_frame_inputs_device(*src)
+    b.record_marker(1)
+    ms = b.marker_elapsed(0, 1)
+    a.close()
+    b.close()
+    return 2.0 * nbytes * reps / (ms * 1e-3) / 1e9
+
+
 def cpu_baseline(frames=8):
     """The CPU oracle (a scalar C restatement, OpenMP over rows/blocks) on the
     host cores: 720p frames of the same scene; ms/frame over frames 1..N-1."""
@@ -121,6 +141,8 @@ def main():
     roof = {"kernel": dom, "bound": "hbm", "achieved": round(stage_gbs[dom], 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(stage_gbs[dom] / HBM_PEAK_GBS, 4),
             "traffic": None, "algorithmic_bytes": int(sb[dom])}
+    copy_gbs = copy_peak()
+    roof["copy_peak_gbs"] = round(copy_gbs, 1)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline()

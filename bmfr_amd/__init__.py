@@ -131,6 +131,12 @@ class Stream:
         arrs = [_f32(x, n) for x in (albedo, normals, positions, noisy)]
         _check(self._lib.bmfr_upload_frame(self._h, *[_fp(a) for a in arrs]))
 
+    def copy_frame_inputs_device(self, albedo, normals, positions, noisy):
+        """Device pointers (ints) of the frame's inputs; see
+        bmfr_copy_frame_inputs_device."""
+        _check(self._lib.bmfr_copy_frame_inputs_device(
+            self._h, *[ctypes.c_void_p(p) for p in (albedo, normals, positions, noisy)]))
+
     # -- the five stages (opencl/bmfr.cpp:429-476) --------------------------------
     def accumulate_noisy_data(self, frame, prev_frame_camera_matrix, pixel_offset):
         m = _f32(prev_frame_camera_matrix, 16)
@@ -216,6 +222,15 @@ class Stream:
         out = np.empty(6, np.float32)
         _check(self._lib.bmfr_stage_times(self._h, _fp(out)))
         return out
+
+
+    def record_marker(self, marker):
+        _check(self._lib.bmfr_record_marker(self._h, marker))
+
+    def marker_elapsed(self, m0, m1):
+        ms = ctypes.c_float()
+        _check(self._lib.bmfr_marker_elapsed(self._h, m0, m1, ctypes.byref(ms)))
+        return ms.value
 
 
 class Scene:

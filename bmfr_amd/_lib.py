@@ -61,6 +61,7 @@ SIGNATURES = [
     ("bmfr_stream_reset", c_int32, [ctypes.c_void_p]),
     ("bmfr_set_fitter_config", c_int32, [ctypes.c_void_p, c_int32, c_int32]),
     ("bmfr_upload_frame", c_int32, [ctypes.c_void_p, c_float_p, c_float_p, c_float_p, c_float_p]),
+    ("bmfr_copy_frame_inputs_device", c_int32, [ctypes.c_void_p] * 5),
     ("bmfr_accumulate_noisy_data", c_int32, [ctypes.c_void_p, c_int32, c_float_p, c_float_p]),
     ("bmfr_fitter", c_int32, [ctypes.c_void_p, c_int32]),
     ("bmfr_weighted_sum", c_int32, [ctypes.c_void_p, c_int32]),
@@ -76,6 +77,8 @@ SIGNATURES = [
     ("bmfr_buffer_device_ptr", c_int32, [ctypes.c_void_p, c_int32, ctypes.POINTER(ctypes.c_void_p)]),
     ("bmfr_set_timing", c_int32, [ctypes.c_void_p, c_int32]),
     ("bmfr_stage_times", c_int32, [ctypes.c_void_p, c_float_p]),
+    ("bmfr_record_marker", c_int32, [ctypes.c_void_p, c_int32]),
+    ("bmfr_marker_elapsed", c_int32, [ctypes.c_void_p, c_int32, c_int32, c_float_p]),
     ("bmfr_scene_create", c_int32, [ctypes.c_uint64, c_int32, ctypes.c_float, c_int32, c_int32,
                                     c_int32, ctypes.POINTER(ctypes.c_void_p)]),
     ("bmfr_scene_destroy", c_int32, [ctypes.c_void_p]),

@@ -47,6 +47,7 @@ int32_t fail(int32_t code, const char *fmt, ...) {
 
 constexpr int kStages = 5;
 constexpr int kMinImageSize = 47;
+constexpr int kMarkers = 8;
 
 // Physical buffers; double buffers take two slots.
 enum Slot {
@@ -138,8 +139,9 @@ struct bmfr_stream_s {
     int last_result = -1;   // slot holding the latest TAA output
     bool timing = false;
     hipEvent_t ev_start[kStages]{}, ev_end[kStages]{};
+    hipEvent_t markers[kMarkers]{};
     bool recorded[kStages]{};
-    bmfr::FitterConfig fitter{8, 1};
+    bmfr::FitterConfig fitter{16, 1};
 
     int cur(int a_slot) const { return swapped ? a_slot : a_slot + 1; }
     int prev(int a_slot) const { return swapped ? a_slot + 1 : a_slot; }
@@ -338,6 +340,7 @@ int32_t bmfr_stream_open(const bmfr_params *params, bmfr_stream *out) {
         e = hipEventCreate(&s->ev_start[i]);
         if (e == hipSuccess) e = hipEventCreate(&s->ev_end[i]);
     }
+    for (int i = 0; i < kMarkers && e == hipSuccess; ++i) e = hipEventCreate(&s->markers[i]);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) {
         bmfr_stream_close(s);
@@ -349,14 +352,16 @@ int32_t bmfr_stream_open(const bmfr_params *params, bmfr_stream *out) {
 
 int32_t bmfr_stream_close(bmfr_stream s) {
     if (!s) return BMFR_OK;
-    hipSetDevice(s->device);
-    if (s->stream) hipStreamSynchronize(s->stream);
+    (void)hipSetDevice(s->device);
+    if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (int i = 0; i < kStages; ++i) {
-        if (s->ev_start[i]) hipEventDestroy(s->ev_start[i]);
-        if (s->ev_end[i]) hipEventDestroy(s->ev_end[i]);
+        if (s->ev_start[i]) (void)hipEventDestroy(s->ev_start[i]);
+        if (s->ev_end[i]) (void)hipEventDestroy(s->ev_end[i]);
     }
-    if (s->stream) hipStreamDestroy(s->stream);
-    if (s->base) hipFree(s->base);
+    for (int i = 0; i < kMarkers; ++i)
+        if (s->markers[i]) (void)hipEventDestroy(s->markers[i]);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    if (s->base) (void)hipFree(s->base);
     delete s;
     return BMFR_OK;
 }
@@ -411,6 +416,26 @@ int32_t bmfr_upload_frame(bmfr_stream s, const float *albedo, const float *norma
             HIP_TRY(hipStreamSynchronize(s->stream));
         }
     }
+    return BMFR_OK;
+}
+
+int32_t bmfr_copy_frame_inputs_device(bmfr_stream s, const void *albedo, const void *normals,
+                                      const void *positions, const void *noisy) {
+    if (int32_t st = check_stream(s)) return st;
+    const void *src[4] = {albedo, normals, positions, noisy};
+    const int dst[4] = {S_ALBEDO, s->cur(S_NORMALS_A), s->cur(S_POSITIONS_A), S_NOISY_IN};
+    bmfr::CopyJob job{};
+    for (int i = 0; i < 4; ++i) {
+        if (!src[i]) return fail(BMFR_ERR_INVALID_ARGUMENT, "null device input %d", i);
+        if (((uintptr_t)src[i] & 15) != 0)
+            return fail(BMFR_ERR_INVALID_ARGUMENT, "device input %d is not 16-byte aligned", i);
+        job.src[i] = src[i];
+        job.dst[i] = s->slot[dst[i]];
+        job.bytes[i] = (int64_t)s->slot_bytes[dst[i]];
+    }
+    HIP_TRY(hipSetDevice(s->device));
+    hipError_t e = bmfr::launch_copy4(job, 4, s->stream);
+    if (e != hipSuccess) return fail(BMFR_ERR_HIP, "copy launch failed: %s", hipGetErrorString(e));
     return BMFR_OK;
 }
 
@@ -565,6 +590,24 @@ int32_t bmfr_stage_times(bmfr_stream s, float *ms6) {
     return BMFR_OK;
 }
 
+int32_t bmfr_record_marker(bmfr_stream s, int32_t marker) {
+    if (int32_t st = check_stream(s)) return st;
+    if (marker < 0 || marker >= kMarkers) return fail(BMFR_ERR_INVALID_ARGUMENT, "marker %d", marker);
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipEventRecord(s->markers[marker], s->stream));
+    return BMFR_OK;
+}
+
+int32_t bmfr_marker_elapsed(bmfr_stream s, int32_t m0, int32_t m1, float *ms) {
+    if (int32_t st = check_stream(s)) return st;
+    if (m0 < 0 || m0 >= kMarkers || m1 < 0 || m1 >= kMarkers || !ms)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "bad marker arguments");
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipEventSynchronize(s->markers[m1]));
+    HIP_TRY(hipEventElapsedTime(ms, s->markers[m0], s->markers[m1]));
+    return BMFR_OK;
+}
+
 // ---- scene -------------------------------------------------------------------
 int32_t bmfr_scene_create(uint64_t seed, int32_t spheres, float firefly, int32_t width, int32_t height,
                           int32_t static_camera, bmfr_scene *out) {
@@ -581,7 +624,7 @@ int32_t bmfr_scene_create(uint64_t seed, int32_t spheres, float firefly, int32_t
 
 int32_t bmfr_scene_destroy(bmfr_scene sc) {
     if (!sc) return BMFR_OK;
-    if (sc->device_copy) hipFree(sc->device_copy);
+    if (sc->device_copy) (void)hipFree(sc->device_copy);
     delete sc;
     return BMFR_OK;
 }

@@ -192,16 +192,13 @@ __global__ __launch_bounds__(64 * (16 / RPL) * BPG) void k_fitter(const FitterPa
 #pragma unroll
     for (int col = 0; col < kFeatures; ++col) {
         // Pivot row `col` sits in lane col/V (wave 0), element col%V.
-        constexpr int dummy = 0;
-        (void)dummy;
-        float r1[2];
+        float r1[1];
         {
             float s = 0.f;
 #pragma unroll
             for (int k = 0; k < RPL; ++k) {
-                const float x = a[col][k];
-                const float sq = x * x;
-                s = s + (row_of(k) > col ? sq : 0.f);
+                const float x = row_of(k) > col ? a[col][k] : 0.f;
+                s = __builtin_fmaf(x, x, s);
             }
             r1[0] = s;
         }
@@ -272,12 +269,16 @@ __global__ __launch_bounds__(64 * (16 / RPL) * BPG) void k_fitter(const FitterPa
             }
             ++round;
         }
+        // a_j -= u * (2 u.a_j / uu): one reciprocal per pivot and a fused
+        // multiply-add per element (the reference divides per element,
+        // bmfr.cl:650; the difference is within a couple of ulp).
+        const float inv_uu = 1.f / uu;
 #pragma unroll
         for (int j = col + 1; j < kBufferCount; ++j) {
-            const float fct = (2.f * d[j - col - 1]) / uu;
+            const float fct = (2.f * d[j - col - 1]) * inv_uu;
 #pragma unroll
             for (int k = 0; k < RPL; ++k) {
-                const float v = a[j][k] - u[k] * fct;
+                const float v = __builtin_fmaf(-u[k], fct, a[j][k]);
                 a[j][k] = HALF_ROUND ? round_half(v) : v;
             }
         }

@@ -52,6 +52,14 @@ struct FitterTask {
     int32_t flags;            // bit 0: tmp is half; bit 1: round to half
 };
 
+// Up to four independent byte copies in one launch (16-byte aligned bases).
+struct CopyJob {
+    const void *src[4];
+    void *dst[4];
+    int64_t bytes[4];
+};
+hipError_t launch_copy4(const CopyJob &job, int count, hipStream_t s);
+
 hipError_t launch_accumulate_noisy_data(const StageArgs &a, const StageBuffers &b, hipStream_t s);
 hipError_t launch_fitter(const StageArgs &a, const StageBuffers &b, const FitterConfig &cfg,
                          int32_t qr_mode, hipStream_t s);

@@ -140,6 +140,13 @@ int32_t bmfr_set_fitter_config(bmfr_stream stream, int32_t rows_per_lane, int32_
 int32_t bmfr_upload_frame(bmfr_stream stream, const float *albedo, const float *normals,
                           const float *positions, const float *noisy);
 
+/* Device-to-device form of bmfr_upload_frame for a renderer whose G-buffers
+ * are already in HBM: albedo as float[3P], the other three in the stream's
+ * storage type (float or half [3P]); 16-byte aligned.  Enqueued on the
+ * stream (asynchronous); the sources must stay valid until it has run. */
+int32_t bmfr_copy_frame_inputs_device(bmfr_stream stream, const void *albedo, const void *normals,
+                                      const void *positions, const void *noisy);
+
 /* ---- the five stages (opencl/bmfr.cpp:429-476) ---------------------------- */
 /* prev_frame_camera_matrix: 16 floats, the previous frame's view-projection
  * in the reference's row-vector layout (camera_matrices[max(f-1,0)],
@@ -179,6 +186,11 @@ int32_t bmfr_buffer_device_ptr(bmfr_stream stream, int32_t buffer, void **dptr);
  * (the reference's "Total time in all kernels"). */
 int32_t bmfr_set_timing(bmfr_stream stream, int32_t enable);
 int32_t bmfr_stage_times(bmfr_stream stream, float *ms6);
+
+/* Markers 0..7: HIP events recorded on the stream's HIP stream, to time any
+ * enqueued work; bmfr_marker_elapsed waits for m1 and returns ms. */
+int32_t bmfr_record_marker(bmfr_stream stream, int32_t marker);
+int32_t bmfr_marker_elapsed(bmfr_stream stream, int32_t m0, int32_t m1, float *ms);
 
 /* ---- procedural scene (replaces the EXR dataset, opencl/bmfr.cpp:145-172) ---- */
 typedef struct bmfr_scene_s *bmfr_scene;

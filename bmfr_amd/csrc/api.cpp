@@ -60,6 +60,7 @@ enum Slot {
     S_PREV_PIXEL, S_ACCEPT, S_TMP, S_WEIGHTS, S_MINS_MAXS, S_FILTERED, S_TONE,
     S_OUT_A, S_OUT_B,
     S_RESULT_A, S_RESULT_B,
+    S_NOISE,   // the fitter's per-frame feature-noise table (not a reference buffer)
     S_COUNT
 };
 
@@ -117,6 +118,7 @@ void sizes_of(const bmfr_params &p, bmfr_sizes &s, size_t *slot_bytes) {
     b[S_TONE] = 12 * P;
     b[S_OUT_A] = b[S_OUT_B] = 3 * g * P;
     b[S_RESULT_A] = b[S_RESULT_B] = 3 * g * P;
+    b[S_NOISE] = 9 * 1024 * sizeof(double);
     size_t total = 0;
     for (int i = 0; i < S_COUNT; ++i) {
         if (slot_bytes) slot_bytes[i] = b[i];
@@ -136,6 +138,7 @@ struct bmfr_stream_s {
     void *slot[S_COUNT]{};
     size_t slot_bytes[S_COUNT]{};
     bool swapped = false;   // Double_buffer::swapped
+    int noise_frame = -1;   // frame whose noise table S_NOISE holds
     int last_result = -1;   // slot holding the latest TAA output
     bool timing = false;
     hipEvent_t ev_start[kStages]{}, ev_end[kStages]{};
@@ -196,6 +199,7 @@ struct bmfr_stream_s {
         b.out_prev = slot[prev(S_OUT_A)];
         b.result_cur = slot[cur(S_RESULT_A)];
         b.result_prev = slot[prev(S_RESULT_A)];
+        b.noise = slot[S_NOISE];
         return b;
     }
 
@@ -380,6 +384,7 @@ int32_t bmfr_stream_reset(bmfr_stream s) {
     HIP_TRY(hipStreamSynchronize(s->stream));
     s->swapped = false;
     s->last_result = -1;
+    s->noise_frame = -1;
     return BMFR_OK;
 }
 
@@ -449,7 +454,9 @@ int32_t bmfr_accumulate_noisy_data(bmfr_stream s, int32_t frame, const float *ma
     a.pixel_offset_x = pixel_offset[0];
     a.pixel_offset_y = pixel_offset[1];
     const bmfr::StageBuffers b = s->buffers();
-    return timed_stage(s, 0, [&] { return bmfr::launch_accumulate_noisy_data(a, b, s->stream); });
+    int32_t st = timed_stage(s, 0, [&] { return bmfr::launch_accumulate_noisy_data(a, b, true, s->stream); });
+    if (st == BMFR_OK) s->noise_frame = frame;
+    return st;
 }
 
 int32_t bmfr_fitter(bmfr_stream s, int32_t frame) {
@@ -457,6 +464,14 @@ int32_t bmfr_fitter(bmfr_stream s, int32_t frame) {
     if (int32_t st = check_frame(frame)) return st;
     const bmfr::StageArgs a = s->args(frame);
     const bmfr::StageBuffers b = s->buffers();
+    if (s->noise_frame != frame) {
+        // fitter called without this frame's stage 1 (e.g. on tmp_data written
+        // through bmfr_write_buffer): build the noise table first
+        HIP_TRY(hipSetDevice(s->device));
+        hipError_t e = bmfr::launch_noise_table(a, b, s->stream);
+        if (e != hipSuccess) return fail(BMFR_ERR_HIP, "noise table launch failed: %s", hipGetErrorString(e));
+        s->noise_frame = frame;
+    }
     return timed_stage(s, 1, [&] { return bmfr::launch_fitter(a, b, s->fitter, s->p.qr_mode, s->stream); });
 }
 

@@ -88,9 +88,8 @@ __device__ __forceinline__ int wrap_mad(int a, int b, int c) {
     return (int)((unsigned)a * (unsigned)b + (unsigned)c);
 }
 
-// opencl/bmfr.cl:162-171 (hash) and 173-182 (feature noise, evaluated in
-// double because NOISE_AMOUNT is a double literal).  The hash argument
-// id + sub_vector*256 equals the row index within the block.
+// opencl/bmfr.cl:162-171.  The hash argument of the feature noise
+// (bmfr.cl:179-181), id + sub_vector*256, equals the row index in the block.
 __device__ __forceinline__ uint32_t hash32(uint32_t a) {
     a = (a + 0x7ed55d16u) + (a << 12);
     a = (a ^ 0xc761c23cu) ^ (a >> 19);
@@ -100,12 +99,18 @@ __device__ __forceinline__ uint32_t hash32(uint32_t a) {
     a = (a ^ 0xb55a4f09u) ^ (a >> 16);
     return a;
 }
-__device__ __forceinline__ float add_noise(float value, int row, int feature, int frame,
-                                           double noise_amount) {
+
+// The feature noise of one frame depends only on (row, feature, frame)
+// (bmfr.cl:179-181 has no block term), so one table of 9 x 1024 FP64
+// noise terms serves every block of a frame: table[(f-1)*1024 + row] =
+// NOISE_AMOUNT * 2 * (random(row + 1024 f + 13312 frame) - 0.5f), in double
+// because NOISE_AMOUNT is a double literal (bmfr.cl:179).
+constexpr int kNoiseTable = (kFeatures - 1) * kBlockPixels;
+__device__ __forceinline__ double noise_term(int row, int feature, int frame, double noise_amount) {
     const uint32_t arg = (uint32_t)row + (uint32_t)feature * kBlockPixels +
                          (uint32_t)frame * (uint32_t)(kBufferCount * kBlockPixels);
     const float r = (float)hash32(arg) / 4294967296.0f - 0.5f;
-    return (float)((double)value + noise_amount * 2.0 * (double)r);
+    return noise_amount * 2.0 * (double)r;
 }
 
 // opencl/bmfr.cl:200-205

@@ -30,6 +30,12 @@ __device__ __forceinline__ float dpp(float old, float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), CTRL,
                                                       ROW_MASK, 0xf, false));
 }
+// Full-mask DPP move: every lane is written, so the compiler can fold it
+// into the consuming VALU op (v_add_f32_dpp / v_min_f32_dpp).
+template <int CTRL>
+__device__ __forceinline__ float dpp_all(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, true));
+}
 
 __device__ __forceinline__ float readlane(float v, int lane) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
@@ -47,10 +53,10 @@ struct OpMin {
 // 64-lane reduction; the result is wave-uniform.
 template <typename Op>
 __device__ __forceinline__ float wave_reduce(float v) {
-    v = Op::f(v, dpp<0xB1>(v, v));    // quad_perm [1,0,3,2]
-    v = Op::f(v, dpp<0x4E>(v, v));    // quad_perm [2,3,0,1]
-    v = Op::f(v, dpp<0x141>(v, v));   // row_half_mirror
-    v = Op::f(v, dpp<0x140>(v, v));   // row_mirror
+    v = Op::f(v, dpp_all<0xB1>(v));    // quad_perm [1,0,3,2]
+    v = Op::f(v, dpp_all<0x4E>(v));    // quad_perm [2,3,0,1]
+    v = Op::f(v, dpp_all<0x141>(v));   // row_half_mirror
+    v = Op::f(v, dpp_all<0x140>(v));   // row_mirror
     v = Op::f(v, dpp<0x142, 0xA>(Op::identity, v));  // row_bcast15 -> rows 1, 3
     v = Op::f(v, dpp<0x143, 0xC>(Op::identity, v));  // row_bcast31 -> rows 2, 3
     return readlane(v, 63);
@@ -107,9 +113,9 @@ struct FitterParams {
     const void *tmp;
     float *mins_maxs;
     float *weights;
+    const double *noise;   // the frame's noise table (kNoiseTable terms)
     int32_t blocks;
     int32_t frame;
-    double noise_amount;
 };
 
 // RPL rows per lane, BPG blocks per work-group, TT tmp element type,
@@ -174,18 +180,45 @@ __global__ __launch_bounds__(64 * (16 / RPL) * BPG) void k_fitter(const FitterPa
             mm[2 * i + 1] = bmax[i];
         }
     }
+    // scale (bmfr.cl:200-205).  (v-min)/(max-min) is evaluated as Markstein's
+    // correctly rounded quotient: y = RN(1/d) once per feature, then
+    // q = RN(a*y), r = RN(a - q*d) (exact), q' = RN(q + r*y) = RN(a/d).
 #pragma unroll
-    for (int i = 0; i < kScaled; ++i)
+    for (int i = 0; i < kScaled; ++i) {
+        const float d = bmax[i] - bmin[i];
+        const bool divide = fabsf(d) > 1.0f;
+        const float y = 1.0f / d;
 #pragma unroll
         for (int k = 0; k < RPL; ++k) {
-            float s = scale_feature(a[kNotScaled + i][k], bmin[i], bmax[i]);
-            a[kNotScaled + i][k] = HALF_ROUND ? round_half(s) : s;
+            const float num = a[kNotScaled + i][k] - bmin[i];
+            float sc = num;
+            if (divide) {
+                const float q = num * y;
+                const float r = __builtin_fmaf(-q, d, num);
+                sc = __builtin_fmaf(r, y, q);
+            }
+            a[kNotScaled + i][k] = HALF_ROUND ? round_half(sc) : sc;
         }
-    // ---- feature noise of the first pass, bmfr.cl:623-627 ----
+    }
+    // ---- feature noise of the first pass, bmfr.cl:623-627: value + noise in
+    // FP64, rounded once to float (NOISE_AMOUNT is a double literal) ----
 #pragma unroll
-    for (int f = 1; f < kFeatures; ++f)
+    for (int f = 1; f < kFeatures; ++f) {
+        const double *nz = p.noise + (f - 1) * kBlockPixels;
 #pragma unroll
-        for (int k = 0; k < RPL; ++k) a[f][k] = add_noise(a[f][k], row_of(k), f, p.frame, p.noise_amount);
+        for (int q = 0; q < Q; ++q) {
+            double n[V];
+#pragma unroll
+            for (int e = 0; e < V; e += 2) {
+                const double2 x = *reinterpret_cast<const double2 *>(nz + q * (T * V) + t * V + e);
+                n[e] = x.x;
+                n[e + 1] = x.y;
+            }
+#pragma unroll
+            for (int e = 0; e < V; ++e)
+                a[f][q * V + e] = (float)((double)a[f][q * V + e] + n[e]);
+        }
+    }
 
     // ---- Householder QR over the 10 feature columns, bmfr.cl:549-656 ----
     float norms[kFeatures];
@@ -352,9 +385,9 @@ hipError_t launch_fitter(const StageArgs &a, const StageBuffers &b, const Fitter
     fp.tmp = b.tmp_data;
     fp.mins_maxs = (float *)b.mins_maxs;
     fp.weights = (float *)b.weights;
+    fp.noise = (const double *)b.noise;
     fp.blocks = a.blocks;
     fp.frame = a.frame;
-    fp.noise_amount = a.noise_amount;
     const bool h = a.tmp_f16 != 0;
     switch (cfg.rows_per_lane * 10 + cfg.blocks_per_group) {
         case 41: return launch_rpl<4, 1>(fp, h, s);

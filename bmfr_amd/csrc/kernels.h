@@ -35,6 +35,7 @@ struct StageBuffers {
     void *filtered, *tone_mapped;
     void *out_cur, *out_prev;
     void *result_cur, *result_prev;
+    void *noise;          // double[9*1024]: the fitter's feature noise of one frame
 };
 
 // Fitter launch shape, chosen per stream (or by the autotuner).
@@ -60,7 +61,10 @@ struct CopyJob {
 };
 hipError_t launch_copy4(const CopyJob &job, int count, hipStream_t s);
 
-hipError_t launch_accumulate_noisy_data(const StageArgs &a, const StageBuffers &b, hipStream_t s);
+// write_noise: stage 1 also fills the frame's noise table (spare threads).
+hipError_t launch_accumulate_noisy_data(const StageArgs &a, const StageBuffers &b, bool write_noise,
+                                        hipStream_t s);
+hipError_t launch_noise_table(const StageArgs &a, const StageBuffers &b, hipStream_t s);
 hipError_t launch_fitter(const StageArgs &a, const StageBuffers &b, const FitterConfig &cfg,
                          int32_t qr_mode, hipStream_t s);
 hipError_t launch_weighted_sum(const StageArgs &a, const StageBuffers &b, hipStream_t s);

@@ -21,9 +21,15 @@ __global__ __launch_bounds__(256) void k_accumulate_noisy_data(
     const GT *__restrict__ cur_noisy, const GT *__restrict__ prev_noisy,
     const uint8_t *__restrict__ prev_spp, GT *__restrict__ acc_noisy,
     float2 *__restrict__ out_prev_pixel, uint8_t *__restrict__ accept_bools,
-    uint8_t *__restrict__ cur_spp, TT *__restrict__ tmp_data) {
+    uint8_t *__restrict__ cur_spp, TT *__restrict__ tmp_data, double *__restrict__ noise) {
     const int gx = blockIdx.x * 32 + (threadIdx.x & 31);
     const int gy = blockIdx.y * 8 + (threadIdx.x >> 5);
+    if (noise) {
+        // the padded grid has at least 96x96 = 9216 work-items (W, H >= 47)
+        const int lin = (blockIdx.y * gridDim.x + blockIdx.x) * 256 + threadIdx.x;
+        if (lin < kNoiseTable)
+            noise[lin] = noise_term(lin & (kBlockPixels - 1), 1 + (lin >> 10), a.frame, a.noise_amount);
+    }
     const int W = a.width, H = a.height;
     const int2 off = block_offset(a.frame);
     const int pwx = gx - kBlock / 2 + off.x;
@@ -351,22 +357,37 @@ __global__ __launch_bounds__(256) void k_taa(const StageArgs a,
 // ---------------------------------------------------------------------------
 // Host-side launchers.
 template <typename GT, typename TT>
-static hipError_t launch_s1(const StageArgs &a, const StageBuffers &b, hipStream_t s) {
+static hipError_t launch_s1(const StageArgs &a, const StageBuffers &b, bool wn, hipStream_t s) {
     dim3 grid(a.margin_width / 32, a.margin_height / 8);
     hipLaunchKernelGGL((k_accumulate_noisy_data<GT, TT>), grid, dim3(256), 0, s, a,
                        (const GT *)b.normals_cur, (const GT *)b.normals_prev,
                        (const GT *)b.positions_cur, (const GT *)b.positions_prev,
                        (const GT *)b.noisy_in, (const GT *)b.noisy_acc_prev,
                        (const uint8_t *)b.spp_prev, (GT *)b.noisy_acc_cur, (float2 *)b.prev_pixel,
-                       (uint8_t *)b.accept, (uint8_t *)b.spp_cur, (TT *)b.tmp_data);
+                       (uint8_t *)b.accept, (uint8_t *)b.spp_cur, (TT *)b.tmp_data,
+                       wn ? (double *)b.noise : nullptr);
     return hipGetLastError();
 }
 
-hipError_t launch_accumulate_noisy_data(const StageArgs &a, const StageBuffers &b, hipStream_t s) {
+hipError_t launch_accumulate_noisy_data(const StageArgs &a, const StageBuffers &b, bool wn,
+                                        hipStream_t s) {
     if (a.storage_f16) {
-        return a.tmp_f16 ? launch_s1<__half, __half>(a, b, s) : launch_s1<__half, float>(a, b, s);
+        return a.tmp_f16 ? launch_s1<__half, __half>(a, b, wn, s) : launch_s1<__half, float>(a, b, wn, s);
     }
-    return a.tmp_f16 ? launch_s1<float, __half>(a, b, s) : launch_s1<float, float>(a, b, s);
+    return a.tmp_f16 ? launch_s1<float, __half>(a, b, wn, s) : launch_s1<float, float>(a, b, wn, s);
+}
+
+__global__ __launch_bounds__(256) void k_noise_table(int frame, double noise_amount,
+                                                     double *__restrict__ noise) {
+    const int lin = blockIdx.x * 256 + threadIdx.x;
+    if (lin < kNoiseTable)
+        noise[lin] = noise_term(lin & (kBlockPixels - 1), 1 + (lin >> 10), frame, noise_amount);
+}
+
+hipError_t launch_noise_table(const StageArgs &a, const StageBuffers &b, hipStream_t s) {
+    hipLaunchKernelGGL(k_noise_table, dim3(kNoiseTable / 256), dim3(256), 0, s, a.frame,
+                       a.noise_amount, (double *)b.noise);
+    return hipGetLastError();
 }
 
 static inline dim3 image_grid(const StageArgs &a) {

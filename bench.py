@@ -76,6 +76,75 @@ def run_plain(args):
     return t, wall, s
 
 
+BATCH_MIX = [(640, 360)] * 4 + [(1280, 720)] * 6 + [(1920, 1080)] * 4 + [(2560, 1440)] * 2
+
+
+def run_batched(steps, warmup):
+    """16 concurrent streams of mixed resolutions, one frame each per step:
+    every stage is one launch over all of them (the fitter a persistent
+    launch over all 22,988 blocks), replayed from a captured hipGraph.  The
+    renderer stand-in writes each stream's G-buffers before a step (not
+    timed); the step is timed with HIP events on the batch's HIP stream."""
+    import bmfr_amd
+    sts = [bmfr_amd.Stream(bmfr_amd.default_params(w, h)) for (w, h) in BATCH_MIX]
+    scs = [bmfr_amd.Scene(0xB3F40000 + 3 + k, SPHERES, 1e-3, w, h) for k, (w, h) in enumerate(BATCH_MIX)]
+    b = bmfr_amd.Batch(sts)
+    n = len(sts)
+    pixels = sum(w * h for (w, h) in BATCH_MIX)
+    blocks = sum(st.sizes.blocks for st in sts)
+
+    def step(f, timed, graph):
+        for st, sc in zip(sts, scs):
+            sc.render_to_stream(f, st)
+        args = [sc.frame_args(f) for sc in scs]
+        b.set_graph(graph)
+        b.run([f] * n, np.concatenate([a[0] for a in args]), np.concatenate([a[1] for a in args]),
+              timed=timed)
+
+    step(0, False, False)
+    b.synchronize()
+    tune = b.autotune(3)
+    stage = None
+    totals = []
+    for f in range(1, warmup + steps + 1):
+        timed = f > warmup
+        if f == warmup:  # one eager, per-stage timed step for the breakdown
+            step(f, True, False)
+            stage = b.stage_times()
+            continue
+        step(f, timed, True)
+        if timed:
+            totals.append(float(b.stage_times()[5]))
+    b.synchronize()
+    b.close()
+    for st in sts:
+        st.close()
+    t = np.array(totals)
+    step_ms = float(t.mean())
+    p720 = 1280 * 720
+    return {
+        "streams": n, "mix": "4x640x360 6x1280x720 4x1920x1080 2x2560x1440",
+        "pixels_per_step": pixels, "blocks_per_step": blocks, "steps": len(t),
+        "step_ms": round(step_ms, 4), "step_ms_p99": round(float(np.percentile(t, 99)), 4),
+        "ms_per_720p_frame_equiv": round(step_ms * p720 / pixels, 5),
+        "aggregate_frames_per_s": round(n / (step_ms * 1e-3), 1),
+        "mpixel_per_s": round(pixels / (step_ms * 1e-3) / 1e6, 1),
+        "stage_ms": {k: round(float(v), 4) for k, v in zip(STAGES, stage[:5])},
+        "graph": True, "autotune": tune,
+    }
+
+
+def measured_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the calibrated rocprofv3 PMC
+    runs summarised in profiles/ (null when that summary is absent)."""
+    path = os.path.join(ROOT, "profiles", "traffic_plain_720p.json")
+    try:
+        d = json.load(open(path))["bytes_per_dispatch"][kernel]
+        return int(d["read"] + d["write"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def copy_peak(reps=20):
     """Measured HBM copy bandwidth of the library's 16-byte copy kernel
     (four 11 MB buffers per launch), the practical streaming ceiling."""
@@ -125,6 +194,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--fitter-config", default="")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=40)
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -140,7 +211,11 @@ def main():
     dom = max(STAGES, key=lambda k: stage_ms[k])
     roof = {"kernel": dom, "bound": "hbm", "achieved": round(stage_gbs[dom], 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(stage_gbs[dom] / HBM_PEAK_GBS, 4),
-            "traffic": None, "algorithmic_bytes": int(sb[dom])}
+            "traffic": measured_traffic(dom), "algorithmic_bytes": int(sb[dom])}
+    batched = None
+    if not args.no_batched:
+        batched = run_batched(args.batched_steps, 5)
+        batched["speedup_vs_plain_720p"] = round(ms_frame / batched["ms_per_720p_frame_equiv"], 2)
     copy_gbs = copy_peak()
     roof["copy_peak_gbs"] = round(copy_gbs, 1)
     cpu = None
@@ -167,6 +242,7 @@ def main():
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "stage_gbs": {k: round(v, 1) for k, v in stage_gbs.items()},
         "roofline": roof,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -233,6 +233,68 @@ class Stream:
         return ms.value
 
 
+class Batch:
+    """Several streams stepped together: one launch per stage over all of
+    them (see bmfr_batch_* in include/bmfr_hip.h)."""
+
+    def __init__(self, streams):
+        self._lib = _lib.load()
+        self.streams = list(streams)
+        arr = (ctypes.c_void_p * len(self.streams))(*[s.handle for s in self.streams])
+        h = ctypes.c_void_p()
+        _check(self._lib.bmfr_batch_create(arr, len(self.streams), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            _check(self._lib.bmfr_batch_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, frames, matrices, offsets, timed=False):
+        n = len(self.streams)
+        fr = np.ascontiguousarray(frames, np.int32)
+        m = _f32(matrices, 16 * n)
+        o = _f32(offsets, 2 * n)
+        if fr.size != n:
+            raise ValueError("one frame number per stream")
+        _check(self._lib.bmfr_batch_run(self._h, fr.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                        _fp(m), _fp(o), 1 if timed else 0))
+
+    def synchronize(self):
+        _check(self._lib.bmfr_batch_synchronize(self._h))
+
+    def set_graph(self, enable=True):
+        _check(self._lib.bmfr_batch_set_graph(self._h, 1 if enable else 0))
+
+    def set_fitter_config(self, rows_per_lane, blocks_per_group, grid_cap=0):
+        _check(self._lib.bmfr_batch_set_fitter_config(self._h, rows_per_lane, blocks_per_group,
+                                                      grid_cap))
+
+    def stage_times(self):
+        out = np.empty(6, np.float32)
+        _check(self._lib.bmfr_batch_stage_times(self._h, _fp(out)))
+        return out
+
+    def autotune(self, reps=3):
+        buf = ctypes.create_string_buffer(4096)
+        _check(self._lib.bmfr_batch_autotune(self._h, reps, buf, len(buf)))
+        return buf.value.decode()
+
+    def record_marker(self, marker):
+        _check(self._lib.bmfr_batch_record_marker(self._h, marker))
+
+    def marker_elapsed(self, m0, m1):
+        ms = ctypes.c_float()
+        _check(self._lib.bmfr_batch_marker_elapsed(self._h, m0, m1, ctypes.byref(ms)))
+        return ms.value
+
+
 class Scene:
     """Seeded procedural plane+spheres scene (the input producer)."""
 

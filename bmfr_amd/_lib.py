@@ -79,6 +79,18 @@ SIGNATURES = [
     ("bmfr_stage_times", c_int32, [ctypes.c_void_p, c_float_p]),
     ("bmfr_record_marker", c_int32, [ctypes.c_void_p, c_int32]),
     ("bmfr_marker_elapsed", c_int32, [ctypes.c_void_p, c_int32, c_int32, c_float_p]),
+    ("bmfr_batch_create", c_int32, [ctypes.POINTER(ctypes.c_void_p), c_int32,
+                                    ctypes.POINTER(ctypes.c_void_p)]),
+    ("bmfr_batch_destroy", c_int32, [ctypes.c_void_p]),
+    ("bmfr_batch_run", c_int32, [ctypes.c_void_p, ctypes.POINTER(c_int32), c_float_p, c_float_p,
+                                 c_int32]),
+    ("bmfr_batch_synchronize", c_int32, [ctypes.c_void_p]),
+    ("bmfr_batch_set_graph", c_int32, [ctypes.c_void_p, c_int32]),
+    ("bmfr_batch_set_fitter_config", c_int32, [ctypes.c_void_p, c_int32, c_int32, c_int32]),
+    ("bmfr_batch_stage_times", c_int32, [ctypes.c_void_p, c_float_p]),
+    ("bmfr_batch_autotune", c_int32, [ctypes.c_void_p, c_int32, ctypes.c_char_p, ctypes.c_size_t]),
+    ("bmfr_batch_record_marker", c_int32, [ctypes.c_void_p, c_int32]),
+    ("bmfr_batch_marker_elapsed", c_int32, [ctypes.c_void_p, c_int32, c_int32, c_float_p]),
     ("bmfr_scene_create", c_int32, [ctypes.c_uint64, c_int32, ctypes.c_float, c_int32, c_int32,
                                     c_int32, ctypes.POINTER(ctypes.c_void_p)]),
     ("bmfr_scene_destroy", c_int32, [ctypes.c_void_p]),

@@ -143,6 +143,7 @@ struct bmfr_stream_s {
     bool timing = false;
     hipEvent_t ev_start[kStages]{}, ev_end[kStages]{};
     hipEvent_t markers[kMarkers]{};
+    hipEvent_t sync_ev = nullptr;   // orders this stream against batch launches
     bool recorded[kStages]{};
     bmfr::FitterConfig fitter{16, 1};
 
@@ -256,6 +257,8 @@ int32_t timed_stage(bmfr_stream s, int stage, F &&launch) {
 
 }  // namespace
 
+extern "C" int32_t bmfr_batch_destroy(bmfr_batch b);
+
 extern "C" {
 
 int32_t bmfr_api_version(void) { return BMFR_API_VERSION; }
@@ -345,6 +348,7 @@ int32_t bmfr_stream_open(const bmfr_params *params, bmfr_stream *out) {
         if (e == hipSuccess) e = hipEventCreate(&s->ev_end[i]);
     }
     for (int i = 0; i < kMarkers && e == hipSuccess; ++i) e = hipEventCreate(&s->markers[i]);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&s->sync_ev, hipEventDisableTiming);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) {
         bmfr_stream_close(s);
@@ -364,6 +368,7 @@ int32_t bmfr_stream_close(bmfr_stream s) {
     }
     for (int i = 0; i < kMarkers; ++i)
         if (s->markers[i]) (void)hipEventDestroy(s->markers[i]);
+    if (s->sync_ev) (void)hipEventDestroy(s->sync_ev);
     if (s->stream) (void)hipStreamDestroy(s->stream);
     if (s->base) (void)hipFree(s->base);
     delete s;
@@ -622,6 +627,404 @@ int32_t bmfr_marker_elapsed(bmfr_stream s, int32_t m0, int32_t m1, float *ms) {
     HIP_TRY(hipEventElapsedTime(ms, s->markers[m0], s->markers[m1]));
     return BMFR_OK;
 }
+
+}  // extern "C"
+
+// ---- batches ---------------------------------------------------------------
+namespace {
+
+struct BatchGroup {
+    bool g16 = false, t16 = false;
+    std::vector<int> members;            // indices into bmfr_batch_s::streams
+    bmfr::BatchStream *d_streams = nullptr;
+    bmfr::BatchStep *d_steps = nullptr;
+    int32_t *d_starts = nullptr;
+    int32_t starts[bmfr::kStartKinds][bmfr::kMaxBatch + 1]{};
+    bmfr::BatchLaunch launch() const {
+        bmfr::BatchLaunch L;
+        L.streams = d_streams;
+        L.steps = d_steps;
+        L.starts = d_starts;
+        L.n = (int32_t)members.size();
+        return L;
+    }
+};
+
+}  // namespace
+
+struct bmfr_batch_s {
+    std::vector<bmfr_stream> streams;
+    std::vector<BatchGroup> groups;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    hipEvent_t ev_start[kStages]{}, ev_end[kStages]{};
+    bool recorded = false;
+    bool recorded_total_only = false;
+    bmfr::FitterConfig fitter{16, 1};
+    int grid_cap = 0;
+    bool use_graph = false;
+    hipGraphExec_t graph = nullptr;
+    // pinned staging of the per-step parameters, one slot per group and ring entry
+    static constexpr int kRing = 4;
+    bmfr::BatchStep *h_steps = nullptr;    // [kRing][groups][kMaxBatch]
+    hipEvent_t ring_ev[kRing]{};
+    int ring = 0;
+};
+
+namespace {
+
+int32_t batch_enqueue_stages(bmfr_batch b, bool timed) {
+    // the five stages, each as one launch per storage group
+    for (int st = 0; st < kStages; ++st) {
+        if (timed) HIP_TRY(hipEventRecord(b->ev_start[st], b->stream));
+        for (auto &g : b->groups) {
+            const bmfr::BatchLaunch L = g.launch();
+            hipError_t e = hipSuccess;
+            switch (st) {
+                case 0:
+                    e = bmfr::launch_batch_accumulate_noisy_data(
+                        L, g.starts[bmfr::kStartsPadded][L.n], g.g16, g.t16, b->stream);
+                    break;
+                case 1:
+                    e = bmfr::launch_batch_fitter(L, g.starts[bmfr::kStartsBlocks][L.n], g.t16, b->fitter,
+                                                  b->grid_cap, b->stream);
+                    break;
+                case 2:
+                    e = bmfr::launch_batch_weighted_sum(L, g.starts[bmfr::kStartsImage][L.n], g.g16, b->stream);
+                    break;
+                case 3:
+                    e = bmfr::launch_batch_accumulate_filtered_data(L, g.starts[bmfr::kStartsImage][L.n],
+                                                                    g.g16, b->stream);
+                    break;
+                default:
+                    e = bmfr::launch_batch_taa(L, g.starts[bmfr::kStartsImage][L.n], g.g16, b->stream);
+                    break;
+            }
+            if (e != hipSuccess) return fail(BMFR_ERR_HIP, "batched stage %d launch failed: %s", st, hipGetErrorString(e));
+        }
+        if (timed) HIP_TRY(hipEventRecord(b->ev_end[st], b->stream));
+    }
+    return BMFR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t bmfr_batch_create(const bmfr_stream *streams, int32_t count, bmfr_batch *out) {
+    if (!streams || !out) return fail(BMFR_ERR_INVALID_ARGUMENT, "null argument");
+    if (count < 1 || count > bmfr::kMaxBatch)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "batch of %d streams outside 1..%d", count, bmfr::kMaxBatch);
+    *out = nullptr;
+    for (int i = 0; i < count; ++i) {
+        if (!streams[i]) return fail(BMFR_ERR_INVALID_ARGUMENT, "stream %d is null", i);
+        if (streams[i]->device != streams[0]->device)
+            return fail(BMFR_ERR_INVALID_ARGUMENT, "streams of a batch must share one device");
+        for (int j = 0; j < i; ++j)
+            if (streams[j] == streams[i]) return fail(BMFR_ERR_INVALID_ARGUMENT, "stream %d listed twice", i);
+    }
+    bmfr_batch b = new (std::nothrow) bmfr_batch_s;
+    if (!b) return fail(BMFR_ERR_INVALID_ARGUMENT, "host allocation failed");
+    b->streams.assign(streams, streams + count);
+    b->device = streams[0]->device;
+    for (int i = 0; i < count; ++i) {
+        const bool g16 = streams[i]->p.storage == BMFR_STORAGE_F16;
+        const bool t16 = streams[i]->p.tmp_storage == BMFR_STORAGE_F16;
+        BatchGroup *grp = nullptr;
+        for (auto &g : b->groups)
+            if (g.g16 == g16 && g.t16 == t16) grp = &g;
+        if (!grp) {
+            b->groups.emplace_back();
+            grp = &b->groups.back();
+            grp->g16 = g16;
+            grp->t16 = t16;
+        }
+        grp->members.push_back(i);
+    }
+    hipError_t e = hipSetDevice(b->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
+    for (int i = 0; i < kStages && e == hipSuccess; ++i) {
+        e = hipEventCreate(&b->ev_start[i]);
+        if (e == hipSuccess) e = hipEventCreate(&b->ev_end[i]);
+    }
+    for (int i = 0; i < bmfr_batch_s::kRing && e == hipSuccess; ++i)
+        e = hipEventCreateWithFlags(&b->ring_ev[i], hipEventDisableTiming);
+    if (e == hipSuccess)
+        e = hipHostMalloc((void **)&b->h_steps, sizeof(bmfr::BatchStep) * bmfr_batch_s::kRing *
+                                                    b->groups.size() * bmfr::kMaxBatch);
+    for (auto &g : b->groups) {
+        if (e != hipSuccess) break;
+        const int n = (int)g.members.size();
+        std::vector<bmfr::BatchStream> hs(n);
+        for (int k = 0; k < 3; ++k) g.starts[k][0] = 0;
+        for (int m = 0; m < n; ++m) {
+            bmfr_stream s = b->streams[g.members[m]];
+            bmfr::BatchStream &d = hs[m];
+            d.args = s->args(0);
+            d.albedo = s->slot[S_ALBEDO];
+            d.noisy_in = s->slot[S_NOISY_IN];
+            d.prev_pixel = s->slot[S_PREV_PIXEL];
+            d.accept = s->slot[S_ACCEPT];
+            d.tmp_data = s->slot[S_TMP];
+            d.weights = s->slot[S_WEIGHTS];
+            d.mins_maxs = s->slot[S_MINS_MAXS];
+            d.filtered = s->slot[S_FILTERED];
+            d.tone_mapped = s->slot[S_TONE];
+            d.noise = s->slot[S_NOISE];
+            for (int h = 0; h < 2; ++h) {
+                d.normals[h] = s->slot[S_NORMALS_A + h];
+                d.positions[h] = s->slot[S_POSITIONS_A + h];
+                d.noisy_acc[h] = s->slot[S_NOISY_ACC_A + h];
+                d.spp[h] = s->slot[S_SPP_A + h];
+                d.out[h] = s->slot[S_OUT_A + h];
+                d.result[h] = s->slot[S_RESULT_A + h];
+            }
+            const bmfr_sizes &z = s->sz;
+            g.starts[bmfr::kStartsPadded][m + 1] =
+                g.starts[bmfr::kStartsPadded][m] + (z.margin_width / 32) * (z.margin_height / 8);
+            g.starts[bmfr::kStartsImage][m + 1] =
+                g.starts[bmfr::kStartsImage][m] + ((s->p.width + 63) / 64) * ((s->p.height + 3) / 4);
+            g.starts[bmfr::kStartsBlocks][m + 1] = g.starts[bmfr::kStartsBlocks][m] + z.blocks;
+        }
+        e = hipMalloc((void **)&g.d_streams, sizeof(bmfr::BatchStream) * bmfr::kMaxBatch);
+        if (e == hipSuccess) e = hipMalloc((void **)&g.d_steps, sizeof(bmfr::BatchStep) * bmfr::kMaxBatch);
+        if (e == hipSuccess) e = hipMalloc((void **)&g.d_starts, sizeof(g.starts));
+        if (e == hipSuccess)
+            e = hipMemcpy(g.d_streams, hs.data(), sizeof(bmfr::BatchStream) * n, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(g.d_starts, g.starts, sizeof(g.starts), hipMemcpyHostToDevice);
+    }
+    if (e != hipSuccess) {
+        bmfr_batch_destroy(b);
+        return fail(BMFR_ERR_HIP, "batch set-up failed: %s", hipGetErrorString(e));
+    }
+    *out = b;
+    return BMFR_OK;
+}
+
+int32_t bmfr_batch_destroy(bmfr_batch b) {
+    if (!b) return BMFR_OK;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    for (auto &g : b->groups) {
+        if (g.d_streams) (void)hipFree(g.d_streams);
+        if (g.d_steps) (void)hipFree(g.d_steps);
+        if (g.d_starts) (void)hipFree(g.d_starts);
+    }
+    for (int i = 0; i < kStages; ++i) {
+        if (b->ev_start[i]) (void)hipEventDestroy(b->ev_start[i]);
+        if (b->ev_end[i]) (void)hipEventDestroy(b->ev_end[i]);
+    }
+    for (auto &ev : b->ring_ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (b->done) (void)hipEventDestroy(b->done);
+    if (b->h_steps) (void)hipHostFree(b->h_steps);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return BMFR_OK;
+}
+
+int32_t bmfr_batch_set_fitter_config(bmfr_batch b, int32_t rows_per_lane, int32_t blocks_per_group,
+                                     int32_t grid_cap) {
+    if (!b) return fail(BMFR_ERR_INVALID_ARGUMENT, "batch is null");
+    const int key = rows_per_lane * 10 + blocks_per_group;
+    if (key != 41 && key != 42 && key != 81 && key != 82 && key != 161 && key != 162 && key != 164)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "unsupported fitter config %d/%d", rows_per_lane,
+                    blocks_per_group);
+    if (grid_cap < 0) return fail(BMFR_ERR_INVALID_ARGUMENT, "grid_cap < 0");
+    b->fitter = {rows_per_lane, blocks_per_group};
+    b->grid_cap = grid_cap;
+    if (b->graph) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    return BMFR_OK;
+}
+
+int32_t bmfr_batch_set_graph(bmfr_batch b, int32_t enable) {
+    if (!b) return fail(BMFR_ERR_INVALID_ARGUMENT, "batch is null");
+    b->use_graph = enable != 0;
+    return BMFR_OK;
+}
+
+int32_t bmfr_batch_run(bmfr_batch b, const int32_t *frames, const float *matrices, const float *offsets,
+                       int32_t timed) {
+    if (!b || !frames || !matrices || !offsets) return fail(BMFR_ERR_INVALID_ARGUMENT, "null argument");
+    const int n = (int)b->streams.size();
+    for (int i = 0; i < n; ++i)
+        if (frames[i] < 0) return fail(BMFR_ERR_INVALID_ARGUMENT, "frame %d of stream %d < 0", frames[i], i);
+    HIP_TRY(hipSetDevice(b->device));
+    // 1. order after every member's pending work (uploads, renders)
+    for (bmfr_stream s : b->streams) {
+        HIP_TRY(hipEventRecord(s->sync_ev, s->stream));
+        HIP_TRY(hipStreamWaitEvent(b->stream, s->sync_ev, 0));
+    }
+    // 2. per-step parameters through a pinned ring slot
+    const int slot = b->ring;
+    b->ring = (b->ring + 1) % bmfr_batch_s::kRing;
+    HIP_TRY(hipEventSynchronize(b->ring_ev[slot]));
+    for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+        BatchGroup &g = b->groups[gi];
+        bmfr::BatchStep *hs = b->h_steps + ((size_t)slot * b->groups.size() + gi) * bmfr::kMaxBatch;
+        for (size_t m = 0; m < g.members.size(); ++m) {
+            const int i = g.members[m];
+            bmfr_stream s = b->streams[i];
+            hs[m].frame = frames[i];
+            hs[m].cur = s->swapped ? 0 : 1;  // Double_buffer::current
+            std::memcpy(hs[m].matrix, matrices + 16 * i, sizeof(hs[m].matrix));
+            hs[m].pixel_offset_x = offsets[2 * i];
+            hs[m].pixel_offset_y = offsets[2 * i + 1];
+        }
+        HIP_TRY(hipMemcpyAsync(g.d_steps, hs, sizeof(bmfr::BatchStep) * g.members.size(),
+                               hipMemcpyHostToDevice, b->stream));
+    }
+    HIP_TRY(hipEventRecord(b->ring_ev[slot], b->stream));
+    // 3. the five batched stages (captured once into a graph when enabled)
+    if (b->use_graph && !timed) {
+        if (!b->graph) {
+            hipGraph_t graph;
+            HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            int32_t st = batch_enqueue_stages(b, false);
+            hipError_t e = hipStreamEndCapture(b->stream, &graph);
+            if (st) return st;
+            if (e != hipSuccess) return fail(BMFR_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+            e = hipGraphInstantiate(&b->graph, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) return fail(BMFR_ERR_HIP, "graph instantiate failed: %s", hipGetErrorString(e));
+        }
+        if (timed) HIP_TRY(hipEventRecord(b->ev_start[0], b->stream));
+        HIP_TRY(hipGraphLaunch(b->graph, b->stream));
+        if (timed) HIP_TRY(hipEventRecord(b->ev_end[kStages - 1], b->stream));
+        b->recorded = timed != 0;
+        b->recorded_total_only = true;
+    } else {
+        if (int32_t st = batch_enqueue_stages(b, timed != 0)) return st;
+        b->recorded = timed != 0;
+        b->recorded_total_only = false;
+    }
+    // 4. members' later work waits for the batch; swap their double buffers
+    HIP_TRY(hipEventRecord(b->done, b->stream));
+    for (int i = 0; i < n; ++i) {
+        bmfr_stream s = b->streams[i];
+        HIP_TRY(hipStreamWaitEvent(s->stream, b->done, 0));
+        s->last_result = s->cur(S_RESULT_A);
+        s->noise_frame = frames[i];
+        s->swapped = !s->swapped;
+    }
+    return BMFR_OK;
+}
+
+int32_t bmfr_batch_synchronize(bmfr_batch b) {
+    if (!b) return fail(BMFR_ERR_INVALID_ARGUMENT, "batch is null");
+    HIP_TRY(hipSetDevice(b->device));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return BMFR_OK;
+}
+
+int32_t bmfr_batch_stage_times(bmfr_batch b, float *ms6) {
+    if (!b || !ms6) return fail(BMFR_ERR_INVALID_ARGUMENT, "null argument");
+    if (!b->recorded) return fail(BMFR_ERR_STATE, "the last batch step was not timed");
+    HIP_TRY(hipSetDevice(b->device));
+    HIP_TRY(hipEventSynchronize(b->ev_end[kStages - 1]));
+    for (int i = 0; i < kStages; ++i) {
+        if (b->recorded_total_only) ms6[i] = -1.f;
+        else HIP_TRY(hipEventElapsedTime(&ms6[i], b->ev_start[i], b->ev_end[i]));
+    }
+    HIP_TRY(hipEventElapsedTime(&ms6[5], b->ev_start[0], b->ev_end[kStages - 1]));
+    return BMFR_OK;
+}
+
+int32_t bmfr_batch_autotune(bmfr_batch b, int32_t reps, char *log, size_t log_len) {
+    if (!b) return fail(BMFR_ERR_INVALID_ARGUMENT, "batch is null");
+    if (reps < 1) return fail(BMFR_ERR_INVALID_ARGUMENT, "reps < 1");
+    HIP_TRY(hipSetDevice(b->device));
+    int cus = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
+    struct Cand {
+        int rpl, bpg, cap;
+    };
+    std::vector<Cand> cands;
+    for (Cand c : {Cand{16, 1, 0}, Cand{16, 2, 0}, Cand{16, 4, 0}, Cand{8, 1, 0}, Cand{8, 2, 0},
+                   Cand{4, 1, 0}, Cand{4, 2, 0}})
+        cands.push_back(c);
+    for (int mult : {4, 8}) {  // persistent grids: a few waves per SIMD
+        cands.push_back({16, 1, cus * 4 * mult / 4});
+        cands.push_back({8, 2, cus * mult / 4});
+    }
+    std::string text = "fitter autotune (us per batched launch, min of reps):";
+    float best = 1e30f;
+    Cand bestc = cands[0];
+    const bmfr::FitterConfig saved = b->fitter;
+    const int saved_cap = b->grid_cap;
+    for (const Cand &c : cands) {
+        b->fitter = {c.rpl, c.bpg};
+        b->grid_cap = c.cap;
+        float tmin = 1e30f;
+        for (int r = 0; r < reps + 1; ++r) {
+            HIP_TRY(hipEventRecord(b->ev_start[1], b->stream));
+            for (auto &g : b->groups) {
+                hipError_t e = bmfr::launch_batch_fitter(g.launch(), g.starts[bmfr::kStartsBlocks][g.members.size()],
+                                                         g.t16, b->fitter, b->grid_cap, b->stream);
+                if (e != hipSuccess) {
+                    b->fitter = saved;
+                    b->grid_cap = saved_cap;
+                    return fail(BMFR_ERR_HIP, "autotune launch failed: %s", hipGetErrorString(e));
+                }
+            }
+            HIP_TRY(hipEventRecord(b->ev_end[1], b->stream));
+            HIP_TRY(hipEventSynchronize(b->ev_end[1]));
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, b->ev_start[1], b->ev_end[1]));
+            if (r > 0) tmin = std::min(tmin, ms);  // first run warms up
+        }
+        char buf[96];
+        snprintf(buf, sizeof(buf), " rpl%d/bpg%d/cap%d=%.1f", c.rpl, c.bpg, c.cap, tmin * 1000.f);
+        text += buf;
+        if (tmin < best) {
+            best = tmin;
+            bestc = c;
+        }
+    }
+    b->fitter = {bestc.rpl, bestc.bpg};
+    b->grid_cap = bestc.cap;
+    if (b->graph) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    b->recorded = false;
+    char buf[96];
+    snprintf(buf, sizeof(buf), "; chosen rpl%d/bpg%d/cap%d", bestc.rpl, bestc.bpg, bestc.cap);
+    text += buf;
+    if (log && log_len > 0) {
+        std::strncpy(log, text.c_str(), log_len - 1);
+        log[log_len - 1] = 0;
+    }
+    return BMFR_OK;
+}
+
+int32_t bmfr_batch_record_marker(bmfr_batch b, int32_t marker) {
+    if (!b) return fail(BMFR_ERR_INVALID_ARGUMENT, "batch is null");
+    if (marker < 0 || marker >= kStages) return fail(BMFR_ERR_INVALID_ARGUMENT, "marker %d", marker);
+    HIP_TRY(hipSetDevice(b->device));
+    HIP_TRY(hipEventRecord(b->ev_start[marker], b->stream));
+    return BMFR_OK;
+}
+
+int32_t bmfr_batch_marker_elapsed(bmfr_batch b, int32_t m0, int32_t m1, float *ms) {
+    if (!b || !ms || m0 < 0 || m0 >= kStages || m1 < 0 || m1 >= kStages)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "bad marker arguments");
+    HIP_TRY(hipSetDevice(b->device));
+    HIP_TRY(hipEventSynchronize(b->ev_start[m1]));
+    HIP_TRY(hipEventElapsedTime(ms, b->ev_start[m0], b->ev_start[m1]));
+    b->recorded = false;
+    return BMFR_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
 
 // ---- scene -------------------------------------------------------------------
 int32_t bmfr_scene_create(uint64_t seed, int32_t spheres, float firefly, int32_t width, int32_t height,

@@ -118,26 +118,28 @@ struct FitterParams {
     int32_t frame;
 };
 
-// RPL rows per lane, BPG blocks per work-group, TT tmp element type,
-// HALF_ROUND: round every stored element to binary16 like the reference's
-// USE_HALF_PRECISION_IN_TMP_DATA=1 (bmfr.cl:255-258, 540, 651).
+// One 32x32 block: RPL rows per lane, BPG blocks per work-group, TT tmp
+// element type, HALF_ROUND: round every stored element to binary16 like the
+// reference's USE_HALF_PRECISION_IN_TMP_DATA=1 (bmfr.cl:255-258, 540, 651).
+// `red` is this work-group's LDS exchange area; `active` false recomputes a
+// valid block (keeping barriers uniform) without storing.
 template <int RPL, int BPG, typename TT, bool HALF_ROUND>
-__global__ __launch_bounds__(64 * (16 / RPL) * BPG) void k_fitter(const FitterParams p) {
+__device__ __forceinline__ void fitter_block(const TT *__restrict__ src, float *__restrict__ mins_maxs,
+                                             float *__restrict__ weights, const double *__restrict__ noise,
+                                             bool active, float (&red)[2][BPG][16 / RPL][16]) {
     constexpr int W = 16 / RPL;            // waves per block
     constexpr int T = 64 * W;              // lanes per block
     constexpr int V = (RPL < (int)(16 / sizeof(TT))) ? RPL : (int)(16 / sizeof(TT));
     constexpr int Q = RPL / V;             // vector loads per feature
-    __shared__ float red[2][BPG][W][16];
 
     const int tid = threadIdx.x;
     const int sub = tid / T;
     const int t = tid - sub * T;
     const int lane = tid & 63;
     const int wave = t >> 6;
-    const int blk_raw = blockIdx.x * BPG + sub;
-    const bool active = blk_raw < p.blocks;
-    const int blk = active ? blk_raw : p.blocks - 1;
-    const TT *src = reinterpret_cast<const TT *>(p.tmp) + (int64_t)blk * (kBufferCount * kBlockPixels);
+    struct {
+        const double *noise;
+    } p = {noise};
 
     // Row held in element k of this lane: rows t*V..t*V+V-1 of each T*V strip.
     auto row_of = [&](int k) { return (k / V) * (T * V) + t * V + (k % V); };
@@ -173,7 +175,7 @@ __global__ __launch_bounds__(64 * (16 / RPL) * BPG) void k_fitter(const FitterPa
         }
     }
     if (active && t == 0) {
-        float *mm = p.mins_maxs + (int64_t)blk * (kScaled * 2);
+        float *mm = mins_maxs;
 #pragma unroll
         for (int i = 0; i < kScaled; ++i) {
             mm[2 * i + 0] = bmin[i];
@@ -360,10 +362,50 @@ __global__ __launch_bounds__(64 * (16 / RPL) * BPG) void k_fitter(const FitterPa
             }
         }
         if (active && cl == 10 && ch < 3) {
-            float *w = p.weights + (int64_t)blk * (kFeatures * 3);
+            float *w = weights;
 #pragma unroll
             for (int r = 0; r < kFeatures; ++r) w[r * 3 + ch] = rc[r];
         }
+    }
+}
+
+template <int RPL, int BPG, typename TT, bool HALF_ROUND>
+__global__ __launch_bounds__(64 * (16 / RPL) * BPG) void k_fitter(const FitterParams p) {
+    __shared__ float red[2][BPG][16 / RPL][16];
+    const int sub = threadIdx.x / (64 * (16 / RPL));
+    const int blk_raw = blockIdx.x * BPG + sub;
+    const bool active = blk_raw < p.blocks;
+    const int blk = active ? blk_raw : p.blocks - 1;
+    fitter_block<RPL, BPG, TT, HALF_ROUND>(
+        reinterpret_cast<const TT *>(p.tmp) + (int64_t)blk * (kBufferCount * kBlockPixels),
+        p.mins_maxs + (int64_t)blk * (kScaled * 2), p.weights + (int64_t)blk * (kFeatures * 3),
+        p.noise, active, red);
+}
+
+// Persistent batched fitter: the work-groups stride over the concatenated
+// block list of every stream in the batch (edge blocks included).  The trip
+// count depends only on blockIdx, so barriers stay uniform.
+template <int RPL, int BPG, typename TT, bool HALF_ROUND>
+__global__ __launch_bounds__(64 * (16 / RPL) * BPG) void kb_fitter(const BatchLaunch L) {
+    __shared__ float red[2][BPG][16 / RPL][16];
+    const int32_t *starts = L.starts + kStartsBlocks * (kMaxBatch + 1);
+    const int total = starts[L.n];
+    const int sub = threadIdx.x / (64 * (16 / RPL));
+    for (int g = blockIdx.x; g * BPG < total; g += gridDim.x) {
+        const int raw = g * BPG + sub;
+        const bool active = raw < total;
+        const int u = active ? raw : total - 1;
+        int s = 0;
+        while (s + 1 < L.n && starts[s + 1] <= u) ++s;
+        const int blk = u - starts[s];
+        const BatchStream &bs = L.streams[s];
+        fitter_block<RPL, BPG, TT, HALF_ROUND>(
+            reinterpret_cast<const TT *>(bs.tmp_data) + (int64_t)blk * (kBufferCount * kBlockPixels),
+            (float *)bs.mins_maxs + (int64_t)blk * (kScaled * 2),
+            (float *)bs.weights + (int64_t)blk * (kFeatures * 3), (const double *)bs.noise, active, red);
+        // A block ends with round 20 in LDS buffer 0 and the next one starts
+        // with round 0 in the same buffer: wait until every wave has read it.
+        if constexpr (16 / RPL > 1) __syncthreads();
     }
 }
 
@@ -375,6 +417,19 @@ static hipError_t launch_rpl(const FitterParams &fp, bool tmp_f16, hipStream_t s
         hipLaunchKernelGGL((k_fitter<RPL, BPG, __half, true>), grid, dim3(threads), 0, s, fp);
     else
         hipLaunchKernelGGL((k_fitter<RPL, BPG, float, false>), grid, dim3(threads), 0, s, fp);
+    return hipGetLastError();
+}
+
+template <int RPL, int BPG>
+static hipError_t launch_batch_rpl(const BatchLaunch &L, int total, bool tmp_f16, int grid_cap,
+                                   hipStream_t s) {
+    constexpr int threads = 64 * (16 / RPL) * BPG;
+    int groups = (total + BPG - 1) / BPG;
+    if (grid_cap > 0 && groups > grid_cap) groups = grid_cap;
+    if (tmp_f16)
+        hipLaunchKernelGGL((kb_fitter<RPL, BPG, __half, true>), dim3(groups), dim3(threads), 0, s, L);
+    else
+        hipLaunchKernelGGL((kb_fitter<RPL, BPG, float, false>), dim3(groups), dim3(threads), 0, s, L);
     return hipGetLastError();
 }
 
@@ -397,6 +452,21 @@ hipError_t launch_fitter(const StageArgs &a, const StageBuffers &b, const Fitter
         case 161: return launch_rpl<16, 1>(fp, h, s);
         case 162: return launch_rpl<16, 2>(fp, h, s);
         case 164: return launch_rpl<16, 4>(fp, h, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_batch_fitter(const BatchLaunch &L, int total, bool t16, const FitterConfig &cfg,
+                               int grid_cap, hipStream_t s) {
+    if (total <= 0) return hipErrorInvalidValue;
+    switch (cfg.rows_per_lane * 10 + cfg.blocks_per_group) {
+        case 41: return launch_batch_rpl<4, 1>(L, total, t16, grid_cap, s);
+        case 42: return launch_batch_rpl<4, 2>(L, total, t16, grid_cap, s);
+        case 81: return launch_batch_rpl<8, 1>(L, total, t16, grid_cap, s);
+        case 82: return launch_batch_rpl<8, 2>(L, total, t16, grid_cap, s);
+        case 161: return launch_batch_rpl<16, 1>(L, total, t16, grid_cap, s);
+        case 162: return launch_batch_rpl<16, 2>(L, total, t16, grid_cap, s);
+        case 164: return launch_batch_rpl<16, 4>(L, total, t16, grid_cap, s);
         default: return hipErrorInvalidValue;
     }
 }

@@ -44,14 +44,45 @@ struct FitterConfig {
     int32_t blocks_per_group; // 32x32 blocks handled by one work-group
 };
 
-// One fitter work item: a 32x32 block of some stream's frame.
-struct FitterTask {
-    const void *tmp_block;    // 13*1024 elements of that block
-    float *mins_maxs;         // 12 floats
-    float *weights;           // 30 floats
-    int32_t frame;
-    int32_t flags;            // bit 0: tmp is half; bit 1: round to half
+// ---- batched launches: many streams per kernel -------------------------
+// Device-resident description of one stream in a batch: its constants and
+// both halves of every double buffer.
+struct BatchStream {
+    StageArgs args;  // constants; frame/matrix/offset come from BatchStep
+    void *albedo, *noisy_in, *prev_pixel, *accept, *tmp_data, *weights, *mins_maxs;
+    void *filtered, *tone_mapped, *noise;
+    void *normals[2], *positions[2], *noisy_acc[2], *spp[2], *out[2], *result[2];
 };
+
+// Per-step parameters of one stream (written by the host before each step).
+struct BatchStep {
+    int32_t frame;
+    int32_t cur;  // index of the current half of the double buffers
+    float matrix[16];
+    float pixel_offset_x, pixel_offset_y;
+};
+
+enum { kStartsPadded = 0, kStartsImage = 1, kStartsBlocks = 2, kStartKinds = 3 };
+constexpr int kMaxBatch = 64;
+
+// Kernel argument of the batched launches.  starts[k][s] is the first work
+// unit of stream s for unit kind k (padded-grid 32x8 tiles, image 64x4
+// tiles, 32x32 blocks); starts[k][n] is the total.
+struct BatchLaunch {
+    const BatchStream *streams;
+    const BatchStep *steps;
+    const int32_t *starts;  // [kStartKinds][kMaxBatch + 1]
+    int32_t n;
+};
+
+hipError_t launch_batch_accumulate_noisy_data(const BatchLaunch &L, int total, bool g16, bool t16,
+                                              hipStream_t s);
+hipError_t launch_batch_fitter(const BatchLaunch &L, int total_blocks, bool t16,
+                               const FitterConfig &cfg, int grid_cap, hipStream_t s);
+hipError_t launch_batch_weighted_sum(const BatchLaunch &L, int total, bool g16, hipStream_t s);
+hipError_t launch_batch_accumulate_filtered_data(const BatchLaunch &L, int total, bool g16,
+                                                 hipStream_t s);
+hipError_t launch_batch_taa(const BatchLaunch &L, int total, bool g16, hipStream_t s);
 
 // Up to four independent byte copies in one launch (16-byte aligned bases).
 struct CopyJob {

@@ -14,21 +14,27 @@ namespace bmfr {
 
 // ---------------------------------------------------------------------------
 // Stage 1: accumulate_noisy_data, opencl/bmfr.cl:287-485.
+// One work-item of the padded grid (gx, gy); `lin` is its linear index,
+// used by the first 9216 work-items to also write the frame's noise table.
 template <typename GT, typename TT>
-__global__ __launch_bounds__(256) void k_accumulate_noisy_data(
-    const StageArgs a, const GT *__restrict__ cur_normals, const GT *__restrict__ prev_normals,
-    const GT *__restrict__ cur_positions, const GT *__restrict__ prev_positions,
-    const GT *__restrict__ cur_noisy, const GT *__restrict__ prev_noisy,
-    const uint8_t *__restrict__ prev_spp, GT *__restrict__ acc_noisy,
-    float2 *__restrict__ out_prev_pixel, uint8_t *__restrict__ accept_bools,
-    uint8_t *__restrict__ cur_spp, TT *__restrict__ tmp_data, double *__restrict__ noise) {
-    const int gx = blockIdx.x * 32 + (threadIdx.x & 31);
-    const int gy = blockIdx.y * 8 + (threadIdx.x >> 5);
-    if (noise) {
+__device__ __forceinline__ void stage1_body(const StageArgs &a, const StageBuffers &bufs, int gx, int gy,
+                                            int lin, bool write_noise) {
+    const GT *__restrict__ cur_normals = (const GT *)bufs.normals_cur;
+    const GT *__restrict__ prev_normals = (const GT *)bufs.normals_prev;
+    const GT *__restrict__ cur_positions = (const GT *)bufs.positions_cur;
+    const GT *__restrict__ prev_positions = (const GT *)bufs.positions_prev;
+    const GT *__restrict__ cur_noisy = (const GT *)bufs.noisy_in;
+    const GT *__restrict__ prev_noisy = (const GT *)bufs.noisy_acc_prev;
+    const uint8_t *__restrict__ prev_spp = (const uint8_t *)bufs.spp_prev;
+    GT *__restrict__ acc_noisy = (GT *)bufs.noisy_acc_cur;
+    float2 *__restrict__ out_prev_pixel = (float2 *)bufs.prev_pixel;
+    uint8_t *__restrict__ accept_bools = (uint8_t *)bufs.accept;
+    uint8_t *__restrict__ cur_spp = (uint8_t *)bufs.spp_cur;
+    TT *__restrict__ tmp_data = (TT *)bufs.tmp_data;
+    if (write_noise && lin < kNoiseTable) {
         // the padded grid has at least 96x96 = 9216 work-items (W, H >= 47)
-        const int lin = (blockIdx.y * gridDim.x + blockIdx.x) * 256 + threadIdx.x;
-        if (lin < kNoiseTable)
-            noise[lin] = noise_term(lin & (kBlockPixels - 1), 1 + (lin >> 10), a.frame, a.noise_amount);
+        ((double *)bufs.noise)[lin] =
+            noise_term(lin & (kBlockPixels - 1), 1 + (lin >> 10), a.frame, a.noise_amount);
     }
     const int W = a.width, H = a.height;
     const int2 off = block_offset(a.frame);
@@ -156,15 +162,24 @@ __global__ __launch_bounds__(256) void k_accumulate_noisy_data(
     }
 }
 
+template <typename GT, typename TT>
+__global__ __launch_bounds__(256) void k_accumulate_noisy_data(const StageArgs a, const StageBuffers b,
+                                                               int write_noise) {
+    const int gx = blockIdx.x * 32 + (threadIdx.x & 31);
+    const int gy = blockIdx.y * 8 + (threadIdx.x >> 5);
+    const int lin = (blockIdx.y * gridDim.x + blockIdx.x) * 256 + threadIdx.x;
+    stage1_body<GT, TT>(a, b, gx, gy, lin, write_noise != 0);
+}
+
 // ---------------------------------------------------------------------------
 // Stage 3: weighted_sum, opencl/bmfr.cl:703-758.
 template <typename GT>
-__global__ __launch_bounds__(256) void k_weighted_sum(
-    const StageArgs a, const float *__restrict__ weights, const float *__restrict__ mins_maxs,
-    float *__restrict__ output, const GT *__restrict__ cur_normals,
-    const GT *__restrict__ cur_positions) {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+__device__ __forceinline__ void stage3_body(const StageArgs &a, const StageBuffers &bufs, int x, int y) {
+    const float *__restrict__ weights = (const float *)bufs.weights;
+    const float *__restrict__ mins_maxs = (const float *)bufs.mins_maxs;
+    float *__restrict__ output = (float *)bufs.filtered;
+    const GT *__restrict__ cur_normals = (const GT *)bufs.normals_cur;
+    const GT *__restrict__ cur_positions = (const GT *)bufs.positions_cur;
     if (x >= a.width || y >= a.height) return;
     const int64_t lp = (int64_t)y * a.width + x;
     const int2 off = block_offset(a.frame);
@@ -190,16 +205,23 @@ __global__ __launch_bounds__(256) void k_weighted_sum(
     output[lp * 3 + 2] = cz < 0.f ? 0.f : cz;
 }
 
+template <typename GT>
+__global__ __launch_bounds__(256) void k_weighted_sum(const StageArgs a, const StageBuffers b) {
+    stage3_body<GT>(a, b, blockIdx.x * 64 + (threadIdx.x & 63), blockIdx.y * 4 + (threadIdx.x >> 6));
+}
+
 // ---------------------------------------------------------------------------
 // Stage 4: accumulate_filtered_data, opencl/bmfr.cl:761-857.
 template <typename GT>
-__global__ __launch_bounds__(256) void k_accumulate_filtered_data(
-    const StageArgs a, const float *__restrict__ filtered, const float2 *__restrict__ prev_pixel,
-    const uint8_t *__restrict__ accept_bools, const float *__restrict__ albedo,
-    float *__restrict__ tone_mapped, const uint8_t *__restrict__ cur_spp,
-    const GT *__restrict__ acc_prev, GT *__restrict__ acc_out) {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+__device__ __forceinline__ void stage4_body(const StageArgs &a, const StageBuffers &bufs, int x, int y) {
+    const float *__restrict__ filtered = (const float *)bufs.filtered;
+    const float2 *__restrict__ prev_pixel = (const float2 *)bufs.prev_pixel;
+    const uint8_t *__restrict__ accept_bools = (const uint8_t *)bufs.accept;
+    const float *__restrict__ albedo = (const float *)bufs.albedo;
+    float *__restrict__ tone_mapped = (float *)bufs.tone_mapped;
+    const uint8_t *__restrict__ cur_spp = (const uint8_t *)bufs.spp_cur;
+    const GT *__restrict__ acc_prev = (const GT *)bufs.out_prev;
+    GT *__restrict__ acc_out = (GT *)bufs.out_cur;
     const int W = a.width;
     if (x >= W || y >= a.height) return;
     const int64_t lp = (int64_t)y * W + x;
@@ -250,6 +272,11 @@ __global__ __launch_bounds__(256) void k_accumulate_filtered_data(
     tone_mapped[lp * 3 + 2] = fminf(fmaxf(powf(fmaxf(0.f, ab * ac.z), 0.454545f), 0.f), 1.f);
 }
 
+template <typename GT>
+__global__ __launch_bounds__(256) void k_accumulate_filtered_data(const StageArgs a, const StageBuffers b) {
+    stage4_body<GT>(a, b, blockIdx.x * 64 + (threadIdx.x & 63), blockIdx.y * 4 + (threadIdx.x >> 6));
+}
+
 // ---------------------------------------------------------------------------
 // Stage 5: taa, opencl/bmfr.cl:860-974.
 __device__ __forceinline__ float3 rgb_to_ycocg(float3 c) {
@@ -263,12 +290,11 @@ __device__ __forceinline__ float3 ycocg_to_rgb(float3 c) {
 }
 
 template <typename GT>
-__global__ __launch_bounds__(256) void k_taa(const StageArgs a,
-                                             const float2 *__restrict__ prev_pixel,
-                                             const float *__restrict__ new_frame,
-                                             GT *__restrict__ result, const GT *__restrict__ prev_result) {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+__device__ __forceinline__ void stage5_body(const StageArgs &a, const StageBuffers &bufs, int x, int y) {
+    const float2 *__restrict__ prev_pixel = (const float2 *)bufs.prev_pixel;
+    const float *__restrict__ new_frame = (const float *)bufs.tone_mapped;
+    GT *__restrict__ result = (GT *)bufs.result_cur;
+    const GT *__restrict__ prev_result = (const GT *)bufs.result_prev;
     const int W = a.width, H = a.height;
     if (x >= W || y >= H) return;
     const int64_t lp = (int64_t)y * W + x;
@@ -354,18 +380,92 @@ __global__ __launch_bounds__(256) void k_taa(const StageArgs a,
     store3(result, lp, r);
 }
 
+template <typename GT>
+__global__ __launch_bounds__(256) void k_taa(const StageArgs a, const StageBuffers b) {
+    stage5_body<GT>(a, b, blockIdx.x * 64 + (threadIdx.x & 63), blockIdx.y * 4 + (threadIdx.x >> 6));
+}
+
+// ---------------------------------------------------------------------------
+// Batched kernels: one 1-D grid over the concatenated tiles of all streams
+// of a batch.  Each work-group finds its stream from the prefix table
+// (uniform, scalar loop over at most kMaxBatch entries), then runs the same
+// per-pixel body as the single-stream kernel.
+__device__ __forceinline__ int find_stream(const int32_t *starts, int n, int unit) {
+    int s = 0;
+    while (s + 1 < n && starts[s + 1] <= unit) ++s;
+    return s;
+}
+
+__device__ __forceinline__ void resolve_batch(const BatchLaunch &L, int s, StageArgs &a, StageBuffers &b) {
+    const BatchStream &bs = L.streams[s];
+    const BatchStep &st = L.steps[s];
+    a = bs.args;
+    a.frame = st.frame;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) a.matrix[i] = st.matrix[i];
+    a.pixel_offset_x = st.pixel_offset_x;
+    a.pixel_offset_y = st.pixel_offset_y;
+    const int c = st.cur, p = 1 - st.cur;
+    b.albedo = bs.albedo;
+    b.normals_cur = bs.normals[c];
+    b.normals_prev = bs.normals[p];
+    b.positions_cur = bs.positions[c];
+    b.positions_prev = bs.positions[p];
+    b.noisy_in = bs.noisy_in;
+    b.noisy_acc_cur = bs.noisy_acc[c];
+    b.noisy_acc_prev = bs.noisy_acc[p];
+    b.spp_cur = bs.spp[c];
+    b.spp_prev = bs.spp[p];
+    b.prev_pixel = bs.prev_pixel;
+    b.accept = bs.accept;
+    b.tmp_data = bs.tmp_data;
+    b.weights = bs.weights;
+    b.mins_maxs = bs.mins_maxs;
+    b.filtered = bs.filtered;
+    b.tone_mapped = bs.tone_mapped;
+    b.out_cur = bs.out[c];
+    b.out_prev = bs.out[p];
+    b.result_cur = bs.result[c];
+    b.result_prev = bs.result[p];
+    b.noise = bs.noise;
+}
+
+template <typename GT, typename TT>
+__global__ __launch_bounds__(256) void kb_accumulate_noisy_data(const BatchLaunch L) {
+    const int32_t *starts = L.starts + kStartsPadded * (kMaxBatch + 1);
+    const int s = find_stream(starts, L.n, blockIdx.x);
+    StageArgs a;
+    StageBuffers b;
+    resolve_batch(L, s, a, b);
+    const int local = blockIdx.x - starts[s];
+    const int tiles_x = a.margin_width / 32;
+    const int ty = local / tiles_x, tx = local - ty * tiles_x;
+    stage1_body<GT, TT>(a, b, tx * 32 + (threadIdx.x & 31), ty * 8 + (threadIdx.x >> 5),
+                        local * 256 + threadIdx.x, true);
+}
+
+template <int STAGE, typename GT>
+__global__ __launch_bounds__(256) void kb_image_stage(const BatchLaunch L) {
+    const int32_t *starts = L.starts + kStartsImage * (kMaxBatch + 1);
+    const int s = find_stream(starts, L.n, blockIdx.x);
+    StageArgs a;
+    StageBuffers b;
+    resolve_batch(L, s, a, b);
+    const int local = blockIdx.x - starts[s];
+    const int tiles_x = (a.width + 63) / 64;
+    const int ty = local / tiles_x, tx = local - ty * tiles_x;
+    const int x = tx * 64 + (threadIdx.x & 63), y = ty * 4 + (threadIdx.x >> 6);
+    if constexpr (STAGE == 3) stage3_body<GT>(a, b, x, y);
+    else if constexpr (STAGE == 4) stage4_body<GT>(a, b, x, y);
+    else stage5_body<GT>(a, b, x, y);
+}
+
 // ---------------------------------------------------------------------------
 // Host-side launchers.
 template <typename GT, typename TT>
 static hipError_t launch_s1(const StageArgs &a, const StageBuffers &b, bool wn, hipStream_t s) {
     dim3 grid(a.margin_width / 32, a.margin_height / 8);
-    hipLaunchKernelGGL((k_accumulate_noisy_data<GT, TT>), grid, dim3(256), 0, s, a,
-                       (const GT *)b.normals_cur, (const GT *)b.normals_prev,
-                       (const GT *)b.positions_cur, (const GT *)b.positions_prev,
-                       (const GT *)b.noisy_in, (const GT *)b.noisy_acc_prev,
-                       (const uint8_t *)b.spp_prev, (GT *)b.noisy_acc_cur, (float2 *)b.prev_pixel,
-                       (uint8_t *)b.accept, (uint8_t *)b.spp_cur, (TT *)b.tmp_data,
-                       wn ? (double *)b.noise : nullptr);
+    hipLaunchKernelGGL((k_accumulate_noisy_data<GT, TT>), grid, dim3(256), 0, s, a, b, wn ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -396,38 +496,58 @@ static inline dim3 image_grid(const StageArgs &a) {
 
 hipError_t launch_weighted_sum(const StageArgs &a, const StageBuffers &b, hipStream_t s) {
     if (a.storage_f16)
-        hipLaunchKernelGGL((k_weighted_sum<__half>), image_grid(a), dim3(256), 0, s, a,
-                           (const float *)b.weights, (const float *)b.mins_maxs, (float *)b.filtered,
-                           (const __half *)b.normals_cur, (const __half *)b.positions_cur);
+        hipLaunchKernelGGL((k_weighted_sum<__half>), image_grid(a), dim3(256), 0, s, a, b);
     else
-        hipLaunchKernelGGL((k_weighted_sum<float>), image_grid(a), dim3(256), 0, s, a,
-                           (const float *)b.weights, (const float *)b.mins_maxs, (float *)b.filtered,
-                           (const float *)b.normals_cur, (const float *)b.positions_cur);
-    return hipGetLastError();
-}
-
-template <typename GT>
-static hipError_t launch_s4(const StageArgs &a, const StageBuffers &b, hipStream_t s) {
-    hipLaunchKernelGGL((k_accumulate_filtered_data<GT>), image_grid(a), dim3(256), 0, s, a,
-                       (const float *)b.filtered, (const float2 *)b.prev_pixel,
-                       (const uint8_t *)b.accept, (const float *)b.albedo, (float *)b.tone_mapped,
-                       (const uint8_t *)b.spp_cur, (const GT *)b.out_prev, (GT *)b.out_cur);
+        hipLaunchKernelGGL((k_weighted_sum<float>), image_grid(a), dim3(256), 0, s, a, b);
     return hipGetLastError();
 }
 
 hipError_t launch_accumulate_filtered_data(const StageArgs &a, const StageBuffers &b, hipStream_t s) {
-    return a.storage_f16 ? launch_s4<__half>(a, b, s) : launch_s4<float>(a, b, s);
-}
-
-template <typename GT>
-static hipError_t launch_s5(const StageArgs &a, const StageBuffers &b, hipStream_t s) {
-    hipLaunchKernelGGL((k_taa<GT>), image_grid(a), dim3(256), 0, s, a, (const float2 *)b.prev_pixel,
-                       (const float *)b.tone_mapped, (GT *)b.result_cur, (const GT *)b.result_prev);
+    if (a.storage_f16)
+        hipLaunchKernelGGL((k_accumulate_filtered_data<__half>), image_grid(a), dim3(256), 0, s, a, b);
+    else
+        hipLaunchKernelGGL((k_accumulate_filtered_data<float>), image_grid(a), dim3(256), 0, s, a, b);
     return hipGetLastError();
 }
 
 hipError_t launch_taa(const StageArgs &a, const StageBuffers &b, hipStream_t s) {
-    return a.storage_f16 ? launch_s5<__half>(a, b, s) : launch_s5<float>(a, b, s);
+    if (a.storage_f16)
+        hipLaunchKernelGGL((k_taa<__half>), image_grid(a), dim3(256), 0, s, a, b);
+    else
+        hipLaunchKernelGGL((k_taa<float>), image_grid(a), dim3(256), 0, s, a, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_accumulate_noisy_data(const BatchLaunch &L, int total, bool g16, bool t16,
+                                              hipStream_t s) {
+    if (total <= 0) return hipErrorInvalidValue;
+    if (g16) {
+        if (t16) hipLaunchKernelGGL((kb_accumulate_noisy_data<__half, __half>), dim3(total), dim3(256), 0, s, L);
+        else hipLaunchKernelGGL((kb_accumulate_noisy_data<__half, float>), dim3(total), dim3(256), 0, s, L);
+    } else {
+        if (t16) hipLaunchKernelGGL((kb_accumulate_noisy_data<float, __half>), dim3(total), dim3(256), 0, s, L);
+        else hipLaunchKernelGGL((kb_accumulate_noisy_data<float, float>), dim3(total), dim3(256), 0, s, L);
+    }
+    return hipGetLastError();
+}
+
+template <int STAGE>
+static hipError_t launch_batch_image(const BatchLaunch &L, int total, bool g16, hipStream_t s) {
+    if (total <= 0) return hipErrorInvalidValue;
+    if (g16) hipLaunchKernelGGL((kb_image_stage<STAGE, __half>), dim3(total), dim3(256), 0, s, L);
+    else hipLaunchKernelGGL((kb_image_stage<STAGE, float>), dim3(total), dim3(256), 0, s, L);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_weighted_sum(const BatchLaunch &L, int total, bool g16, hipStream_t s) {
+    return launch_batch_image<3>(L, total, g16, s);
+}
+hipError_t launch_batch_accumulate_filtered_data(const BatchLaunch &L, int total, bool g16,
+                                                 hipStream_t s) {
+    return launch_batch_image<4>(L, total, g16, s);
+}
+hipError_t launch_batch_taa(const BatchLaunch &L, int total, bool g16, hipStream_t s) {
+    return launch_batch_image<5>(L, total, g16, s);
 }
 
 }  // namespace bmfr

@@ -192,6 +192,41 @@ int32_t bmfr_stage_times(bmfr_stream stream, float *ms6);
 int32_t bmfr_record_marker(bmfr_stream stream, int32_t marker);
 int32_t bmfr_marker_elapsed(bmfr_stream stream, int32_t m0, int32_t m1, float *ms);
 
+/* ---- batches: many streams, one launch per stage ------------------------------
+ * A batch runs one frame of each member stream per step: each of the five
+ * stages is one launch over the concatenated tiles/blocks of every member
+ * (one launch per storage type present), the fitter a persistent launch over
+ * all members' blocks, edge blocks included.  No counterpart in the
+ * reference, which runs one stream with five launches per frame
+ * (opencl/bmfr.cpp:417-485); each member's results are those of running it
+ * alone.  Members keep their own buffers and history; a stream belongs to at
+ * most one batch at a time and must not be stepped on its own meanwhile. */
+typedef struct bmfr_batch_s *bmfr_batch;
+int32_t bmfr_batch_create(const bmfr_stream *streams, int32_t count, bmfr_batch *out);
+int32_t bmfr_batch_destroy(bmfr_batch batch);
+/* One step: frames[i], matrices[16*i..], offsets[2*i..] are member i's
+ * bmfr_accumulate_noisy_data arguments.  Ordered after each member's pending
+ * work; each member's later work waits for it; members' double buffers are
+ * swapped.  timed != 0 records per-stage events (no graph replay). */
+int32_t bmfr_batch_run(bmfr_batch batch, const int32_t *frames, const float *matrices,
+                       const float *offsets, int32_t timed);
+int32_t bmfr_batch_synchronize(bmfr_batch batch);
+/* hipGraph capture of the five batched launches, replayed each step. */
+int32_t bmfr_batch_set_graph(bmfr_batch batch, int32_t enable);
+/* Fitter shape for the batch; grid_cap > 0 bounds the persistent grid. */
+int32_t bmfr_batch_set_fitter_config(bmfr_batch batch, int32_t rows_per_lane,
+                                     int32_t blocks_per_group, int32_t grid_cap);
+/* Per-stage ms of the last timed step (as bmfr_stage_times); after a timed
+ * graph replay only the total (ms6[5]) is measured and ms6[0..4] are -1. */
+int32_t bmfr_batch_stage_times(bmfr_batch batch, float *ms6);
+/* Autotuner: times every supported fitter shape (rows per lane, blocks per
+ * work-group, persistent grid cap) on the data of the last step with HIP
+ * events, keeps the fastest and writes a log line into `log`. */
+int32_t bmfr_batch_autotune(bmfr_batch batch, int32_t reps, char *log, size_t log_len);
+/* Markers 0..4 on the batch's HIP stream (shared with the stage events). */
+int32_t bmfr_batch_record_marker(bmfr_batch batch, int32_t marker);
+int32_t bmfr_batch_marker_elapsed(bmfr_batch batch, int32_t m0, int32_t m1, float *ms);
+
 /* ---- procedural scene (replaces the EXR dataset, opencl/bmfr.cpp:145-172) ---- */
 typedef struct bmfr_scene_s *bmfr_scene;
 /* Plane y=0 (|x|,|z| <= 64) with `spheres` spheres, seeded; a pinhole camera

@@ -1,0 +1,143 @@
+"""Batched execution (one launch per stage over many streams): each stream's
+results must be those of running it alone, bit for bit -- the batched
+kernels run the same per-pixel and per-block code on the same inputs.
+Covers mixed resolutions, mixed storage precision (several launch groups),
+edge blocks, hipGraph replay interleaved with eager work, and errors."""
+import numpy as np
+import pytest
+
+import bmfr_amd
+from tests.harness import gpu_params, scene
+
+pytestmark = pytest.mark.gpu
+
+MIX = [  # (w, h, storage_f16, tmp_f16, seed offset)
+    (96, 64, False, True, 0),
+    (130, 70, False, True, 1),     # ragged: W, H not multiples of 32
+    (64, 48, True, True, 2),       # FP16 storage: a second launch group
+    (200, 120, False, False, 3),   # FP32 tmp_data: a third group
+]
+
+
+def _streams(mix):
+    return [bmfr_amd.Stream(gpu_params(w, h, tmp_f16=t16, storage_f16=g16)) for (w, h, g16, t16, _) in mix]
+
+
+def _scenes(mix):
+    return [scene(w, h, seed=0xB3F40000 + 3 + k) for (w, h, _, _, k) in mix]
+
+
+def _buffers(st):
+    return {n: st.read_buffer(n) for n in ("result_prev", "out_prev", "noisy_acc_prev", "spp_prev",
+                                            "weights", "mins_maxs", "prev_pixel", "accept", "tone_mapped")}
+
+
+def _run(mix, frames, batched, graph=False, eager_between=False):
+    sts, scs = _streams(mix), _scenes(mix)
+    b = bmfr_amd.Batch(sts) if batched else None
+    if graph:
+        b.set_graph(True)
+    try:
+        for fr in range(frames):
+            for st, sc in zip(sts, scs):
+                sc.render_to_stream(fr, st)
+            args = [sc.frame_args(fr) for sc in scs]
+            if batched:
+                b.run([fr] * len(sts), np.concatenate([a[0] for a in args]),
+                      np.concatenate([a[1] for a in args]))
+                if eager_between:
+                    # eager work and synchronisation between graph replays
+                    b.synchronize()
+                    for st in sts:
+                        st.read_buffer("weights")
+            else:
+                for st, (vp, off) in zip(sts, args):
+                    st.run_frame(fr, vp, off)
+        out = []
+        for st in sts:
+            st.synchronize()
+            d = _buffers(st)
+            d["download"] = st.download_result()
+            out.append(d)
+        return out
+    finally:
+        if b:
+            b.close()
+        for st in sts:
+            st.close()
+
+
+def _same(a, b):
+    return np.array_equal(a, b) or np.array_equal(np.isnan(a), np.isnan(b)) and np.array_equal(
+        np.nan_to_num(a), np.nan_to_num(b))
+
+
+def test_batch_equals_each_stream_alone(gpu):
+    alone = _run(MIX, 5, batched=False)
+    mixed = _run(MIX, 5, batched=True)
+    for k, (a, m) in enumerate(zip(alone, mixed)):
+        for name in a:
+            assert _same(a[name], m[name]), (k, name)
+
+
+def test_batch_graph_replay_equals_eager(gpu):
+    eager = _run(MIX[:3], 6, batched=True)
+    graph = _run(MIX[:3], 6, batched=True, graph=True, eager_between=True)
+    for k, (a, m) in enumerate(zip(eager, graph)):
+        for name in a:
+            assert _same(a[name], m[name]), (k, name)
+
+
+@pytest.mark.parametrize("cfg", [(4, 1, 0), (8, 2, 0), (16, 1, 64), (16, 4, 8)])
+def test_batch_fitter_configs_and_persistent_grid(gpu, cfg):
+    """Every fitter shape, with a capped (persistent, several blocks per
+    work-group) grid, gives the stream-alone weights of the same shape."""
+    mix = MIX[:2]
+    sts, scs = _streams(mix), _scenes(mix)
+    alone = _streams(mix)
+    for st in alone:
+        st.set_fitter_config(cfg[0], cfg[1])
+    b = bmfr_amd.Batch(sts)
+    b.set_fitter_config(*cfg)
+    try:
+        for fr in range(3):
+            args = [sc.frame_args(fr) for sc in scs]
+            for st, a_st, sc in zip(sts, alone, scs):
+                sc.render_to_stream(fr, st)
+                sc.render_to_stream(fr, a_st)
+            b.run([fr] * 2, np.concatenate([a[0] for a in args]), np.concatenate([a[1] for a in args]))
+            for a_st, (vp, off) in zip(alone, args):
+                a_st.run_frame(fr, vp, off)
+        for st, a_st in zip(sts, alone):
+            assert np.array_equal(st.read_buffer("weights"), a_st.read_buffer("weights"))
+            assert np.array_equal(st.download_result(), a_st.download_result())
+    finally:
+        b.close()
+        for st in sts + alone:
+            st.close()
+
+
+def test_batch_timing_and_errors(gpu):
+    sts = _streams(MIX[:2])
+    try:
+        with pytest.raises(bmfr_amd.BmfrError):
+            bmfr_amd.Batch([sts[0], sts[0]])
+        b = bmfr_amd.Batch(sts)
+        with pytest.raises(bmfr_amd.BmfrError):
+            b.set_fitter_config(3, 1)
+        with pytest.raises(bmfr_amd.BmfrError):
+            b.stage_times()
+        scs = _scenes(MIX[:2])
+        args = [sc.frame_args(0) for sc in scs]
+        for st, sc in zip(sts, scs):
+            sc.render_to_stream(0, st)
+        b.run([0, 0], np.concatenate([a[0] for a in args]), np.concatenate([a[1] for a in args]),
+              timed=True)
+        t = b.stage_times()
+        assert (t > 0).all() and t[5] >= t[:5].max()
+        with pytest.raises(ValueError):
+            b.run([0], np.zeros(32), np.zeros(4))
+        b.close()
+    finally:
+        for st in sts:
+            st.close()

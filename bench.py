@@ -134,6 +134,150 @@ def run_batched(steps, warmup):
     }
 
 
+SERVE_RES = [(640, 360), (1280, 720), (1920, 1080), (2560, 1440)]
+SERVE_P = [0.2, 0.5, 0.25, 0.05]
+
+
+def run_served(duration, budget_gib, seed=0xB3F40000 + 4):
+    """64 render streams opened and closed over `duration` seconds of wall
+    time; per stream a resolution from SERVE_RES with probabilities SERVE_P,
+    Poisson frame arrivals at 30 or 60 Hz and FP16 or FP32 storage.  Streams
+    are admitted by the server's HBM budget (a FIFO waits for memory), and
+    every step batches the oldest pending frame of each stream with work
+    (continuous batching, at most two steps in flight).  Latency = host time
+    at which a frame's step is seen complete minus the frame's arrival."""
+    import bmfr_amd
+    rng = np.random.default_rng(seed)
+    n = 64
+    specs = []
+    for k in range(n):
+        w, h = SERVE_RES[rng.choice(4, p=SERVE_P)]
+        t_open = rng.uniform(0.0, 0.8 * duration)
+        t_close = min(duration, t_open + rng.uniform(0.15, 0.4) * duration)
+        rate = 30.0 if rng.random() < 0.5 else 60.0
+        f16 = bool(rng.random() < 0.5)
+        arr, t = [], t_open
+        while True:
+            t += rng.exponential(1.0 / rate)
+            if t >= t_close:
+                break
+            arr.append(t)
+        specs.append(dict(w=w, h=h, open=t_open, close=t_close, rate=rate, f16=f16, arrivals=arr))
+    sv = bmfr_amd.Server(int(budget_gib * (1 << 30)))
+    state = [dict(stream=None, scene=None, next=0, pending=[], frame=0, done=False) for _ in specs]
+    waiting, admitted, waited = [], 0, 0
+    latencies, step_ms, step_sizes, in_flight = [], [], [], []
+    frames_done, pixels_done = 0, 0
+    t0 = time.perf_counter()
+    order = sorted(range(n), key=lambda k: specs[k]["open"])
+    oi = 0
+    while True:
+        now = time.perf_counter() - t0
+        # streams that open now join the admission queue
+        while oi < n and specs[order[oi]]["open"] <= now:
+            waiting.append(order[oi])
+            oi += 1
+        still = []
+        for k in waiting:
+            sp = specs[k]
+            if now >= sp["close"]:
+                continue  # never admitted before its end
+            try:
+                p = bmfr_amd.default_params(sp["w"], sp["h"],
+                                            storage=bmfr_amd.STORAGE_F16 if sp["f16"] else bmfr_amd.STORAGE_F32)
+                st = sv.open_stream(p)
+            except bmfr_amd.BmfrError as e:
+                if e.status != bmfr_amd.ERR_OUT_OF_BUDGET:
+                    raise
+                still.append(k)
+                continue
+            if now - sp["open"] > 0.005:
+                waited += 1
+            state[k]["stream"] = st
+            state[k]["scene"] = bmfr_amd.Scene(seed + 100 + k, SPHERES, 1e-3, sp["w"], sp["h"])
+            # frames that arrived while the stream waited for memory are dropped
+            while state[k]["next"] < len(sp["arrivals"]) and sp["arrivals"][state[k]["next"]] < now:
+                state[k]["next"] += 1
+            admitted += 1
+        waiting = still
+        # arrivals
+        for k in range(n):
+            s_ = state[k]
+            if s_["stream"] is None:
+                continue
+            arr = specs[k]["arrivals"]
+            while s_["next"] < len(arr) and arr[s_["next"]] <= now:
+                s_["pending"].append(arr[s_["next"]])
+                s_["next"] += 1
+        # completions
+        keep = []
+        for (sid, members) in in_flight:
+            done, ms = sv.query(sid)
+            if done:
+                tnow = time.perf_counter() - t0
+                step_ms.append(ms)
+                for (k, t_arr, px) in members:
+                    latencies.append(tnow - t_arr)
+                    frames_done += 1
+                    pixels_done += px
+            else:
+                keep.append((sid, members))
+        in_flight = keep
+        # closes (after the stream's in-flight work, which close waits for)
+        for k in range(n):
+            s_ = state[k]
+            if s_["stream"] is not None and now >= specs[k]["close"] and not s_["pending"] and \
+                    not any(k == m[0] for (_, ms_) in in_flight for m in ms_):
+                s_["stream"].close()
+                s_["scene"].close()
+                s_["stream"] = None
+                s_["done"] = True
+        # launch a step when fewer than two are in flight
+        if len(in_flight) < 2:
+            members = [k for k in range(n) if state[k]["stream"] is not None and state[k]["pending"]]
+            members = members[:64]
+            if members:
+                sts, frs, mats, offs, info = [], [], [], [], []
+                for k in members:
+                    s_ = state[k]
+                    t_arr = s_["pending"].pop(0)
+                    fr = s_["frame"]
+                    s_["frame"] += 1
+                    s_["scene"].render_to_stream(fr, s_["stream"])
+                    vp, off = s_["scene"].frame_args(fr)
+                    sts.append(s_["stream"])
+                    frs.append(fr)
+                    mats.append(vp)
+                    offs.append(off)
+                    info.append((k, t_arr, specs[k]["w"] * specs[k]["h"]))
+                sid = sv.step(sts, frs, np.concatenate(mats), np.concatenate(offs))
+                in_flight.append((sid, info))
+                step_sizes.append(len(members))
+        if now >= duration and not in_flight and oi >= n:
+            break
+        if now >= duration + 5.0:
+            break
+    wall = time.perf_counter() - t0
+    sv.synchronize()
+    for s_ in state:
+        if s_["stream"] is not None:
+            s_["stream"].close()
+            s_["scene"].close()
+    sv.close()
+    lat = np.array(latencies) * 1e3
+    return {
+        "streams": n, "duration_s": round(wall, 2), "budget_gib": budget_gib,
+        "admitted": admitted, "admitted_after_waiting": waited,
+        "frames": frames_done, "aggregate_frames_per_s": round(frames_done / wall, 1),
+        "mpixel_per_s": round(pixels_done / wall / 1e6, 1),
+        "latency_ms_p50": round(float(np.percentile(lat, 50)), 3) if len(lat) else None,
+        "latency_ms_p99": round(float(np.percentile(lat, 99)), 3) if len(lat) else None,
+        "latency_ms_max": round(float(lat.max()), 3) if len(lat) else None,
+        "steps": len(step_sizes), "mean_streams_per_step": round(float(np.mean(step_sizes)), 2),
+        "step_gpu_ms_mean": round(float(np.mean(step_ms)), 4) if step_ms else None,
+    }
+
+
 def measured_traffic(kernel):
     """HBM bytes per dispatch of `kernel` from the calibrated rocprofv3 PMC
     runs summarised in profiles/ (null when that summary is absent)."""
@@ -196,6 +340,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=40)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=30.0)
+    ap.add_argument("--serve-budget-gib", type=float, default=4.0)
+    ap.add_argument("--no-served", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -216,6 +363,9 @@ def main():
     if not args.no_batched:
         batched = run_batched(args.batched_steps, 5)
         batched["speedup_vs_plain_720p"] = round(ms_frame / batched["ms_per_720p_frame_equiv"], 2)
+    served = None
+    if not args.no_served:
+        served = run_served(args.serve_seconds, args.serve_budget_gib)
     copy_gbs = copy_peak()
     roof["copy_peak_gbs"] = round(copy_gbs, 1)
     cpu = None
@@ -243,6 +393,7 @@ def main():
         "stage_gbs": {k: round(v, 1) for k, v in stage_gbs.items()},
         "roofline": roof,
         "batched": batched,
+        "served": served,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

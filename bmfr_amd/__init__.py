@@ -110,7 +110,11 @@ class Stream:
 
     def close(self):
         if self._h:
-            _check(self._lib.bmfr_stream_close(self._h))
+            owner = getattr(self, "_owner", None)
+            if owner is not None:
+                owner.close_stream(self)
+            else:
+                _check(self._lib.bmfr_stream_close(self._h))
             self._h = None
 
     def __enter__(self):
@@ -293,6 +297,70 @@ class Batch:
         ms = ctypes.c_float()
         _check(self._lib.bmfr_batch_marker_elapsed(self._h, m0, m1, ctypes.byref(ms)))
         return ms.value
+
+
+class Server:
+    """Streams admitted by an HBM budget and stepped in dynamic batches
+    (bmfr_server_* in include/bmfr_hip.h)."""
+
+    def __init__(self, budget_bytes):
+        self._lib = _lib.load()
+        h = ctypes.c_void_p()
+        _check(self._lib.bmfr_server_create(int(budget_bytes), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            _check(self._lib.bmfr_server_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def usage(self):
+        u, b, n = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32()
+        _check(self._lib.bmfr_server_usage(self._h, ctypes.byref(u), ctypes.byref(b), ctypes.byref(n)))
+        return u.value, b.value, n.value
+
+    def open_stream(self, params):
+        """A server-owned Stream; raises BmfrError(ERR_OUT_OF_BUDGET) when it
+        does not fit the budget."""
+        h = ctypes.c_void_p()
+        _check(self._lib.bmfr_server_open_stream(self._h, ctypes.byref(params), ctypes.byref(h)))
+        st = Stream.__new__(Stream)
+        st._lib, st.params, st._h = self._lib, params, h
+        sz = BmfrSizes()
+        _check(self._lib.bmfr_stream_sizes(h, ctypes.byref(sz)))
+        st.sizes = sz
+        st._owner = self
+        return st
+
+    def close_stream(self, st):
+        _check(self._lib.bmfr_server_close_stream(self._h, st._h))
+        st._h = None
+
+    def step(self, streams, frames, matrices, offsets):
+        n = len(streams)
+        arr = (ctypes.c_void_p * n)(*[s.handle for s in streams])
+        fr = np.ascontiguousarray(frames, np.int32)
+        m = _f32(matrices, 16 * n)
+        o = _f32(offsets, 2 * n)
+        sid = ctypes.c_int32()
+        _check(self._lib.bmfr_server_step(self._h, n, arr, fr.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                          _fp(m), _fp(o), ctypes.byref(sid)))
+        return sid.value
+
+    def query(self, step_id):
+        """(done, gpu_ms) of a step id returned by step()."""
+        done, ms = ctypes.c_int32(), ctypes.c_float()
+        _check(self._lib.bmfr_server_step_query(self._h, step_id, ctypes.byref(done), ctypes.byref(ms)))
+        return bool(done.value), ms.value
+
+    def synchronize(self):
+        _check(self._lib.bmfr_server_synchronize(self._h))
 
 
 class Scene:

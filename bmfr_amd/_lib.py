@@ -91,6 +91,19 @@ SIGNATURES = [
     ("bmfr_batch_autotune", c_int32, [ctypes.c_void_p, c_int32, ctypes.c_char_p, ctypes.c_size_t]),
     ("bmfr_batch_record_marker", c_int32, [ctypes.c_void_p, c_int32]),
     ("bmfr_batch_marker_elapsed", c_int32, [ctypes.c_void_p, c_int32, c_int32, c_float_p]),
+    ("bmfr_server_create", c_int32, [ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p)]),
+    ("bmfr_server_destroy", c_int32, [ctypes.c_void_p]),
+    ("bmfr_server_usage", c_int32, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64),
+                                    ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(c_int32)]),
+    ("bmfr_server_open_stream", c_int32, [ctypes.c_void_p, ctypes.POINTER(BmfrParams),
+                                          ctypes.POINTER(ctypes.c_void_p)]),
+    ("bmfr_server_close_stream", c_int32, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("bmfr_server_step", c_int32, [ctypes.c_void_p, c_int32, ctypes.POINTER(ctypes.c_void_p),
+                                   ctypes.POINTER(c_int32), c_float_p, c_float_p,
+                                   ctypes.POINTER(c_int32)]),
+    ("bmfr_server_step_query", c_int32, [ctypes.c_void_p, c_int32, ctypes.POINTER(c_int32),
+                                         c_float_p]),
+    ("bmfr_server_synchronize", c_int32, [ctypes.c_void_p]),
     ("bmfr_scene_create", c_int32, [ctypes.c_uint64, c_int32, ctypes.c_float, c_int32, c_int32,
                                     c_int32, ctypes.POINTER(ctypes.c_void_p)]),
     ("bmfr_scene_destroy", c_int32, [ctypes.c_void_p]),

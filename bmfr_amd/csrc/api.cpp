@@ -258,6 +258,7 @@ int32_t timed_stage(bmfr_stream s, int stage, F &&launch) {
 }  // namespace
 
 extern "C" int32_t bmfr_batch_destroy(bmfr_batch b);
+extern "C" int32_t bmfr_server_destroy(bmfr_server sv);
 
 extern "C" {
 
@@ -1019,6 +1020,254 @@ int32_t bmfr_batch_marker_elapsed(bmfr_batch b, int32_t m0, int32_t m1, float *m
     HIP_TRY(hipEventSynchronize(b->ev_start[m1]));
     HIP_TRY(hipEventElapsedTime(ms, b->ev_start[m0], b->ev_start[m1]));
     b->recorded = false;
+    return BMFR_OK;
+}
+
+}  // extern "C"
+
+
+// ---- server: admission by HBM budget and dynamic batched steps ------------
+struct bmfr_server_s {
+    int device = 0;
+    int64_t budget = 0, used = 0;
+    std::vector<bmfr_stream> open;
+    hipStream_t stream = nullptr;
+    // device tables: one slot per storage group (F32/F16 x tmp F32/F16)
+    static constexpr int kGroups = 4;
+    bmfr::BatchStream *d_streams = nullptr;  // [kGroups][kMaxBatch]
+    bmfr::BatchStep *d_steps = nullptr;      // [kGroups][kMaxBatch]
+    int32_t *d_starts = nullptr;             // [kGroups][kStartKinds][kMaxBatch+1]
+    static constexpr int kRing = 4;
+    struct Staging {
+        bmfr::BatchStream streams[kGroups][bmfr::kMaxBatch];
+        bmfr::BatchStep steps[kGroups][bmfr::kMaxBatch];
+        int32_t starts[kGroups][bmfr::kStartKinds][bmfr::kMaxBatch + 1];
+    };
+    Staging *h_stage = nullptr;              // [kRing], pinned
+    hipEvent_t ring_ev[kRing]{};
+    int ring = 0;
+    hipEvent_t step_start[kRing]{}, step_end[kRing]{};
+    int64_t steps_launched = 0;
+    bmfr::FitterConfig fitter{16, 1};
+    int grid_cap = 1024;
+};
+
+extern "C" {
+
+int32_t bmfr_server_create(int64_t hbm_budget_bytes, bmfr_server *out) {
+    if (!out) return fail(BMFR_ERR_INVALID_ARGUMENT, "out is null");
+    if (hbm_budget_bytes <= 0) return fail(BMFR_ERR_INVALID_ARGUMENT, "budget must be positive");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(BMFR_ERR_NO_DEVICE, "no HIP device visible");
+    bmfr_server sv = new (std::nothrow) bmfr_server_s;
+    if (!sv) return fail(BMFR_ERR_INVALID_ARGUMENT, "host allocation failed");
+    sv->budget = hbm_budget_bytes;
+    hipError_t e = hipGetDevice(&sv->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking);
+    const size_t G = bmfr_server_s::kGroups, M = bmfr::kMaxBatch;
+    if (e == hipSuccess) e = hipMalloc((void **)&sv->d_streams, sizeof(bmfr::BatchStream) * G * M);
+    if (e == hipSuccess) e = hipMalloc((void **)&sv->d_steps, sizeof(bmfr::BatchStep) * G * M);
+    if (e == hipSuccess) e = hipMalloc((void **)&sv->d_starts, sizeof(int32_t) * G * bmfr::kStartKinds * (M + 1));
+    if (e == hipSuccess)
+        e = hipHostMalloc((void **)&sv->h_stage, sizeof(bmfr_server_s::Staging) * bmfr_server_s::kRing);
+    for (int i = 0; i < bmfr_server_s::kRing && e == hipSuccess; ++i) {
+        e = hipEventCreateWithFlags(&sv->ring_ev[i], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreate(&sv->step_start[i]);
+        if (e == hipSuccess) e = hipEventCreate(&sv->step_end[i]);
+    }
+    if (e != hipSuccess) {
+        bmfr_server_destroy(sv);
+        return fail(BMFR_ERR_HIP, "server set-up failed: %s", hipGetErrorString(e));
+    }
+    *out = sv;
+    return BMFR_OK;
+}
+
+int32_t bmfr_server_destroy(bmfr_server sv) {
+    if (!sv) return BMFR_OK;
+    (void)hipSetDevice(sv->device);
+    if (sv->stream) (void)hipStreamSynchronize(sv->stream);
+    for (bmfr_stream s : sv->open) bmfr_stream_close(s);
+    sv->open.clear();
+    for (int i = 0; i < bmfr_server_s::kRing; ++i) {
+        if (sv->ring_ev[i]) (void)hipEventDestroy(sv->ring_ev[i]);
+        if (sv->step_start[i]) (void)hipEventDestroy(sv->step_start[i]);
+        if (sv->step_end[i]) (void)hipEventDestroy(sv->step_end[i]);
+    }
+    if (sv->h_stage) (void)hipHostFree(sv->h_stage);
+    if (sv->d_streams) (void)hipFree(sv->d_streams);
+    if (sv->d_steps) (void)hipFree(sv->d_steps);
+    if (sv->d_starts) (void)hipFree(sv->d_starts);
+    if (sv->stream) (void)hipStreamDestroy(sv->stream);
+    delete sv;
+    return BMFR_OK;
+}
+
+int32_t bmfr_server_usage(bmfr_server sv, int64_t *used, int64_t *budget, int32_t *streams) {
+    if (!sv) return fail(BMFR_ERR_INVALID_ARGUMENT, "server is null");
+    if (used) *used = sv->used;
+    if (budget) *budget = sv->budget;
+    if (streams) *streams = (int32_t)sv->open.size();
+    return BMFR_OK;
+}
+
+int32_t bmfr_server_open_stream(bmfr_server sv, const bmfr_params *params, bmfr_stream *out) {
+    if (!sv || !out) return fail(BMFR_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    bmfr_sizes z;
+    if (int32_t st = bmfr_compute_sizes(params, &z)) return st;
+    if (sv->used + z.device_bytes > sv->budget)
+        return fail(BMFR_ERR_OUT_OF_BUDGET, "stream needs %lld bytes, %lld of %lld in use",
+                    (long long)z.device_bytes, (long long)sv->used, (long long)sv->budget);
+    if ((int)sv->open.size() >= 4 * bmfr::kMaxBatch)
+        return fail(BMFR_ERR_OUT_OF_BUDGET, "too many open streams");
+    HIP_TRY(hipSetDevice(sv->device));
+    bmfr_stream s;
+    if (int32_t st = bmfr_stream_open(params, &s)) return st;
+    sv->open.push_back(s);
+    sv->used += z.device_bytes;
+    *out = s;
+    return BMFR_OK;
+}
+
+int32_t bmfr_server_close_stream(bmfr_server sv, bmfr_stream s) {
+    if (!sv || !s) return fail(BMFR_ERR_INVALID_ARGUMENT, "null argument");
+    auto it = std::find(sv->open.begin(), sv->open.end(), s);
+    if (it == sv->open.end()) return fail(BMFR_ERR_INVALID_ARGUMENT, "stream does not belong to this server");
+    sv->open.erase(it);
+    sv->used -= s->sz.device_bytes;
+    // bmfr_stream_close synchronises the stream's own HIP stream, which waits
+    // for every server step the stream took part in.
+    return bmfr_stream_close(s);
+}
+
+int32_t bmfr_server_step(bmfr_server sv, int32_t count, const bmfr_stream *streams, const int32_t *frames,
+                         const float *matrices, const float *offsets, int32_t *step_id) {
+    if (!sv || !streams || !frames || !matrices || !offsets)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "null argument");
+    if (count < 1 || count > bmfr::kMaxBatch)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "step of %d streams outside 1..%d", count, bmfr::kMaxBatch);
+    for (int i = 0; i < count; ++i) {
+        if (!streams[i] || frames[i] < 0) return fail(BMFR_ERR_INVALID_ARGUMENT, "bad stream or frame %d", i);
+        if (std::find(sv->open.begin(), sv->open.end(), streams[i]) == sv->open.end())
+            return fail(BMFR_ERR_INVALID_ARGUMENT, "stream %d was not opened by this server", i);
+        for (int j = 0; j < i; ++j)
+            if (streams[j] == streams[i]) return fail(BMFR_ERR_INVALID_ARGUMENT, "stream %d listed twice", i);
+    }
+    HIP_TRY(hipSetDevice(sv->device));
+    const int slot = sv->ring;
+    sv->ring = (sv->ring + 1) % bmfr_server_s::kRing;
+    HIP_TRY(hipEventSynchronize(sv->ring_ev[slot]));  // staging slot free again
+    bmfr_server_s::Staging &hs = sv->h_stage[slot];
+    int gcount[bmfr_server_s::kGroups] = {0, 0, 0, 0};
+    for (int i = 0; i < count; ++i) {
+        bmfr_stream s = streams[i];
+        const int g = (s->p.storage == BMFR_STORAGE_F16 ? 2 : 0) + (s->p.tmp_storage == BMFR_STORAGE_F16 ? 1 : 0);
+        const int m = gcount[g]++;
+        bmfr::BatchStream &d = hs.streams[g][m];
+        d.args = s->args(0);
+        d.albedo = s->slot[S_ALBEDO];
+        d.noisy_in = s->slot[S_NOISY_IN];
+        d.prev_pixel = s->slot[S_PREV_PIXEL];
+        d.accept = s->slot[S_ACCEPT];
+        d.tmp_data = s->slot[S_TMP];
+        d.weights = s->slot[S_WEIGHTS];
+        d.mins_maxs = s->slot[S_MINS_MAXS];
+        d.filtered = s->slot[S_FILTERED];
+        d.tone_mapped = s->slot[S_TONE];
+        d.noise = s->slot[S_NOISE];
+        for (int h = 0; h < 2; ++h) {
+            d.normals[h] = s->slot[S_NORMALS_A + h];
+            d.positions[h] = s->slot[S_POSITIONS_A + h];
+            d.noisy_acc[h] = s->slot[S_NOISY_ACC_A + h];
+            d.spp[h] = s->slot[S_SPP_A + h];
+            d.out[h] = s->slot[S_OUT_A + h];
+            d.result[h] = s->slot[S_RESULT_A + h];
+        }
+        bmfr::BatchStep &st = hs.steps[g][m];
+        st.frame = frames[i];
+        st.cur = s->swapped ? 0 : 1;
+        std::memcpy(st.matrix, matrices + 16 * i, sizeof(st.matrix));
+        st.pixel_offset_x = offsets[2 * i];
+        st.pixel_offset_y = offsets[2 * i + 1];
+        int32_t(&sts)[bmfr::kStartKinds][bmfr::kMaxBatch + 1] = hs.starts[g];
+        if (m == 0) sts[0][0] = sts[1][0] = sts[2][0] = 0;
+        sts[bmfr::kStartsPadded][m + 1] = sts[bmfr::kStartsPadded][m] + (s->sz.margin_width / 32) * (s->sz.margin_height / 8);
+        sts[bmfr::kStartsImage][m + 1] = sts[bmfr::kStartsImage][m] + ((s->p.width + 63) / 64) * ((s->p.height + 3) / 4);
+        sts[bmfr::kStartsBlocks][m + 1] = sts[bmfr::kStartsBlocks][m] + s->sz.blocks;
+        // order after this stream's pending work (its inputs)
+        HIP_TRY(hipEventRecord(s->sync_ev, s->stream));
+        HIP_TRY(hipStreamWaitEvent(sv->stream, s->sync_ev, 0));
+    }
+    const size_t M = bmfr::kMaxBatch;
+    for (int g = 0; g < bmfr_server_s::kGroups; ++g) {
+        if (!gcount[g]) continue;
+        HIP_TRY(hipMemcpyAsync(sv->d_streams + g * M, hs.streams[g], sizeof(bmfr::BatchStream) * gcount[g],
+                               hipMemcpyHostToDevice, sv->stream));
+        HIP_TRY(hipMemcpyAsync(sv->d_steps + g * M, hs.steps[g], sizeof(bmfr::BatchStep) * gcount[g],
+                               hipMemcpyHostToDevice, sv->stream));
+        HIP_TRY(hipMemcpyAsync(sv->d_starts + g * bmfr::kStartKinds * (M + 1), hs.starts[g],
+                               sizeof(hs.starts[g]), hipMemcpyHostToDevice, sv->stream));
+    }
+    HIP_TRY(hipEventRecord(sv->ring_ev[slot], sv->stream));
+    HIP_TRY(hipEventRecord(sv->step_start[slot], sv->stream));
+    for (int st = 0; st < kStages; ++st) {
+        for (int g = 0; g < bmfr_server_s::kGroups; ++g) {
+            if (!gcount[g]) continue;
+            bmfr::BatchLaunch L;
+            L.streams = sv->d_streams + g * M;
+            L.steps = sv->d_steps + g * M;
+            L.starts = sv->d_starts + g * bmfr::kStartKinds * (M + 1);
+            L.n = gcount[g];
+            const bool g16 = g >= 2, t16 = g & 1;
+            const int32_t(&sts)[bmfr::kStartKinds][bmfr::kMaxBatch + 1] = hs.starts[g];
+            hipError_t e;
+            switch (st) {
+                case 0: e = bmfr::launch_batch_accumulate_noisy_data(L, sts[0][L.n], g16, t16, sv->stream); break;
+                case 1: e = bmfr::launch_batch_fitter(L, sts[2][L.n], t16, sv->fitter, sv->grid_cap, sv->stream); break;
+                case 2: e = bmfr::launch_batch_weighted_sum(L, sts[1][L.n], g16, sv->stream); break;
+                case 3: e = bmfr::launch_batch_accumulate_filtered_data(L, sts[1][L.n], g16, sv->stream); break;
+                default: e = bmfr::launch_batch_taa(L, sts[1][L.n], g16, sv->stream); break;
+            }
+            if (e != hipSuccess) return fail(BMFR_ERR_HIP, "server stage %d launch failed: %s", st, hipGetErrorString(e));
+        }
+    }
+    HIP_TRY(hipEventRecord(sv->step_end[slot], sv->stream));
+    for (int i = 0; i < count; ++i) {
+        bmfr_stream s = streams[i];
+        HIP_TRY(hipStreamWaitEvent(s->stream, sv->step_end[slot], 0));
+        s->last_result = s->cur(S_RESULT_A);
+        s->noise_frame = frames[i];
+        s->swapped = !s->swapped;
+    }
+    if (step_id) *step_id = slot;
+    ++sv->steps_launched;
+    return BMFR_OK;
+}
+
+// Completion of a step launched by bmfr_server_step: *done = 1 and its GPU
+// time in *ms once it has finished, *done = 0 while it runs.  Step ids are
+// reused after kRing (4) further steps.
+int32_t bmfr_server_step_query(bmfr_server sv, int32_t step_id, int32_t *done, float *ms) {
+    if (!sv || !done || step_id < 0 || step_id >= bmfr_server_s::kRing)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "bad arguments");
+    HIP_TRY(hipSetDevice(sv->device));
+    const hipError_t q = hipEventQuery(sv->step_end[step_id]);
+    if (q == hipErrorNotReady) {
+        *done = 0;
+        return BMFR_OK;
+    }
+    if (q != hipSuccess) return fail(BMFR_ERR_HIP, "step query failed: %s", hipGetErrorString(q));
+    *done = 1;
+    if (ms) HIP_TRY(hipEventElapsedTime(ms, sv->step_start[step_id], sv->step_end[step_id]));
+    return BMFR_OK;
+}
+
+int32_t bmfr_server_synchronize(bmfr_server sv) {
+    if (!sv) return fail(BMFR_ERR_INVALID_ARGUMENT, "server is null");
+    HIP_TRY(hipSetDevice(sv->device));
+    HIP_TRY(hipStreamSynchronize(sv->stream));
     return BMFR_OK;
 }
 

@@ -95,6 +95,7 @@ typedef enum bmfr_buffer {
 } bmfr_buffer;
 
 typedef struct bmfr_stream_s *bmfr_stream;
+typedef struct bmfr_server_s *bmfr_server;
 
 /* Sizes derived exactly as opencl/bmfr.cpp:104-118. */
 typedef struct bmfr_sizes {
@@ -226,6 +227,27 @@ int32_t bmfr_batch_autotune(bmfr_batch batch, int32_t reps, char *log, size_t lo
 /* Markers 0..4 on the batch's HIP stream (shared with the stage events). */
 int32_t bmfr_batch_record_marker(bmfr_batch batch, int32_t marker);
 int32_t bmfr_batch_marker_elapsed(bmfr_batch batch, int32_t m0, int32_t m1, float *ms);
+
+/* ---- server: streams admitted by HBM budget, continuous batching ------------
+ * The server owns the streams it admits (bmfr_server_open_stream refuses with
+ * BMFR_ERR_OUT_OF_BUDGET when the stream's device footprint would exceed the
+ * budget) and runs dynamic batched steps: each bmfr_server_step takes one
+ * frame of any subset of its streams (the ones with work pending), one
+ * launch per stage per storage group, ordered after each member's pending
+ * work (its inputs) and before its later work.  Steps are asynchronous; a
+ * step id (reused after 4 steps) can be polled for completion and GPU time.
+ * No counterpart in the reference (one stream, opencl/bmfr.cpp:417-485). */
+int32_t bmfr_server_create(int64_t hbm_budget_bytes, bmfr_server *out);
+int32_t bmfr_server_destroy(bmfr_server server);
+int32_t bmfr_server_usage(bmfr_server server, int64_t *used_bytes, int64_t *budget_bytes,
+                          int32_t *open_streams);
+int32_t bmfr_server_open_stream(bmfr_server server, const bmfr_params *params, bmfr_stream *out);
+int32_t bmfr_server_close_stream(bmfr_server server, bmfr_stream stream);
+int32_t bmfr_server_step(bmfr_server server, int32_t count, const bmfr_stream *streams,
+                         const int32_t *frames, const float *matrices, const float *offsets,
+                         int32_t *step_id);
+int32_t bmfr_server_step_query(bmfr_server server, int32_t step_id, int32_t *done, float *gpu_ms);
+int32_t bmfr_server_synchronize(bmfr_server server);
 
 /* ---- procedural scene (replaces the EXR dataset, opencl/bmfr.cpp:145-172) ---- */
 typedef struct bmfr_scene_s *bmfr_scene;

@@ -150,6 +150,17 @@ class Stream:
     def fitter(self, frame):
         _check(self._lib.bmfr_fitter(self._h, frame))
 
+    def fitter_fp64_weights(self):
+        """FP64 least-squares weights of every block of the last fitter call."""
+        out = np.empty(30 * self.sizes.blocks, np.float32)
+        _check(self._lib.bmfr_fitter_fp64_weights(self._h, _fp(out)))
+        return out
+
+    def refactor_count(self):
+        c = ctypes.c_int32()
+        _check(self._lib.bmfr_refactor_count(self._h, ctypes.byref(c)))
+        return c.value
+
     def weighted_sum(self, frame):
         _check(self._lib.bmfr_weighted_sum(self._h, frame))
 

@@ -65,6 +65,8 @@ SIGNATURES = [
     ("bmfr_accumulate_noisy_data", c_int32, [ctypes.c_void_p, c_int32, c_float_p, c_float_p]),
     ("bmfr_fitter", c_int32, [ctypes.c_void_p, c_int32]),
     ("bmfr_weighted_sum", c_int32, [ctypes.c_void_p, c_int32]),
+    ("bmfr_fitter_fp64_weights", c_int32, [ctypes.c_void_p, c_float_p]),
+    ("bmfr_refactor_count", c_int32, [ctypes.c_void_p, ctypes.POINTER(c_int32)]),
     ("bmfr_accumulate_filtered_data", c_int32, [ctypes.c_void_p, c_int32]),
     ("bmfr_taa", c_int32, [ctypes.c_void_p, c_int32]),
     ("bmfr_end_frame", c_int32, [ctypes.c_void_p]),

@@ -61,6 +61,8 @@ enum Slot {
     S_OUT_A, S_OUT_B,
     S_RESULT_A, S_RESULT_B,
     S_NOISE,   // the fitter's per-frame feature-noise table (not a reference buffer)
+    S_FLAGS,   // per-block FP64 re-factorisation flags
+    S_W64,     // FP64 least-squares weights (diagnostic)
     S_COUNT
 };
 
@@ -119,6 +121,8 @@ void sizes_of(const bmfr_params &p, bmfr_sizes &s, size_t *slot_bytes) {
     b[S_OUT_A] = b[S_OUT_B] = 3 * g * P;
     b[S_RESULT_A] = b[S_RESULT_B] = 3 * g * P;
     b[S_NOISE] = 9 * 1024 * sizeof(double);
+    b[S_FLAGS] = 4 * (size_t)s.blocks;
+    b[S_W64] = 30 * 4 * (size_t)s.blocks;
     size_t total = 0;
     for (int i = 0; i < S_COUNT; ++i) {
         if (slot_bytes) slot_bytes[i] = b[i];
@@ -201,6 +205,7 @@ struct bmfr_stream_s {
         b.result_cur = slot[cur(S_RESULT_A)];
         b.result_prev = slot[prev(S_RESULT_A)];
         b.noise = slot[S_NOISE];
+        b.flags = slot[S_FLAGS];
         return b;
     }
 
@@ -221,6 +226,8 @@ struct bmfr_stream_s {
         a.frame = frame;
         a.storage_f16 = p.storage == BMFR_STORAGE_F16;
         a.tmp_f16 = p.tmp_storage == BMFR_STORAGE_F16;
+        a.qr_mode = p.qr_mode;
+        a.refactor_threshold = p.refactor_threshold;
         return a;
     }
 };
@@ -481,6 +488,33 @@ int32_t bmfr_fitter(bmfr_stream s, int32_t frame) {
     return timed_stage(s, 1, [&] { return bmfr::launch_fitter(a, b, s->fitter, s->p.qr_mode, s->stream); });
 }
 
+int32_t bmfr_fitter_fp64_weights(bmfr_stream s, float *out) {
+    if (int32_t st = check_stream(s)) return st;
+    if (!out) return fail(BMFR_ERR_INVALID_ARGUMENT, "out is null");
+    if (s->noise_frame < 0) return fail(BMFR_ERR_STATE, "the fitter has not run yet");
+    const bmfr::StageArgs a = s->args(s->noise_frame);
+    const bmfr::StageBuffers b = s->buffers();
+    HIP_TRY(hipSetDevice(s->device));
+    hipError_t e = bmfr::launch_fitter_fp64_all(a, b, (float *)s->slot[S_W64], s->stream);
+    if (e != hipSuccess) return fail(BMFR_ERR_HIP, "FP64 fitter launch failed: %s", hipGetErrorString(e));
+    HIP_TRY(hipMemcpyAsync(out, s->slot[S_W64], s->slot_bytes[S_W64], hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    return BMFR_OK;
+}
+
+int32_t bmfr_refactor_count(bmfr_stream s, int32_t *count) {
+    if (int32_t st = check_stream(s)) return st;
+    if (!count) return fail(BMFR_ERR_INVALID_ARGUMENT, "count is null");
+    std::vector<int32_t> f((size_t)s->sz.blocks);
+    HIP_TRY(hipSetDevice(s->device));
+    HIP_TRY(hipMemcpyAsync(f.data(), s->slot[S_FLAGS], f.size() * 4, hipMemcpyDeviceToHost, s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    int32_t c = 0;
+    for (int32_t v : f) c += v != 0;
+    *count = c;
+    return BMFR_OK;
+}
+
 int32_t bmfr_weighted_sum(bmfr_stream s, int32_t frame) {
     if (int32_t st = check_stream(s)) return st;
     if (int32_t st = check_frame(frame)) return st;
@@ -636,6 +670,7 @@ namespace {
 
 struct BatchGroup {
     bool g16 = false, t16 = false;
+    int qr = 0;
     std::vector<int> members;            // indices into bmfr_batch_s::streams
     bmfr::BatchStream *d_streams = nullptr;
     bmfr::BatchStep *d_steps = nullptr;
@@ -689,7 +724,7 @@ int32_t batch_enqueue_stages(bmfr_batch b, bool timed) {
                     break;
                 case 1:
                     e = bmfr::launch_batch_fitter(L, g.starts[bmfr::kStartsBlocks][L.n], g.t16, b->fitter,
-                                                  b->grid_cap, b->stream);
+                                                  b->grid_cap, g.qr, b->stream);
                     break;
                 case 2:
                     e = bmfr::launch_batch_weighted_sum(L, g.starts[bmfr::kStartsImage][L.n], g.g16, b->stream);
@@ -732,14 +767,16 @@ int32_t bmfr_batch_create(const bmfr_stream *streams, int32_t count, bmfr_batch 
     for (int i = 0; i < count; ++i) {
         const bool g16 = streams[i]->p.storage == BMFR_STORAGE_F16;
         const bool t16 = streams[i]->p.tmp_storage == BMFR_STORAGE_F16;
+        const int qr = streams[i]->p.qr_mode;
         BatchGroup *grp = nullptr;
         for (auto &g : b->groups)
-            if (g.g16 == g16 && g.t16 == t16) grp = &g;
+            if (g.g16 == g16 && g.t16 == t16 && g.qr == qr) grp = &g;
         if (!grp) {
             b->groups.emplace_back();
             grp = &b->groups.back();
             grp->g16 = g16;
             grp->t16 = t16;
+            grp->qr = qr;
         }
         grp->members.push_back(i);
     }
@@ -774,6 +811,7 @@ int32_t bmfr_batch_create(const bmfr_stream *streams, int32_t count, bmfr_batch 
             d.filtered = s->slot[S_FILTERED];
             d.tone_mapped = s->slot[S_TONE];
             d.noise = s->slot[S_NOISE];
+            d.flags = s->slot[S_FLAGS];
             for (int h = 0; h < 2; ++h) {
                 d.normals[h] = s->slot[S_NORMALS_A + h];
                 d.positions[h] = s->slot[S_POSITIONS_A + h];
@@ -967,7 +1005,7 @@ int32_t bmfr_batch_autotune(bmfr_batch b, int32_t reps, char *log, size_t log_le
             HIP_TRY(hipEventRecord(b->ev_start[1], b->stream));
             for (auto &g : b->groups) {
                 hipError_t e = bmfr::launch_batch_fitter(g.launch(), g.starts[bmfr::kStartsBlocks][g.members.size()],
-                                                         g.t16, b->fitter, b->grid_cap, b->stream);
+                                                         g.t16, b->fitter, b->grid_cap, g.qr, b->stream);
                 if (e != hipSuccess) {
                     b->fitter = saved;
                     b->grid_cap = saved_cap;
@@ -1032,8 +1070,9 @@ struct bmfr_server_s {
     int64_t budget = 0, used = 0;
     std::vector<bmfr_stream> open;
     hipStream_t stream = nullptr;
-    // device tables: one slot per storage group (F32/F16 x tmp F32/F16)
-    static constexpr int kGroups = 4;
+    // device tables: one slot per launch group (storage F32/F16 x tmp
+    // F32/F16 x qr mode)
+    static constexpr int kGroups = 12;
     bmfr::BatchStream *d_streams = nullptr;  // [kGroups][kMaxBatch]
     bmfr::BatchStep *d_steps = nullptr;      // [kGroups][kMaxBatch]
     int32_t *d_starts = nullptr;             // [kGroups][kStartKinds][kMaxBatch+1]
@@ -1160,10 +1199,11 @@ int32_t bmfr_server_step(bmfr_server sv, int32_t count, const bmfr_stream *strea
     sv->ring = (sv->ring + 1) % bmfr_server_s::kRing;
     HIP_TRY(hipEventSynchronize(sv->ring_ev[slot]));  // staging slot free again
     bmfr_server_s::Staging &hs = sv->h_stage[slot];
-    int gcount[bmfr_server_s::kGroups] = {0, 0, 0, 0};
+    int gcount[bmfr_server_s::kGroups] = {};
     for (int i = 0; i < count; ++i) {
         bmfr_stream s = streams[i];
-        const int g = (s->p.storage == BMFR_STORAGE_F16 ? 2 : 0) + (s->p.tmp_storage == BMFR_STORAGE_F16 ? 1 : 0);
+        const int g = ((s->p.storage == BMFR_STORAGE_F16 ? 2 : 0) + (s->p.tmp_storage == BMFR_STORAGE_F16 ? 1 : 0)) * 3 +
+                      s->p.qr_mode;
         const int m = gcount[g]++;
         bmfr::BatchStream &d = hs.streams[g][m];
         d.args = s->args(0);
@@ -1177,6 +1217,7 @@ int32_t bmfr_server_step(bmfr_server sv, int32_t count, const bmfr_stream *strea
         d.filtered = s->slot[S_FILTERED];
         d.tone_mapped = s->slot[S_TONE];
         d.noise = s->slot[S_NOISE];
+        d.flags = s->slot[S_FLAGS];
         for (int h = 0; h < 2; ++h) {
             d.normals[h] = s->slot[S_NORMALS_A + h];
             d.positions[h] = s->slot[S_POSITIONS_A + h];
@@ -1220,12 +1261,14 @@ int32_t bmfr_server_step(bmfr_server sv, int32_t count, const bmfr_stream *strea
             L.steps = sv->d_steps + g * M;
             L.starts = sv->d_starts + g * bmfr::kStartKinds * (M + 1);
             L.n = gcount[g];
-            const bool g16 = g >= 2, t16 = g & 1;
+            const int sg = g / 3;
+            const bool g16 = sg >= 2, t16 = sg & 1;
+            const int qr = g % 3;
             const int32_t(&sts)[bmfr::kStartKinds][bmfr::kMaxBatch + 1] = hs.starts[g];
             hipError_t e;
             switch (st) {
                 case 0: e = bmfr::launch_batch_accumulate_noisy_data(L, sts[0][L.n], g16, t16, sv->stream); break;
-                case 1: e = bmfr::launch_batch_fitter(L, sts[2][L.n], t16, sv->fitter, sv->grid_cap, sv->stream); break;
+                case 1: e = bmfr::launch_batch_fitter(L, sts[2][L.n], t16, sv->fitter, sv->grid_cap, qr, sv->stream); break;
                 case 2: e = bmfr::launch_batch_weighted_sum(L, sts[1][L.n], g16, sv->stream); break;
                 case 3: e = bmfr::launch_batch_accumulate_filtered_data(L, sts[1][L.n], g16, sv->stream); break;
                 default: e = bmfr::launch_batch_taa(L, sts[1][L.n], g16, sv->stream); break;

@@ -64,24 +64,60 @@ __device__ __forceinline__ float wave_reduce(float v) {
 
 // Reduce NV values over the W waves of one block.  `red` points at this
 // block's [W][16] slot of the round's LDS buffer.
-template <int W, int NV, typename Op>
+template <int W, int NV, typename Op, int RS = 16>
 __device__ __forceinline__ void block_reduce(float (&v)[NV], float *red, int wave, int lane) {
 #pragma unroll
     for (int i = 0; i < NV; ++i) v[i] = wave_reduce<Op>(v[i]);
     if constexpr (W > 1) {
         if (lane == 0) {
 #pragma unroll
-            for (int i = 0; i < NV; ++i) red[wave * 16 + i] = v[i];
+            for (int i = 0; i < NV; ++i) red[wave * RS + i] = v[i];
         }
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < NV; ++i) {
             float acc = red[i];
 #pragma unroll
-            for (int w = 1; w < W; ++w) acc = Op::f(acc, red[w * 16 + i]);
+            for (int w = 1; w < W; ++w) acc = Op::f(acc, red[w * RS + i]);
             v[i] = acc;
         }
     }
+}
+
+// Error-free transformations for the compensated (BMFR_QR_KAHAN) sums.
+__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
+    s = a + b;
+    const float z = s - a;
+    e = (a - (s - z)) + (b - z);
+}
+// s + c += x * y with the product's and the sum's rounding errors kept in c.
+__device__ __forceinline__ void dot2_acc(float &s, float &c, float x, float y) {
+    const float p = x * y;
+    const float pe = __builtin_fmaf(x, y, -p);
+    float t, e;
+    two_sum(s, p, t, e);
+    s = t;
+    c = c + (e + pe);
+}
+// 64-lane reduction of a compensated pair; the result (s + c) is uniform.
+__device__ __forceinline__ float wave_reduce_pair(float s, float c) {
+    float t, e;
+#define BMFR_PAIR_STEP(S2, C2)  \
+    {                           \
+        const float s2 = (S2);  \
+        const float c2 = (C2);  \
+        two_sum(s, s2, t, e);   \
+        s = t;                  \
+        c = c + (c2 + e);       \
+    }
+    BMFR_PAIR_STEP(dpp_all<0xB1>(s), dpp_all<0xB1>(c));
+    BMFR_PAIR_STEP(dpp_all<0x4E>(s), dpp_all<0x4E>(c));
+    BMFR_PAIR_STEP(dpp_all<0x141>(s), dpp_all<0x141>(c));
+    BMFR_PAIR_STEP(dpp_all<0x140>(s), dpp_all<0x140>(c));
+    BMFR_PAIR_STEP((dpp<0x142, 0xA>(0.f, s)), (dpp<0x142, 0xA>(0.f, c)));
+    BMFR_PAIR_STEP((dpp<0x143, 0xC>(0.f, s)), (dpp<0x143, 0xC>(0.f, c)));
+#undef BMFR_PAIR_STEP
+    return readlane(s, 63) + readlane(c, 63);
 }
 
 template <typename TT, int V>
@@ -114,8 +150,16 @@ struct FitterParams {
     float *mins_maxs;
     float *weights;
     const double *noise;   // the frame's noise table (kNoiseTable terms)
+    int32_t *flags;        // per block: 1 = re-factor in FP64 (null: not wanted)
+    float threshold;       // flag when min |R_ii| < threshold * max |R_ii|
     int32_t blocks;
     int32_t frame;
+};
+
+// Per-block outputs of the FP32 fitter besides weights and mins_maxs.
+struct BlockFlag {
+    int32_t *flag;
+    float threshold;
 };
 
 // One 32x32 block: RPL rows per lane, BPG blocks per work-group, TT tmp
@@ -123,10 +167,16 @@ struct FitterParams {
 // reference's USE_HALF_PRECISION_IN_TMP_DATA=1 (bmfr.cl:255-258, 540, 651).
 // `red` is this work-group's LDS exchange area; `active` false recomputes a
 // valid block (keeping barriers uniform) without storing.
-template <int RPL, int BPG, typename TT, bool HALF_ROUND>
+// COMP: compensated norm and dot sums (BMFR_QR_KAHAN).  `flag` (optional)
+// receives 1 when min |R_ii| < threshold * max |R_ii|.
+constexpr int red_slots(bool comp) { return comp ? 32 : 16; }
+
+template <int RPL, int BPG, typename TT, bool HALF_ROUND, bool COMP>
 __device__ __forceinline__ void fitter_block(const TT *__restrict__ src, float *__restrict__ mins_maxs,
                                              float *__restrict__ weights, const double *__restrict__ noise,
-                                             bool active, float (&red)[2][BPG][16 / RPL][16]) {
+                                             BlockFlag flag, bool active,
+                                             float (&red)[2][BPG][16 / RPL][red_slots(COMP)]) {
+    constexpr int RS = red_slots(COMP);
     constexpr int W = 16 / RPL;            // waves per block
     constexpr int T = 64 * W;              // lanes per block
     constexpr int V = (RPL < (int)(16 / sizeof(TT))) ? RPL : (int)(16 / sizeof(TT));
@@ -166,7 +216,7 @@ __device__ __forceinline__ void fitter_block(const TT *__restrict__ src, float *
             v[i] = mn;
             v[kScaled + i] = nmx;
         }
-        block_reduce<W, 2 * kScaled, OpMin>(v, &red[round & 1][sub][0][0], wave, lane);
+        block_reduce<W, 2 * kScaled, OpMin, RS>(v, &red[round & 1][sub][0][0], wave, lane);
         ++round;
 #pragma unroll
         for (int i = 0; i < kScaled; ++i) {
@@ -228,31 +278,45 @@ __device__ __forceinline__ void fitter_block(const TT *__restrict__ src, float *
     for (int col = 0; col < kFeatures; ++col) {
         // Pivot row `col` sits in lane col/V (wave 0), element col%V.
         float r1[1];
-        {
+        float xc = readlane(a[col][col % V], col / V);
+        if constexpr (!COMP) {
             float s = 0.f;
 #pragma unroll
             for (int k = 0; k < RPL; ++k) {
                 const float x = row_of(k) > col ? a[col][k] : 0.f;
                 s = __builtin_fmaf(x, x, s);
             }
-            r1[0] = s;
+            r1[0] = wave_reduce<OpAdd>(s);
+        } else {
+            float s = 0.f, c = 0.f;
+#pragma unroll
+            for (int k = 0; k < RPL; ++k) {
+                const float x = row_of(k) > col ? a[col][k] : 0.f;
+                dot2_acc(s, c, x, x);
+            }
+            r1[0] = wave_reduce_pair(s, c);
         }
-        float xc = readlane(a[col][col % V], col / V);
         if constexpr (W > 1) {
-            r1[0] = wave_reduce<OpAdd>(r1[0]);
             float *rd = &red[round & 1][sub][0][0];
             if (lane == 0) {
-                rd[wave * 16 + 0] = r1[0];
+                rd[wave * RS + 0] = r1[0];
                 if (wave == 0) rd[1] = xc;
             }
             __syncthreads();
-            float acc = rd[0];
+            float acc = rd[0], cc = 0.f;
 #pragma unroll
-            for (int w = 1; w < W; ++w) acc = acc + rd[w * 16];
-            r1[0] = acc;
+            for (int w = 1; w < W; ++w) {
+                if constexpr (COMP) {
+                    float t, e;
+                    two_sum(acc, rd[w * RS], t, e);
+                    acc = t;
+                    cc = cc + e;
+                } else {
+                    acc = acc + rd[w * RS];
+                }
+            }
+            r1[0] = acc + cc;
             xc = rd[1];
-        } else {
-            r1[0] = wave_reduce<OpAdd>(r1[0]);
         }
         ++round;
         const float sigma = r1[0];
@@ -271,34 +335,49 @@ __device__ __forceinline__ void fitter_block(const TT *__restrict__ src, float *
         float d[kMaxTrail];
 #pragma unroll
         for (int j = 0; j < kMaxTrail; ++j) d[j] = 0.f;
-#pragma unroll
-        for (int j = col + 1; j < kBufferCount; ++j) {
-            float s = 0.f;
-#pragma unroll
-            for (int k = 0; k < RPL; ++k) s = __builtin_fmaf(u[k], a[j][k], s);
-            d[j - col - 1] = s;
-        }
-        // reduce only the live trailing columns
         {
-            float *rd = &red[round & 1][sub][0][0];
             const int nv = kBufferCount - 1 - col;
+            if constexpr (!COMP) {
 #pragma unroll
-            for (int i = 0; i < kMaxTrail; ++i)
-                if (i < nv) d[i] = wave_reduce<OpAdd>(d[i]);
+                for (int j = col + 1; j < kBufferCount; ++j) {
+                    float s = 0.f;
+#pragma unroll
+                    for (int k = 0; k < RPL; ++k) s = __builtin_fmaf(u[k], a[j][k], s);
+                    d[j - col - 1] = wave_reduce<OpAdd>(s);
+                }
+            } else {
+#pragma unroll
+                for (int j = col + 1; j < kBufferCount; ++j) {
+                    float s = 0.f, c = 0.f;
+#pragma unroll
+                    for (int k = 0; k < RPL; ++k) dot2_acc(s, c, u[k], a[j][k]);
+                    d[j - col - 1] = wave_reduce_pair(s, c);
+                }
+            }
             if constexpr (W > 1) {
+                float *rd = &red[round & 1][sub][0][0];
                 if (lane == 0) {
 #pragma unroll
                     for (int i = 0; i < kMaxTrail; ++i)
-                        if (i < nv) rd[wave * 16 + i] = d[i];
+                        if (i < nv) rd[wave * RS + i] = d[i];
                 }
                 __syncthreads();
 #pragma unroll
                 for (int i = 0; i < kMaxTrail; ++i) {
                     if (i < nv) {
-                        float acc = rd[i];
+                        float acc = rd[i], cc = 0.f;
 #pragma unroll
-                        for (int w = 1; w < W; ++w) acc = acc + rd[w * 16 + i];
-                        d[i] = acc;
+                        for (int w = 1; w < W; ++w) {
+                            if constexpr (COMP) {
+                                float t, e;
+                                two_sum(acc, rd[w * RS + i], t, e);
+                                acc = t;
+                                cc = cc + e;
+                            } else {
+                                acc = acc + rd[w * RS + i];
+                            }
+                        }
+                        d[i] = acc + cc;
                     }
                 }
             }
@@ -317,6 +396,17 @@ __device__ __forceinline__ void fitter_block(const TT *__restrict__ src, float *
                 a[j][k] = HALF_ROUND ? round_half(v) : v;
             }
         }
+    }
+
+    // conditioning flag for the FP64 re-factorisation (BMFR_QR_KAHAN_FP64)
+    if (flag.flag && active && t == 0) {
+        float rmin = norms[0], rmax = norms[0];
+#pragma unroll
+        for (int i = 1; i < kFeatures; ++i) {
+            rmin = fminf(rmin, fabsf(norms[i]));
+            rmax = fmaxf(rmax, fabsf(norms[i]));
+        }
+        *flag.flag = (rmin < flag.threshold * rmax || !(rmin == rmin)) ? 1 : 0;
     }
 
     // ---- back substitution, bmfr.cl:658-699, in wave 0 ----
@@ -369,25 +459,25 @@ __device__ __forceinline__ void fitter_block(const TT *__restrict__ src, float *
     }
 }
 
-template <int RPL, int BPG, typename TT, bool HALF_ROUND>
+template <int RPL, int BPG, typename TT, bool HALF_ROUND, bool COMP>
 __global__ __launch_bounds__(64 * (16 / RPL) * BPG) void k_fitter(const FitterParams p) {
-    __shared__ float red[2][BPG][16 / RPL][16];
+    __shared__ float red[2][BPG][16 / RPL][red_slots(COMP)];
     const int sub = threadIdx.x / (64 * (16 / RPL));
     const int blk_raw = blockIdx.x * BPG + sub;
     const bool active = blk_raw < p.blocks;
     const int blk = active ? blk_raw : p.blocks - 1;
-    fitter_block<RPL, BPG, TT, HALF_ROUND>(
+    fitter_block<RPL, BPG, TT, HALF_ROUND, COMP>(
         reinterpret_cast<const TT *>(p.tmp) + (int64_t)blk * (kBufferCount * kBlockPixels),
         p.mins_maxs + (int64_t)blk * (kScaled * 2), p.weights + (int64_t)blk * (kFeatures * 3),
-        p.noise, active, red);
+        p.noise, BlockFlag{p.flags ? p.flags + blk : nullptr, p.threshold}, active, red);
 }
 
 // Persistent batched fitter: the work-groups stride over the concatenated
 // block list of every stream in the batch (edge blocks included).  The trip
 // count depends only on blockIdx, so barriers stay uniform.
-template <int RPL, int BPG, typename TT, bool HALF_ROUND>
+template <int RPL, int BPG, typename TT, bool HALF_ROUND, bool COMP>
 __global__ __launch_bounds__(64 * (16 / RPL) * BPG) void kb_fitter(const BatchLaunch L) {
-    __shared__ float red[2][BPG][16 / RPL][16];
+    __shared__ float red[2][BPG][16 / RPL][red_slots(COMP)];
     const int32_t *starts = L.starts + kStartsBlocks * (kMaxBatch + 1);
     const int total = starts[L.n];
     const int sub = threadIdx.x / (64 * (16 / RPL));
@@ -399,76 +489,264 @@ __global__ __launch_bounds__(64 * (16 / RPL) * BPG) void kb_fitter(const BatchLa
         while (s + 1 < L.n && starts[s + 1] <= u) ++s;
         const int blk = u - starts[s];
         const BatchStream &bs = L.streams[s];
-        fitter_block<RPL, BPG, TT, HALF_ROUND>(
+        int32_t *fl = (int32_t *)bs.flags;
+        fitter_block<RPL, BPG, TT, HALF_ROUND, COMP>(
             reinterpret_cast<const TT *>(bs.tmp_data) + (int64_t)blk * (kBufferCount * kBlockPixels),
             (float *)bs.mins_maxs + (int64_t)blk * (kScaled * 2),
-            (float *)bs.weights + (int64_t)blk * (kFeatures * 3), (const double *)bs.noise, active, red);
+            (float *)bs.weights + (int64_t)blk * (kFeatures * 3), (const double *)bs.noise,
+            BlockFlag{fl ? fl + blk : nullptr, bs.args.refactor_threshold}, active, red);
         // A block ends with round 20 in LDS buffer 0 and the next one starts
         // with round 0 in the same buffer: wait until every wave has read it.
         if constexpr (16 / RPL > 1) __syncthreads();
     }
 }
 
-template <int RPL, int BPG>
+// ---- FP64 re-factorisation (BMFR_QR_KAHAN_FP64) ------------------------------
+// One 256-thread work-group per flagged block (unflagged blocks exit at once).
+// The block matrix is rebuilt exactly as the FP32 fitter saw it at column 0
+// (the same scaled, noised float values), widened to double and kept in
+// registers (4 rows x 13 columns per thread); Householder QR with the stable
+// sign choice (v = x - alpha e_c, alpha = -sign(x_c) ||x||), then back
+// substitution, all in FP64.
+template <int NV>
+__device__ __forceinline__ void block_sum_f64(double (&v)[NV], double (*red)[4]) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v[i] += __shfl_xor(v[i], o);
+    __syncthreads();
+    if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < NV; ++i) red[i][wave] = v[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[i] = (red[i][0] + red[i][1]) + (red[i][2] + red[i][3]);
+}
+
+template <typename TT, bool HALF_ROUND>
+__device__ __forceinline__ void refactor_block(const TT *__restrict__ src, const float *__restrict__ mm,
+                                               const double *__restrict__ noise, float *__restrict__ w_out) {
+    __shared__ double red[16][4];
+    __shared__ double rl[kFeatures][kBufferCount];
+    const int t = threadIdx.x;
+    double a[kBufferCount][4];
+#pragma unroll
+    for (int f = 0; f < kBufferCount; ++f) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int row = k * 256 + t;
+            float v;
+            if constexpr (sizeof(TT) == 2) v = __half2float(src[f * kBlockPixels + row]);
+            else v = src[f * kBlockPixels + row];
+            if (f >= kNotScaled && f < kFeatures) {
+                const float mn = mm[(f - kNotScaled) * 2], mx = mm[(f - kNotScaled) * 2 + 1];
+                const float d = mx - mn;
+                float sc = v - mn;
+                if (fabsf(d) > 1.0f) {
+                    const float y = 1.0f / d;
+                    const float q = sc * y;
+                    const float r = __builtin_fmaf(-q, d, sc);
+                    sc = __builtin_fmaf(r, y, q);
+                }
+                v = HALF_ROUND ? round_half(sc) : sc;
+            }
+            if (f >= 1 && f < kFeatures) v = (float)((double)v + noise[(f - 1) * kBlockPixels + row]);
+            a[f][k] = (double)v;
+        }
+    }
+    double alpha[kFeatures];
+#pragma unroll
+    for (int c = 0; c < kFeatures; ++c) {
+        double r2[2];
+        r2[0] = 0.0;
+        r2[1] = 0.0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int row = k * 256 + t;
+            if (row >= c) r2[0] += a[c][k] * a[c][k];
+            if (row == c) r2[1] = a[c][k];  // exact: all other threads add zeros
+        }
+        block_sum_f64<2>(r2, red);
+        const double norm = sqrt(r2[0]), x0 = r2[1];
+        const double al = x0 >= 0.0 ? -norm : norm;
+        const double vc = x0 - al;
+        const double vv = 2.0 * norm * (norm + fabs(x0));
+        alpha[c] = al;
+        double v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int row = k * 256 + t;
+            v[k] = row > c ? a[c][k] : (row == c ? vc : 0.0);
+        }
+        double d[kBufferCount - 1];
+#pragma unroll
+        for (int j = 0; j < kBufferCount - 1; ++j) d[j] = 0.0;
+#pragma unroll
+        for (int j = c + 1; j < kBufferCount; ++j)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) d[j - c - 1] += v[k] * a[j][k];
+        block_sum_f64<kBufferCount - 1>(d, red);
+        if (vv > 0.0) {
+#pragma unroll
+            for (int j = c + 1; j < kBufferCount; ++j) {
+                const double f = 2.0 * d[j - c - 1] / vv;
+#pragma unroll
+                for (int k = 0; k < 4; ++k) a[j][k] -= f * v[k];
+            }
+        }
+    }
+    // R (rows 0..9) and Q^T b: thread i < 10 holds row i of every column
+    if (t < kFeatures) {
+#pragma unroll
+        for (int j = 0; j < kBufferCount; ++j) rl[t][j] = a[j][0];
+    }
+    __syncthreads();
+    if (t < 3) {
+        double w[kFeatures];
+#pragma unroll
+        for (int i = kFeatures - 1; i >= 0; --i) {
+            double acc = rl[i][kFeatures + t];
+#pragma unroll
+            for (int j = i + 1; j < kFeatures; ++j) acc -= rl[i][j] * w[j];
+            w[i] = acc / alpha[i];
+        }
+#pragma unroll
+        for (int i = 0; i < kFeatures; ++i) w_out[i * 3 + t] = (float)w[i];
+    }
+}
+
+struct RefactorParams {
+    const void *tmp;
+    const float *mins_maxs;
+    const double *noise;
+    const int32_t *flags;   // null: every block
+    float *out;             // weights (in place) or a separate FP64-solution buffer
+    int32_t blocks;
+};
+
+template <typename TT, bool HALF_ROUND>
+__global__ __launch_bounds__(256) void k_refactor(const RefactorParams p) {
+    const int blk = blockIdx.x;
+    if (blk >= p.blocks || (p.flags && !p.flags[blk])) return;
+    refactor_block<TT, HALF_ROUND>(reinterpret_cast<const TT *>(p.tmp) + (int64_t)blk * (kBufferCount * kBlockPixels),
+                                   p.mins_maxs + (int64_t)blk * (kScaled * 2), p.noise,
+                                   p.out + (int64_t)blk * (kFeatures * 3));
+}
+
+template <typename TT, bool HALF_ROUND>
+__global__ __launch_bounds__(256) void kb_refactor(const BatchLaunch L) {
+    const int32_t *starts = L.starts + kStartsBlocks * (kMaxBatch + 1);
+    const int u = blockIdx.x;
+    if (u >= starts[L.n]) return;
+    int s = 0;
+    while (s + 1 < L.n && starts[s + 1] <= u) ++s;
+    const int blk = u - starts[s];
+    const BatchStream &bs = L.streams[s];
+    if (!bs.flags || !((const int32_t *)bs.flags)[blk]) return;
+    refactor_block<TT, HALF_ROUND>(reinterpret_cast<const TT *>(bs.tmp_data) + (int64_t)blk * (kBufferCount * kBlockPixels),
+                                   (const float *)bs.mins_maxs + (int64_t)blk * (kScaled * 2),
+                                   (const double *)bs.noise, (float *)bs.weights + (int64_t)blk * (kFeatures * 3));
+}
+
+template <int RPL, int BPG, bool COMP>
 static hipError_t launch_rpl(const FitterParams &fp, bool tmp_f16, hipStream_t s) {
     constexpr int threads = 64 * (16 / RPL) * BPG;
     const dim3 grid((fp.blocks + BPG - 1) / BPG);
     if (tmp_f16)
-        hipLaunchKernelGGL((k_fitter<RPL, BPG, __half, true>), grid, dim3(threads), 0, s, fp);
+        hipLaunchKernelGGL((k_fitter<RPL, BPG, __half, true, COMP>), grid, dim3(threads), 0, s, fp);
     else
-        hipLaunchKernelGGL((k_fitter<RPL, BPG, float, false>), grid, dim3(threads), 0, s, fp);
+        hipLaunchKernelGGL((k_fitter<RPL, BPG, float, false, COMP>), grid, dim3(threads), 0, s, fp);
     return hipGetLastError();
 }
 
-template <int RPL, int BPG>
+template <int RPL, int BPG, bool COMP>
 static hipError_t launch_batch_rpl(const BatchLaunch &L, int total, bool tmp_f16, int grid_cap,
                                    hipStream_t s) {
     constexpr int threads = 64 * (16 / RPL) * BPG;
     int groups = (total + BPG - 1) / BPG;
     if (grid_cap > 0 && groups > grid_cap) groups = grid_cap;
     if (tmp_f16)
-        hipLaunchKernelGGL((kb_fitter<RPL, BPG, __half, true>), dim3(groups), dim3(threads), 0, s, L);
+        hipLaunchKernelGGL((kb_fitter<RPL, BPG, __half, true, COMP>), dim3(groups), dim3(threads), 0, s, L);
     else
-        hipLaunchKernelGGL((kb_fitter<RPL, BPG, float, false>), dim3(groups), dim3(threads), 0, s, L);
+        hipLaunchKernelGGL((kb_fitter<RPL, BPG, float, false, COMP>), dim3(groups), dim3(threads), 0, s, L);
+    return hipGetLastError();
+}
+
+template <bool COMP>
+static hipError_t dispatch_single(const FitterParams &fp, bool h, const FitterConfig &cfg, hipStream_t s) {
+    switch (cfg.rows_per_lane * 10 + cfg.blocks_per_group) {
+        case 41: return launch_rpl<4, 1, COMP>(fp, h, s);
+        case 42: return launch_rpl<4, 2, COMP>(fp, h, s);
+        case 81: return launch_rpl<8, 1, COMP>(fp, h, s);
+        case 82: return launch_rpl<8, 2, COMP>(fp, h, s);
+        case 161: return launch_rpl<16, 1, COMP>(fp, h, s);
+        case 162: return launch_rpl<16, 2, COMP>(fp, h, s);
+        case 164: return launch_rpl<16, 4, COMP>(fp, h, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <bool COMP>
+static hipError_t dispatch_batch(const BatchLaunch &L, int total, bool h, const FitterConfig &cfg,
+                                 int grid_cap, hipStream_t s) {
+    switch (cfg.rows_per_lane * 10 + cfg.blocks_per_group) {
+        case 41: return launch_batch_rpl<4, 1, COMP>(L, total, h, grid_cap, s);
+        case 42: return launch_batch_rpl<4, 2, COMP>(L, total, h, grid_cap, s);
+        case 81: return launch_batch_rpl<8, 1, COMP>(L, total, h, grid_cap, s);
+        case 82: return launch_batch_rpl<8, 2, COMP>(L, total, h, grid_cap, s);
+        case 161: return launch_batch_rpl<16, 1, COMP>(L, total, h, grid_cap, s);
+        case 162: return launch_batch_rpl<16, 2, COMP>(L, total, h, grid_cap, s);
+        case 164: return launch_batch_rpl<16, 4, COMP>(L, total, h, grid_cap, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+static hipError_t launch_refactor_all(const StageArgs &a, const StageBuffers &b, const int32_t *flags,
+                                      float *out, hipStream_t s) {
+    RefactorParams rp;
+    rp.tmp = b.tmp_data;
+    rp.mins_maxs = (const float *)b.mins_maxs;
+    rp.noise = (const double *)b.noise;
+    rp.flags = flags;
+    rp.out = out;
+    rp.blocks = a.blocks;
+    if (a.tmp_f16) hipLaunchKernelGGL((k_refactor<__half, true>), dim3(a.blocks), dim3(256), 0, s, rp);
+    else hipLaunchKernelGGL((k_refactor<float, false>), dim3(a.blocks), dim3(256), 0, s, rp);
     return hipGetLastError();
 }
 
 hipError_t launch_fitter(const StageArgs &a, const StageBuffers &b, const FitterConfig &cfg,
                          int32_t qr_mode, hipStream_t s) {
-    (void)qr_mode;
     FitterParams fp;
     fp.tmp = b.tmp_data;
     fp.mins_maxs = (float *)b.mins_maxs;
     fp.weights = (float *)b.weights;
     fp.noise = (const double *)b.noise;
+    fp.flags = qr_mode == 2 ? (int32_t *)b.flags : nullptr;
+    fp.threshold = a.refactor_threshold;
     fp.blocks = a.blocks;
     fp.frame = a.frame;
     const bool h = a.tmp_f16 != 0;
-    switch (cfg.rows_per_lane * 10 + cfg.blocks_per_group) {
-        case 41: return launch_rpl<4, 1>(fp, h, s);
-        case 42: return launch_rpl<4, 2>(fp, h, s);
-        case 81: return launch_rpl<8, 1>(fp, h, s);
-        case 82: return launch_rpl<8, 2>(fp, h, s);
-        case 161: return launch_rpl<16, 1>(fp, h, s);
-        case 162: return launch_rpl<16, 2>(fp, h, s);
-        case 164: return launch_rpl<16, 4>(fp, h, s);
-        default: return hipErrorInvalidValue;
-    }
+    hipError_t e = qr_mode >= 1 ? dispatch_single<true>(fp, h, cfg, s) : dispatch_single<false>(fp, h, cfg, s);
+    if (e != hipSuccess || qr_mode != 2) return e;
+    return launch_refactor_all(a, b, (const int32_t *)b.flags, (float *)b.weights, s);
+}
+
+hipError_t launch_fitter_fp64_all(const StageArgs &a, const StageBuffers &b, float *out, hipStream_t s) {
+    return launch_refactor_all(a, b, nullptr, out, s);
 }
 
 hipError_t launch_batch_fitter(const BatchLaunch &L, int total, bool t16, const FitterConfig &cfg,
-                               int grid_cap, hipStream_t s) {
+                               int grid_cap, int32_t qr_mode, hipStream_t s) {
     if (total <= 0) return hipErrorInvalidValue;
-    switch (cfg.rows_per_lane * 10 + cfg.blocks_per_group) {
-        case 41: return launch_batch_rpl<4, 1>(L, total, t16, grid_cap, s);
-        case 42: return launch_batch_rpl<4, 2>(L, total, t16, grid_cap, s);
-        case 81: return launch_batch_rpl<8, 1>(L, total, t16, grid_cap, s);
-        case 82: return launch_batch_rpl<8, 2>(L, total, t16, grid_cap, s);
-        case 161: return launch_batch_rpl<16, 1>(L, total, t16, grid_cap, s);
-        case 162: return launch_batch_rpl<16, 2>(L, total, t16, grid_cap, s);
-        case 164: return launch_batch_rpl<16, 4>(L, total, t16, grid_cap, s);
-        default: return hipErrorInvalidValue;
-    }
+    hipError_t e = qr_mode >= 1 ? dispatch_batch<true>(L, total, t16, cfg, grid_cap, s)
+                                : dispatch_batch<false>(L, total, t16, cfg, grid_cap, s);
+    if (e != hipSuccess || qr_mode != 2) return e;
+    if (t16) hipLaunchKernelGGL((kb_refactor<__half, true>), dim3(total), dim3(256), 0, s, L);
+    else hipLaunchKernelGGL((kb_refactor<float, false>), dim3(total), dim3(256), 0, s, L);
+    return hipGetLastError();
 }
 
 }  // namespace bmfr

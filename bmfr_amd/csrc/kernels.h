@@ -20,6 +20,8 @@ struct StageArgs {
     int32_t tmp_f16;       // tmp_data stored as half (reference default)
     float matrix[16];      // previous frame's view-projection, row-vector
     float pixel_offset_x, pixel_offset_y;
+    int32_t qr_mode;       // bmfr_qr_mode
+    float refactor_threshold;
 };
 
 // Device pointers of one stream's buffer set for the current frame (the
@@ -36,6 +38,7 @@ struct StageBuffers {
     void *out_cur, *out_prev;
     void *result_cur, *result_prev;
     void *noise;          // double[9*1024]: the fitter's feature noise of one frame
+    void *flags;          // int32[blocks]: FP64 re-factorisation flags
 };
 
 // Fitter launch shape, chosen per stream (or by the autotuner).
@@ -50,7 +53,7 @@ struct FitterConfig {
 struct BatchStream {
     StageArgs args;  // constants; frame/matrix/offset come from BatchStep
     void *albedo, *noisy_in, *prev_pixel, *accept, *tmp_data, *weights, *mins_maxs;
-    void *filtered, *tone_mapped, *noise;
+    void *filtered, *tone_mapped, *noise, *flags;
     void *normals[2], *positions[2], *noisy_acc[2], *spp[2], *out[2], *result[2];
 };
 
@@ -78,7 +81,9 @@ struct BatchLaunch {
 hipError_t launch_batch_accumulate_noisy_data(const BatchLaunch &L, int total, bool g16, bool t16,
                                               hipStream_t s);
 hipError_t launch_batch_fitter(const BatchLaunch &L, int total_blocks, bool t16,
-                               const FitterConfig &cfg, int grid_cap, hipStream_t s);
+                               const FitterConfig &cfg, int grid_cap, int32_t qr_mode, hipStream_t s);
+// FP64 least-squares weights of every block (diagnostic: error against FP64).
+hipError_t launch_fitter_fp64_all(const StageArgs &a, const StageBuffers &b, float *out, hipStream_t s);
 hipError_t launch_batch_weighted_sum(const BatchLaunch &L, int total, bool g16, hipStream_t s);
 hipError_t launch_batch_accumulate_filtered_data(const BatchLaunch &L, int total, bool g16,
                                                  hipStream_t s);

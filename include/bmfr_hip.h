@@ -157,6 +157,12 @@ int32_t bmfr_accumulate_noisy_data(bmfr_stream stream, int32_t frame_number,
                                    const float *pixel_offset);
 int32_t bmfr_fitter(bmfr_stream stream, int32_t frame_number);
 int32_t bmfr_weighted_sum(bmfr_stream stream, int32_t frame_number);
+/* FP64 least-squares weights of every block of the last fitter call, from
+ * the same scaled and noised matrix (W = float[30 nB]); for stating the
+ * fitter's weight error against FP64.  No counterpart in the reference. */
+int32_t bmfr_fitter_fp64_weights(bmfr_stream stream, float *weights_out);
+/* Blocks the last BMFR_QR_KAHAN_FP64 fitter call re-factorised in FP64. */
+int32_t bmfr_refactor_count(bmfr_stream stream, int32_t *count);
 int32_t bmfr_accumulate_filtered_data(bmfr_stream stream, int32_t frame_number);
 int32_t bmfr_taa(bmfr_stream stream, int32_t frame_number);
 /* Swaps the double buffers (opencl/bmfr.cpp:482-484). */

@@ -47,33 +47,40 @@ STAGES = ["accumulate_noisy_data", "fitter", "weighted_sum", "accumulate_filtere
 
 
 def run_plain(args):
+    """Timed frames record events only at the first stage's start and the
+    last stage's end (nothing between kernels); a second pass with events
+    around every stage gives the per-stage breakdown."""
     import bmfr_amd
     p = bmfr_amd.default_params(W, H)
     st = bmfr_amd.Stream(p)
     if args.fitter_config:
         st.set_fitter_config(*[int(x) for x in args.fitter_config.split(",")])
     sc = bmfr_amd.Scene(SEED, SPHERES, 1e-3, W, H)
-    st.set_timing(True)
-    n_frames = args.warmup + args.steps
+    n_frames = args.warmup + 2 * args.steps
     cams = [sc.frame_args(f) for f in range(n_frames)]
-    per_frame = []
-    wall0 = None
+    totals, per_stage = [], []
+    wall0, wall = None, None
     for f in range(n_frames):
         if f == args.warmup:
             st.synchronize()
+            st.set_timing(2)
             wall0 = time.perf_counter()
+        if f == args.warmup + args.steps:
+            st.synchronize()
+            wall = time.perf_counter() - wall0
+            st.set_timing(1)
         sc.render_to_stream(f, st)
         vp, off = cams[f]
         st.run_frame(f, vp, off)
-        if f >= args.warmup:
-            per_frame.append(st.stage_times())
+        if args.warmup <= f < args.warmup + args.steps:
+            totals.append(float(st.stage_times()[5]))
+        elif f >= args.warmup + args.steps:
+            per_stage.append(st.stage_times())
     st.synchronize()
-    wall = time.perf_counter() - wall0
-    t = np.array(per_frame, np.float64)  # [frames, 6]
     s = st.sizes
     st.close()
     sc.close()
-    return t, wall, s
+    return np.array(totals), np.array(per_stage, np.float64), wall, s
 
 
 BATCH_MIX = [(640, 360)] * 4 + [(1280, 720)] * 6 + [(1920, 1080)] * 4 + [(2560, 1440)] * 2
@@ -132,6 +139,57 @@ def run_batched(steps, warmup):
         "stage_ms": {k: round(float(v), 4) for k, v in zip(STAGES, stage[:5])},
         "graph": True, "autotune": tune,
     }
+
+
+def run_precision(frames):
+    """1920x1080, 48 spheres, fireflies 5e-3: FP16 normals/positions/noisy
+    colour/histories, compensated FP32 QR and FP64 re-factorisation of blocks
+    with min |R_ii| < 1e-6 max |R_ii|; beside it the same scene with the
+    plain-path settings (FP32 storage, plain FP32 QR).  Reports ms/frame,
+    the weight error of both against FP64 least squares on the same block
+    matrices, and the final frame's error of FP16 storage against FP32."""
+    import bmfr_amd
+    w, h = 1920, 1080
+    sc = bmfr_amd.Scene(0xB3F40000 + 2, 48, 5e-3, w, h)
+    a = bmfr_amd.Stream(bmfr_amd.default_params(w, h, storage=bmfr_amd.STORAGE_F16,
+                                                qr_mode=bmfr_amd.QR_KAHAN_FP64))
+    b = bmfr_amd.Stream(bmfr_amd.default_params(w, h))
+    a.set_timing(True)
+    t = []
+    for f in range(frames):
+        sc.render_to_stream(f, a)
+        sc.render_to_stream(f, b)
+        vp, off = sc.frame_args(f)
+        a.run_frame(f, vp, off)
+        b.run_frame(f, vp, off)
+        if f >= 3:
+            t.append(float(a.stage_times()[5]))
+
+    def werr(st):
+        w32 = st.read_buffer("weights").reshape(-1, 10, 3).astype(np.float64)
+        w64 = st.fitter_fp64_weights().reshape(-1, 10, 3).astype(np.float64)
+        e = np.abs(w32 - w64).max(axis=(1, 2)) / (np.abs(w64).max(axis=(1, 2)) + 1e-6)
+        e = e[np.isfinite(e)]
+        return {"max": float(e.max()), "median": float(np.median(e)), "p99": float(np.percentile(e, 99))}
+
+    ea, eb = werr(a), werr(b)
+    refactored = a.refactor_count()
+    ra, rb = a.download_result(), b.download_result()
+    fin = np.isfinite(ra) & np.isfinite(rb)
+    mse = float(np.mean((np.clip(ra[fin], 0, 1) - np.clip(rb[fin], 0, 1)) ** 2))
+    rel = np.abs(ra[fin] - rb[fin]) / np.maximum(np.abs(rb[fin]), 1e-2)
+    a.close()
+    b.close()
+    sc.close()
+    return {"workload": "1920x1080 (2135 blocks), 48 spheres, fireflies 5e-3, FP16 storage, "
+                        "QR_KAHAN_FP64", "frames": frames,
+            "ms_per_frame": round(float(np.mean(t)), 4),
+            "weight_err_vs_fp64": {k: round(v, 8) for k, v in ea.items()},
+            "weight_err_vs_fp64_plain_fp32": {k: round(v, 8) for k, v in eb.items()},
+            "blocks_refactored_last_frame": refactored,
+            "final_frame_vs_fp32_storage": {"psnr_db": round(10 * np.log10(1.0 / max(mse, 1e-30)), 2),
+                                            "median_rel": float(np.median(rel)),
+                                            "p99_rel": float(np.percentile(rel, 99))}}
 
 
 SERVE_RES = [(640, 360), (1280, 720), (1920, 1080), (2560, 1440)]
@@ -343,19 +401,23 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=30.0)
     ap.add_argument("--serve-budget-gib", type=float, default=4.0)
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--precision-frames", type=int, default=60)
+    ap.add_argument("--no-precision", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               "(the path shards by whole render streams, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
-    t, wall, s = run_plain(args)
-    ms_frame = float(t[:, 5].mean())
+    totals, t, wall, s = run_plain(args)
+    ms_frame = float(totals.mean())
     stage_ms = {name: float(t[:, i].mean()) for i, name in enumerate(STAGES)}
+    stage_ms["weighted_sum"] = 0.0  # fused into accumulate_filtered_data
     P, Q, nB = s.pixels, s.padded_pixels, s.blocks
     sb = stage_bytes(P, Q, nB)
-    stage_gbs = {k: sb[k] / (stage_ms[k] * 1e-3) / 1e9 for k in STAGES}
-    dom = max(STAGES, key=lambda k: stage_ms[k])
+    sb["accumulate_filtered_data"] += sb.pop("weighted_sum") - 2 * 12 * P  # no filtered round trip
+    stage_gbs = {k: sb[k] / (stage_ms[k] * 1e-3) / 1e9 for k in sb}
+    dom = max(sb, key=lambda k: stage_ms[k])
     roof = {"kernel": dom, "bound": "hbm", "achieved": round(stage_gbs[dom], 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(stage_gbs[dom] / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(dom), "algorithmic_bytes": int(sb[dom])}
@@ -363,6 +425,9 @@ def main():
     if not args.no_batched:
         batched = run_batched(args.batched_steps, 5)
         batched["speedup_vs_plain_720p"] = round(ms_frame / batched["ms_per_720p_frame_equiv"], 2)
+    precision = None
+    if not args.no_precision:
+        precision = run_precision(args.precision_frames)
     served = None
     if not args.no_served:
         served = run_served(args.serve_seconds, args.serve_budget_gib)
@@ -386,14 +451,16 @@ def main():
         "data": "synthetic: seeded procedural plane + 24 spheres, moving camera, rendered on the GPU",
         "config": {"workload": "plain BMFR single stream 1280x720 (984 blocks/frame), FP32 "
                                "G-buffers, FP16 tmp_data", "width": W, "height": H,
-                   "blocks": nB, "frames": args.warmup + args.steps},
+                   "blocks": nB, "timed_frames": args.steps},
         "mpixel_per_s": round(P / (ms_frame * 1e-3) / 1e6, 1),
         "frames_per_s_wall": round(args.steps / wall, 1),
+        "ms_per_frame_with_stage_events": round(float(t[:, 5].mean()), 4),
         "stage_ms": {k: round(v, 4) for k, v in stage_ms.items()},
         "stage_gbs": {k: round(v, 1) for k, v in stage_gbs.items()},
         "roofline": roof,
         "batched": batched,
         "served": served,
+        "precision": precision,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

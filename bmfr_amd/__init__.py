@@ -178,6 +178,9 @@ class Stream:
         o = _f32(pixel_offset, 2)
         _check(self._lib.bmfr_run_frame(self._h, frame, _fp(m), _fp(o)))
 
+    def set_fused(self, enable=True):
+        _check(self._lib.bmfr_set_fused(self._h, 1 if enable else 0))
+
     def set_fitter_config(self, rows_per_lane, blocks_per_group):
         _check(self._lib.bmfr_set_fitter_config(self._h, rows_per_lane, blocks_per_group))
 
@@ -229,7 +232,9 @@ class Stream:
 
     # -- timing -------------------------------------------------------------------
     def set_timing(self, enable=True):
-        _check(self._lib.bmfr_set_timing(self._h, 1 if enable else 0))
+        """True/1: per-stage events; 2: first-start/last-end events only."""
+        level = int(enable) if not isinstance(enable, bool) else (1 if enable else 0)
+        _check(self._lib.bmfr_set_timing(self._h, level))
 
     def stage_times(self):
         """ms of the last frame's five stages plus the first-start to

@@ -70,6 +70,7 @@ SIGNATURES = [
     ("bmfr_accumulate_filtered_data", c_int32, [ctypes.c_void_p, c_int32]),
     ("bmfr_taa", c_int32, [ctypes.c_void_p, c_int32]),
     ("bmfr_end_frame", c_int32, [ctypes.c_void_p]),
+    ("bmfr_set_fused", c_int32, [ctypes.c_void_p, c_int32]),
     ("bmfr_run_frame", c_int32, [ctypes.c_void_p, c_int32, c_float_p, c_float_p]),
     ("bmfr_download_result", c_int32, [ctypes.c_void_p, c_float_p]),
     ("bmfr_synchronize", c_int32, [ctypes.c_void_p]),

@@ -144,12 +144,13 @@ struct bmfr_stream_s {
     bool swapped = false;   // Double_buffer::swapped
     int noise_frame = -1;   // frame whose noise table S_NOISE holds
     int last_result = -1;   // slot holding the latest TAA output
-    bool timing = false;
+    int timing = 0;         // 0 off, 1 per stage, 2 first start / last end only
     hipEvent_t ev_start[kStages]{}, ev_end[kStages]{};
     hipEvent_t markers[kMarkers]{};
     hipEvent_t sync_ev = nullptr;   // orders this stream against batch launches
     bool recorded[kStages]{};
     bmfr::FitterConfig fitter{16, 1};
+    bool fuse = true;       // bmfr_run_frame runs stages 3+4 as one kernel
 
     int cur(int a_slot) const { return swapped ? a_slot : a_slot + 1; }
     int prev(int a_slot) const { return swapped ? a_slot + 1 : a_slot; }
@@ -252,13 +253,15 @@ int32_t check_frame(int32_t frame) {
 template <typename F>
 int32_t timed_stage(bmfr_stream s, int stage, F &&launch) {
     HIP_TRY(hipSetDevice(s->device));
-    if (s->timing) HIP_TRY(hipEventRecord(s->ev_start[stage], s->stream));
+    // level 1: events around every stage; level 2: only before the first
+    // and after the last stage, so that no event sits between kernels
+    const bool rec_start = s->timing == 1 || (s->timing == 2 && stage == 0);
+    const bool rec_end = s->timing == 1 || (s->timing == 2 && stage == kStages - 1);
+    if (rec_start) HIP_TRY(hipEventRecord(s->ev_start[stage], s->stream));
     hipError_t e = launch();
     if (e != hipSuccess) return fail(BMFR_ERR_HIP, "stage %d launch failed: %s", stage, hipGetErrorString(e));
-    if (s->timing) {
-        HIP_TRY(hipEventRecord(s->ev_end[stage], s->stream));
-        s->recorded[stage] = true;
-    }
+    if (rec_end) HIP_TRY(hipEventRecord(s->ev_end[stage], s->stream));
+    if (s->timing) s->recorded[stage] = true;
     return BMFR_OK;
 }
 
@@ -550,11 +553,25 @@ int32_t bmfr_end_frame(bmfr_stream s) {
 int32_t bmfr_run_frame(bmfr_stream s, int32_t frame, const float *matrix, const float *pixel_offset) {
     int32_t st = bmfr_accumulate_noisy_data(s, frame, matrix, pixel_offset);
     if (!st) st = bmfr_fitter(s, frame);
-    if (!st) st = bmfr_weighted_sum(s, frame);
-    if (!st) st = bmfr_accumulate_filtered_data(s, frame);
+    if (!st && !s->fuse) st = bmfr_weighted_sum(s, frame);
+    if (!st && !s->fuse) st = bmfr_accumulate_filtered_data(s, frame);
+    if (!st && s->fuse) {
+        // weighted_sum fused into accumulate_filtered_data: stage 2's events
+        // bracket nothing (0 ms), stage 3's bracket the fused kernel
+        const bmfr::StageArgs a = s->args(frame);
+        const bmfr::StageBuffers b = s->buffers();
+        st = timed_stage(s, 2, [&] { return hipSuccess; });
+        if (!st) st = timed_stage(s, 3, [&] { return bmfr::launch_weighted_sum_filtered(a, b, s->stream); });
+    }
     if (!st) st = bmfr_taa(s, frame);
     if (!st) st = bmfr_end_frame(s);
     return st;
+}
+
+int32_t bmfr_set_fused(bmfr_stream s, int32_t enable) {
+    if (int32_t st = check_stream(s)) return st;
+    s->fuse = enable != 0;
+    return BMFR_OK;
 }
 
 int32_t bmfr_synchronize(bmfr_stream s) {
@@ -628,7 +645,8 @@ int32_t bmfr_buffer_device_ptr(bmfr_stream s, int32_t buffer, void **dptr) {
 
 int32_t bmfr_set_timing(bmfr_stream s, int32_t enable) {
     if (int32_t st = check_stream(s)) return st;
-    s->timing = enable != 0;
+    if (enable < 0 || enable > 2) return fail(BMFR_ERR_INVALID_ARGUMENT, "timing level %d", enable);
+    s->timing = enable;
     for (int i = 0; i < kStages; ++i) s->recorded[i] = false;
     return BMFR_OK;
 }
@@ -640,7 +658,10 @@ int32_t bmfr_stage_times(bmfr_stream s, float *ms6) {
         if (!s->recorded[i]) return fail(BMFR_ERR_STATE, "stage %d has no timed launch", i);
     HIP_TRY(hipSetDevice(s->device));
     HIP_TRY(hipEventSynchronize(s->ev_end[kStages - 1]));
-    for (int i = 0; i < kStages; ++i) HIP_TRY(hipEventElapsedTime(&ms6[i], s->ev_start[i], s->ev_end[i]));
+    for (int i = 0; i < kStages; ++i) {
+        if (s->timing == 2) ms6[i] = -1.f;
+        else HIP_TRY(hipEventElapsedTime(&ms6[i], s->ev_start[i], s->ev_end[i]));
+    }
     HIP_TRY(hipEventElapsedTime(&ms6[5], s->ev_start[0], s->ev_end[kStages - 1]));
     return BMFR_OK;
 }
@@ -726,12 +747,11 @@ int32_t batch_enqueue_stages(bmfr_batch b, bool timed) {
                     e = bmfr::launch_batch_fitter(L, g.starts[bmfr::kStartsBlocks][L.n], g.t16, b->fitter,
                                                   b->grid_cap, g.qr, b->stream);
                     break;
-                case 2:
-                    e = bmfr::launch_batch_weighted_sum(L, g.starts[bmfr::kStartsImage][L.n], g.g16, b->stream);
+                case 2:  // weighted_sum fused into stage 4 (0 ms)
                     break;
                 case 3:
-                    e = bmfr::launch_batch_accumulate_filtered_data(L, g.starts[bmfr::kStartsImage][L.n],
-                                                                    g.g16, b->stream);
+                    e = bmfr::launch_batch_weighted_sum_filtered(L, g.starts[bmfr::kStartsImage][L.n],
+                                                                 g.g16, b->stream);
                     break;
                 default:
                     e = bmfr::launch_batch_taa(L, g.starts[bmfr::kStartsImage][L.n], g.g16, b->stream);
@@ -1269,8 +1289,8 @@ int32_t bmfr_server_step(bmfr_server sv, int32_t count, const bmfr_stream *strea
             switch (st) {
                 case 0: e = bmfr::launch_batch_accumulate_noisy_data(L, sts[0][L.n], g16, t16, sv->stream); break;
                 case 1: e = bmfr::launch_batch_fitter(L, sts[2][L.n], t16, sv->fitter, sv->grid_cap, qr, sv->stream); break;
-                case 2: e = bmfr::launch_batch_weighted_sum(L, sts[1][L.n], g16, sv->stream); break;
-                case 3: e = bmfr::launch_batch_accumulate_filtered_data(L, sts[1][L.n], g16, sv->stream); break;
+                case 2: e = hipSuccess; break;  // fused into stage 4
+                case 3: e = bmfr::launch_batch_weighted_sum_filtered(L, sts[1][L.n], g16, sv->stream); break;
                 default: e = bmfr::launch_batch_taa(L, sts[1][L.n], g16, sv->stream); break;
             }
             if (e != hipSuccess) return fail(BMFR_ERR_HIP, "server stage %d launch failed: %s", st, hipGetErrorString(e));

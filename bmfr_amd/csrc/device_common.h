@@ -119,6 +119,19 @@ __device__ __forceinline__ float scale_feature(float v, float mn, float mx) {
     return v - mn;
 }
 
+// Work-groups are dispatched round-robin over the 8 XCDs (work-group b runs
+// on XCD b % 8), and each XCD has its own L2.  Give each XCD a contiguous
+// range of the n tiles so that the halo a tile shares with its neighbours
+// (bilinear taps, 3x3 neighbourhoods, mirrored margins) is read through the
+// same L2.  Bijection on [0, n): XCD x owns q + (x < r) tiles, q = n / 8,
+// r = n % 8, starting at x*q + min(x, r).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+    const int q = n / kXcds, r = n % kXcds;
+    const int x = b % kXcds, j = b / kXcds;
+    return x * q + (x < r ? x : r) + j;
+}
+
 __device__ __forceinline__ float round_half(float v) {
     return __half2float(__float2half_rn(v));
 }

@@ -106,5 +106,8 @@ hipError_t launch_fitter(const StageArgs &a, const StageBuffers &b, const Fitter
 hipError_t launch_weighted_sum(const StageArgs &a, const StageBuffers &b, hipStream_t s);
 hipError_t launch_accumulate_filtered_data(const StageArgs &a, const StageBuffers &b, hipStream_t s);
 hipError_t launch_taa(const StageArgs &a, const StageBuffers &b, hipStream_t s);
+// Stages 3 and 4 in one kernel (the filtered colour is not written).
+hipError_t launch_weighted_sum_filtered(const StageArgs &a, const StageBuffers &b, hipStream_t s);
+hipError_t launch_batch_weighted_sum_filtered(const BatchLaunch &L, int total, bool g16, hipStream_t s);
 
 }  // namespace bmfr

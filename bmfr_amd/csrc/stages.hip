@@ -165,23 +165,33 @@ __device__ __forceinline__ void stage1_body(const StageArgs &a, const StageBuffe
 template <typename GT, typename TT>
 __global__ __launch_bounds__(256) void k_accumulate_noisy_data(const StageArgs a, const StageBuffers b,
                                                                int write_noise) {
-    const int gx = blockIdx.x * 32 + (threadIdx.x & 31);
-    const int gy = blockIdx.y * 8 + (threadIdx.x >> 5);
-    const int lin = (blockIdx.y * gridDim.x + blockIdx.x) * 256 + threadIdx.x;
-    stage1_body<GT, TT>(a, b, gx, gy, lin, write_noise != 0);
+    const int tiles_x = a.margin_width / 32;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
+    stage1_body<GT, TT>(a, b, tx * 32 + (threadIdx.x & 31), ty * 8 + (threadIdx.x >> 5),
+                        tile * 256 + threadIdx.x, write_noise != 0);
+}
+
+// 64x4 image tile of a single-stream launch, XCD-contiguous.
+__device__ __forceinline__ void image_tile(const StageArgs &a, int &x, int &y) {
+    const int tiles_x = (a.width + 63) / 64;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
+    x = tx * 64 + (threadIdx.x & 63);
+    y = ty * 4 + (threadIdx.x >> 6);
 }
 
 // ---------------------------------------------------------------------------
 // Stage 3: weighted_sum, opencl/bmfr.cl:703-758.
+// Filtered colour of in-image pixel (x, y): sum of block weights times the
+// pixel's (scaled) features, negatives clamped (bmfr.cl:712-750).
 template <typename GT>
-__device__ __forceinline__ void stage3_body(const StageArgs &a, const StageBuffers &bufs, int x, int y) {
+__device__ __forceinline__ float3 stage3_color(const StageArgs &a, const StageBuffers &bufs, int x, int y,
+                                              int64_t lp) {
     const float *__restrict__ weights = (const float *)bufs.weights;
     const float *__restrict__ mins_maxs = (const float *)bufs.mins_maxs;
-    float *__restrict__ output = (float *)bufs.filtered;
     const GT *__restrict__ cur_normals = (const GT *)bufs.normals_cur;
     const GT *__restrict__ cur_positions = (const GT *)bufs.positions_cur;
-    if (x >= a.width || y >= a.height) return;
-    const int64_t lp = (int64_t)y * a.width + x;
     const int2 off = block_offset(a.frame);
     const int ox = x + kBlock / 2 - off.x, oy = y + kBlock / 2 - off.y;
     const int64_t g = (int64_t)(ox / kBlock) + (int64_t)(oy / kBlock) * a.blocks_x;
@@ -200,21 +210,32 @@ __device__ __forceinline__ void stage3_body(const StageArgs &a, const StageBuffe
         cy = cy + w[f * 3 + 1] * v;
         cz = cz + w[f * 3 + 2] * v;
     }
-    output[lp * 3 + 0] = cx < 0.f ? 0.f : cx;
-    output[lp * 3 + 1] = cy < 0.f ? 0.f : cy;
-    output[lp * 3 + 2] = cz < 0.f ? 0.f : cz;
+    return make_float3(cx < 0.f ? 0.f : cx, cy < 0.f ? 0.f : cy, cz < 0.f ? 0.f : cz);
+}
+
+template <typename GT>
+__device__ __forceinline__ void stage3_body(const StageArgs &a, const StageBuffers &bufs, int x, int y) {
+    if (x >= a.width || y >= a.height) return;
+    const int64_t lp = (int64_t)y * a.width + x;
+    const float3 c = stage3_color<GT>(a, bufs, x, y, lp);
+    float *__restrict__ output = (float *)bufs.filtered;
+    output[lp * 3 + 0] = c.x;
+    output[lp * 3 + 1] = c.y;
+    output[lp * 3 + 2] = c.z;
 }
 
 template <typename GT>
 __global__ __launch_bounds__(256) void k_weighted_sum(const StageArgs a, const StageBuffers b) {
-    stage3_body<GT>(a, b, blockIdx.x * 64 + (threadIdx.x & 63), blockIdx.y * 4 + (threadIdx.x >> 6));
+    int x, y;
+    image_tile(a, x, y);
+    stage3_body<GT>(a, b, x, y);
 }
 
 // ---------------------------------------------------------------------------
 // Stage 4: accumulate_filtered_data, opencl/bmfr.cl:761-857.
+// Stage 4 of in-image pixel lp given its filtered colour fc.
 template <typename GT>
-__device__ __forceinline__ void stage4_body(const StageArgs &a, const StageBuffers &bufs, int x, int y) {
-    const float *__restrict__ filtered = (const float *)bufs.filtered;
+__device__ __forceinline__ void stage4_from(const StageArgs &a, const StageBuffers &bufs, int64_t lp, float3 fc) {
     const float2 *__restrict__ prev_pixel = (const float2 *)bufs.prev_pixel;
     const uint8_t *__restrict__ accept_bools = (const uint8_t *)bufs.accept;
     const float *__restrict__ albedo = (const float *)bufs.albedo;
@@ -223,9 +244,6 @@ __device__ __forceinline__ void stage4_body(const StageArgs &a, const StageBuffe
     const GT *__restrict__ acc_prev = (const GT *)bufs.out_prev;
     GT *__restrict__ acc_out = (GT *)bufs.out_cur;
     const int W = a.width;
-    if (x >= W || y >= a.height) return;
-    const int64_t lp = (int64_t)y * W + x;
-    const float3 fc = make_float3(filtered[lp * 3 + 0], filtered[lp * 3 + 1], filtered[lp * 3 + 2]);
     float3 pc = make_float3(0.f, 0.f, 0.f);
     float blend_alpha = 1.f;
     if (a.frame > 0) {
@@ -273,8 +291,34 @@ __device__ __forceinline__ void stage4_body(const StageArgs &a, const StageBuffe
 }
 
 template <typename GT>
+__device__ __forceinline__ void stage4_body(const StageArgs &a, const StageBuffers &bufs, int x, int y) {
+    if (x >= a.width || y >= a.height) return;
+    const int64_t lp = (int64_t)y * a.width + x;
+    const float *__restrict__ filtered = (const float *)bufs.filtered;
+    stage4_from<GT>(a, bufs, lp, make_float3(filtered[lp * 3 + 0], filtered[lp * 3 + 1], filtered[lp * 3 + 2]));
+}
+
+// Stages 3 and 4 fused for the chained paths: the filtered colour stays in
+// registers (the `filtered` buffer is not written); same arithmetic.
+template <typename GT>
+__device__ __forceinline__ void stage34_body(const StageArgs &a, const StageBuffers &bufs, int x, int y) {
+    if (x >= a.width || y >= a.height) return;
+    const int64_t lp = (int64_t)y * a.width + x;
+    stage4_from<GT>(a, bufs, lp, stage3_color<GT>(a, bufs, x, y, lp));
+}
+
+template <typename GT>
+__global__ __launch_bounds__(256) void k_weighted_sum_filtered(const StageArgs a, const StageBuffers b) {
+    int x, y;
+    image_tile(a, x, y);
+    stage34_body<GT>(a, b, x, y);
+}
+
+template <typename GT>
 __global__ __launch_bounds__(256) void k_accumulate_filtered_data(const StageArgs a, const StageBuffers b) {
-    stage4_body<GT>(a, b, blockIdx.x * 64 + (threadIdx.x & 63), blockIdx.y * 4 + (threadIdx.x >> 6));
+    int x, y;
+    image_tile(a, x, y);
+    stage4_body<GT>(a, b, x, y);
 }
 
 // ---------------------------------------------------------------------------
@@ -289,16 +333,35 @@ __device__ __forceinline__ float3 ycocg_to_rgb(float3 c) {
                        (c.x * 0.25f + c.y * -0.25f) + c.z * -0.25f);
 }
 
+// The 64x4 work-group tile's 3x3 neighbourhoods are read from LDS: the tile
+// plus a one-pixel halo (66 x 6 pixels) is loaded once, each in-image pixel
+// exactly once, so no value differs from reading it directly.
+constexpr int kTaaTileW = 66, kTaaTileH = 6;
+
 template <typename GT>
 __device__ __forceinline__ void stage5_body(const StageArgs &a, const StageBuffers &bufs, int x, int y) {
+    __shared__ float tile[3][kTaaTileH][kTaaTileW];
     const float2 *__restrict__ prev_pixel = (const float2 *)bufs.prev_pixel;
     const float *__restrict__ new_frame = (const float *)bufs.tone_mapped;
     GT *__restrict__ result = (GT *)bufs.result_cur;
     const GT *__restrict__ prev_result = (const GT *)bufs.result_prev;
     const int W = a.width, H = a.height;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int x0 = x - tx, y0 = y - ty;
+    for (int e = threadIdx.x; e < kTaaTileW * kTaaTileH; e += 256) {
+        const int ry = e / kTaaTileW, rx = e - ry * kTaaTileW;
+        const int gx = x0 - 1 + rx, gy = y0 - 1 + ry;
+        if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
+            const int64_t ls = (int64_t)gy * W + gx;
+            tile[0][ry][rx] = new_frame[ls * 3 + 0];
+            tile[1][ry][rx] = new_frame[ls * 3 + 1];
+            tile[2][ry][rx] = new_frame[ls * 3 + 2];
+        }
+    }
+    __syncthreads();
     if (x >= W || y >= H) return;
     const int64_t lp = (int64_t)y * W + x;
-    const float3 mine = make_float3(new_frame[lp * 3 + 0], new_frame[lp * 3 + 1], new_frame[lp * 3 + 2]);
+    const float3 mine = make_float3(tile[0][ty + 1][tx + 1], tile[1][ty + 1][tx + 1], tile[2][ty + 1][tx + 1]);
     const float2 pf = prev_pixel[lp];
     const int ix = cvt_rtn(pf.x), iy = cvt_rtn(pf.y);
     if (a.frame == 0 || ix < -1 || iy < -1 || ix >= W || iy >= H) {
@@ -313,13 +376,8 @@ __device__ __forceinline__ void stage5_body(const StageArgs &a, const StageBuffe
         for (int dx = -1; dx < 2; ++dx) {
             const int sx = x + dx, sy = y + dy;
             if (sx >= 0 && sy >= 0 && sx < W && sy < H) {
-                float3 sc;
-                if (dx == 0 && dy == 0) {
-                    sc = mine;
-                } else {
-                    const int64_t ls = (int64_t)sy * W + sx;
-                    sc = make_float3(new_frame[ls * 3 + 0], new_frame[ls * 3 + 1], new_frame[ls * 3 + 2]);
-                }
+                const float3 sc = make_float3(tile[0][ty + 1 + dy][tx + 1 + dx], tile[1][ty + 1 + dy][tx + 1 + dx],
+                                              tile[2][ty + 1 + dy][tx + 1 + dx]);
                 const float3 yc = rgb_to_ycocg(sc);
                 if (dx == 0 || dy == 0) {
                     mnc = make_float3(fminf(mnc.x, yc.x), fminf(mnc.y, yc.y), fminf(mnc.z, yc.z));
@@ -382,7 +440,9 @@ __device__ __forceinline__ void stage5_body(const StageArgs &a, const StageBuffe
 
 template <typename GT>
 __global__ __launch_bounds__(256) void k_taa(const StageArgs a, const StageBuffers b) {
-    stage5_body<GT>(a, b, blockIdx.x * 64 + (threadIdx.x & 63), blockIdx.y * 4 + (threadIdx.x >> 6));
+    int x, y;
+    image_tile(a, x, y);
+    stage5_body<GT>(a, b, x, y);
 }
 
 // ---------------------------------------------------------------------------
@@ -433,11 +493,12 @@ __device__ __forceinline__ void resolve_batch(const BatchLaunch &L, int s, Stage
 template <typename GT, typename TT>
 __global__ __launch_bounds__(256) void kb_accumulate_noisy_data(const BatchLaunch L) {
     const int32_t *starts = L.starts + kStartsPadded * (kMaxBatch + 1);
-    const int s = find_stream(starts, L.n, blockIdx.x);
+    const int unit = xcd_tile(blockIdx.x, gridDim.x);
+    const int s = find_stream(starts, L.n, unit);
     StageArgs a;
     StageBuffers b;
     resolve_batch(L, s, a, b);
-    const int local = blockIdx.x - starts[s];
+    const int local = unit - starts[s];
     const int tiles_x = a.margin_width / 32;
     const int ty = local / tiles_x, tx = local - ty * tiles_x;
     stage1_body<GT, TT>(a, b, tx * 32 + (threadIdx.x & 31), ty * 8 + (threadIdx.x >> 5),
@@ -447,16 +508,18 @@ __global__ __launch_bounds__(256) void kb_accumulate_noisy_data(const BatchLaunc
 template <int STAGE, typename GT>
 __global__ __launch_bounds__(256) void kb_image_stage(const BatchLaunch L) {
     const int32_t *starts = L.starts + kStartsImage * (kMaxBatch + 1);
-    const int s = find_stream(starts, L.n, blockIdx.x);
+    const int unit = xcd_tile(blockIdx.x, gridDim.x);
+    const int s = find_stream(starts, L.n, unit);
     StageArgs a;
     StageBuffers b;
     resolve_batch(L, s, a, b);
-    const int local = blockIdx.x - starts[s];
+    const int local = unit - starts[s];
     const int tiles_x = (a.width + 63) / 64;
     const int ty = local / tiles_x, tx = local - ty * tiles_x;
     const int x = tx * 64 + (threadIdx.x & 63), y = ty * 4 + (threadIdx.x >> 6);
     if constexpr (STAGE == 3) stage3_body<GT>(a, b, x, y);
     else if constexpr (STAGE == 4) stage4_body<GT>(a, b, x, y);
+    else if constexpr (STAGE == 34) stage34_body<GT>(a, b, x, y);
     else stage5_body<GT>(a, b, x, y);
 }
 
@@ -464,7 +527,7 @@ __global__ __launch_bounds__(256) void kb_image_stage(const BatchLaunch L) {
 // Host-side launchers.
 template <typename GT, typename TT>
 static hipError_t launch_s1(const StageArgs &a, const StageBuffers &b, bool wn, hipStream_t s) {
-    dim3 grid(a.margin_width / 32, a.margin_height / 8);
+    dim3 grid((a.margin_width / 32) * (a.margin_height / 8));
     hipLaunchKernelGGL((k_accumulate_noisy_data<GT, TT>), grid, dim3(256), 0, s, a, b, wn ? 1 : 0);
     return hipGetLastError();
 }
@@ -491,7 +554,7 @@ hipError_t launch_noise_table(const StageArgs &a, const StageBuffers &b, hipStre
 }
 
 static inline dim3 image_grid(const StageArgs &a) {
-    return dim3((a.width + 63) / 64, (a.height + 3) / 4);
+    return dim3(((a.width + 63) / 64) * ((a.height + 3) / 4));
 }
 
 hipError_t launch_weighted_sum(const StageArgs &a, const StageBuffers &b, hipStream_t s) {
@@ -507,6 +570,14 @@ hipError_t launch_accumulate_filtered_data(const StageArgs &a, const StageBuffer
         hipLaunchKernelGGL((k_accumulate_filtered_data<__half>), image_grid(a), dim3(256), 0, s, a, b);
     else
         hipLaunchKernelGGL((k_accumulate_filtered_data<float>), image_grid(a), dim3(256), 0, s, a, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_weighted_sum_filtered(const StageArgs &a, const StageBuffers &b, hipStream_t s) {
+    if (a.storage_f16)
+        hipLaunchKernelGGL((k_weighted_sum_filtered<__half>), image_grid(a), dim3(256), 0, s, a, b);
+    else
+        hipLaunchKernelGGL((k_weighted_sum_filtered<float>), image_grid(a), dim3(256), 0, s, a, b);
     return hipGetLastError();
 }
 
@@ -545,6 +616,9 @@ hipError_t launch_batch_weighted_sum(const BatchLaunch &L, int total, bool g16, 
 hipError_t launch_batch_accumulate_filtered_data(const BatchLaunch &L, int total, bool g16,
                                                  hipStream_t s) {
     return launch_batch_image<4>(L, total, g16, s);
+}
+hipError_t launch_batch_weighted_sum_filtered(const BatchLaunch &L, int total, bool g16, hipStream_t s) {
+    return launch_batch_image<34>(L, total, g16, s);
 }
 hipError_t launch_batch_taa(const BatchLaunch &L, int total, bool g16, hipStream_t s) {
     return launch_batch_image<5>(L, total, g16, s);

@@ -167,7 +167,12 @@ int32_t bmfr_accumulate_filtered_data(bmfr_stream stream, int32_t frame_number);
 int32_t bmfr_taa(bmfr_stream stream, int32_t frame_number);
 /* Swaps the double buffers (opencl/bmfr.cpp:482-484). */
 int32_t bmfr_end_frame(bmfr_stream stream);
-/* All five stages in order, then bmfr_end_frame. */
+/* All five stages in order, then bmfr_end_frame.  By default stages 3 and 4
+ * run as one kernel here (same arithmetic; the intermediate filtered buffer
+ * is then not written, and bmfr_stage_times reports 0 ms for stage 3 with
+ * the fused time under stage 4); bmfr_set_fused(stream, 0) runs them
+ * separately.  Batches and server steps always fuse them. */
+int32_t bmfr_set_fused(bmfr_stream stream, int32_t enable);
 int32_t bmfr_run_frame(bmfr_stream stream, int32_t frame_number,
                        const float *prev_frame_camera_matrix, const float *pixel_offset);
 
@@ -186,11 +191,13 @@ int32_t bmfr_write_buffer(bmfr_stream stream, int32_t buffer, const void *host, 
 int32_t bmfr_buffer_device_ptr(bmfr_stream stream, int32_t buffer, void **dptr);
 
 /* ---- timing (opencl/bmfr.cpp:386-412, 488-517) -------------------------------- */
-/* When enabled, HIP events bracket every stage on the stream's HIP stream.
- * bmfr_stage_times waits for the last frame and returns 6 values in ms:
- * accumulate_noisy_data, fitter, weighted_sum, accumulate_filtered_data,
- * taa, and the total from the first stage's start to the last stage's end
- * (the reference's "Total time in all kernels"). */
+/* Level 1: HIP events bracket every stage on the stream's HIP stream.
+ * Level 2: events only before the first and after the last stage (nothing
+ * recorded between kernels; per-stage values read -1).  bmfr_stage_times
+ * waits for the last frame and returns 6 values in ms: accumulate_noisy_data,
+ * fitter, weighted_sum, accumulate_filtered_data, taa, and the total from the
+ * first stage's start to the last stage's end (the reference's "Total time in
+ * all kernels"). */
 int32_t bmfr_set_timing(bmfr_stream stream, int32_t enable);
 int32_t bmfr_stage_times(bmfr_stream stream, float *ms6);
 

@@ -215,6 +215,12 @@ def test_timing_and_download(gpu):
         _run_chain(ls, 2)
         t = ls.g.stage_times()
         assert (t > 0).all() and t[5] >= t[:5].max()
+        ls.g.set_timing(2)          # total only: no events between kernels
+        _run_chain(ls, 1)
+        t2 = ls.g.stage_times()
+        assert (t2[:5] == -1).all() and t2[5] > 0
+        with pytest.raises(bmfr_amd.BmfrError):
+            ls.g.set_timing(3)
     finally:
         ls.close()
 
@@ -250,3 +256,27 @@ def test_invalid_fitter_config_and_stream_errors(gpu):
             g.accumulate_noisy_data(-1, np.zeros(16), np.zeros(2))
     finally:
         g.close()
+
+
+@pytest.mark.parametrize("storage_f16", [False, True])
+def test_fused_chain_equals_separate_stages(gpu, storage_f16):
+    """bmfr_run_frame's fused weighted_sum+accumulate_filtered_data kernel
+    gives exactly the results of the two separate stage calls."""
+    w, h = 130, 70
+    sc = scene(w, h)
+    a = bmfr_amd.Stream(bmfr_amd.default_params(w, h, storage=1 if storage_f16 else 0))
+    b = bmfr_amd.Stream(bmfr_amd.default_params(w, h, storage=1 if storage_f16 else 0))
+    b.set_fused(False)
+    try:
+        for fr in range(5):
+            sc.render_to_stream(fr, a)
+            sc.render_to_stream(fr, b)
+            vp, off = sc.frame_args(fr)
+            a.run_frame(fr, vp, off)
+            b.run_frame(fr, vp, off)
+        for name in ("out_prev", "result_prev", "tone_mapped", "weights"):
+            assert np.array_equal(a.read_buffer(name), b.read_buffer(name)), name
+        assert np.array_equal(a.download_result(), b.download_result())
+    finally:
+        a.close()
+        b.close()

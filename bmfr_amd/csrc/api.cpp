@@ -750,7 +750,7 @@ int32_t batch_enqueue_stages(bmfr_batch b, bool timed) {
                 case 2:  // weighted_sum fused into stage 4 (0 ms)
                     break;
                 case 3:
-                    e = bmfr::launch_batch_weighted_sum_filtered(L, g.starts[bmfr::kStartsImage][L.n],
+                    e = bmfr::launch_batch_weighted_sum_filtered(L, g.starts[bmfr::kStartsPadded][L.n],
                                                                  g.g16, b->stream);
                     break;
                 default:
@@ -1290,8 +1290,8 @@ int32_t bmfr_server_step(bmfr_server sv, int32_t count, const bmfr_stream *strea
                 case 0: e = bmfr::launch_batch_accumulate_noisy_data(L, sts[0][L.n], g16, t16, sv->stream); break;
                 case 1: e = bmfr::launch_batch_fitter(L, sts[2][L.n], t16, sv->fitter, sv->grid_cap, qr, sv->stream); break;
                 case 2: e = hipSuccess; break;  // fused into stage 4
-                case 3: e = bmfr::launch_batch_weighted_sum_filtered(L, sts[1][L.n], g16, sv->stream); break;
-                default: e = bmfr::launch_batch_taa(L, sts[1][L.n], g16, sv->stream); break;
+                case 3: e = bmfr::launch_batch_weighted_sum_filtered(L, sts[bmfr::kStartsPadded][L.n], g16, sv->stream); break;
+                default: e = bmfr::launch_batch_taa(L, sts[bmfr::kStartsImage][L.n], g16, sv->stream); break;
             }
             if (e != hipSuccess) return fail(BMFR_ERR_HIP, "server stage %d launch failed: %s", st, hipGetErrorString(e));
         }

@@ -1,7 +1,8 @@
 // Stages 1, 3, 4 and 5 of the BMFR frame chain for gfx950: per-pixel,
 // HBM-streaming kernels.  Each follows the operation order of the
 // corresponding reference kernel so that, with contraction off, its results
-// are reproducible bit for bit (the tone map's powf aside).
+// are reproducible bit for bit (the tone map's power aside: within one ulp
+// of the correctly rounded value, see tone_curve.h).
 //
 // Launch shapes: stage 1 runs over the padded grid in 32x8 tiles so that a
 // wave writes whole 32-pixel rows of one block of tmp_data (block-major,
@@ -9,6 +10,7 @@
 // that a wave reads 64 consecutive AoS pixels (768 contiguous bytes).
 #include "device_common.h"
 #include "kernels.h"
+#include "tone_curve.h"
 
 namespace bmfr {
 
@@ -183,18 +185,30 @@ __device__ __forceinline__ void image_tile(const StageArgs &a, int &x, int &y) {
 
 // ---------------------------------------------------------------------------
 // Stage 3: weighted_sum, opencl/bmfr.cl:703-758.
-// Filtered colour of in-image pixel (x, y): sum of block weights times the
-// pixel's (scaled) features, negatives clamped (bmfr.cl:712-750).
+// Launch shape: the padded grid's 32x8 tiles, four per block, so that every
+// pixel of a work-group belongs to block g (uniform): its 30 weights and 12
+// bounds come through scalar loads instead of per-lane vector loads.  Tile
+// pixel (gx, gy) of the padded grid is image pixel
+// (gx - 16 + offset.x, gy - 16 + offset.y) (bmfr.cl:707-710 inverted);
+// margin positions outside the image do nothing.
+__device__ __forceinline__ void block_tile(const StageArgs &a, int local, int &x, int &y, int64_t &g) {
+    const int tiles_x = a.margin_width / 32;
+    const int ty = local / tiles_x, tx = local - ty * tiles_x;
+    const int2 off = block_offset(a.frame);
+    x = tx * 32 + (threadIdx.x & 31) - kBlock / 2 + off.x;
+    y = ty * 8 + (threadIdx.x >> 5) - kBlock / 2 + off.y;
+    g = (int64_t)(ty >> 2) * a.blocks_x + tx;
+}
+
+// Filtered colour of in-image pixel lp in block g: sum of block weights
+// times the pixel's (scaled) features, negatives clamped (bmfr.cl:712-750).
 template <typename GT>
-__device__ __forceinline__ float3 stage3_color(const StageArgs &a, const StageBuffers &bufs, int x, int y,
-                                              int64_t lp) {
+__device__ __forceinline__ float3 stage3_color(const StageArgs &a, const StageBuffers &bufs, int64_t lp,
+                                              int64_t g) {
     const float *__restrict__ weights = (const float *)bufs.weights;
     const float *__restrict__ mins_maxs = (const float *)bufs.mins_maxs;
     const GT *__restrict__ cur_normals = (const GT *)bufs.normals_cur;
     const GT *__restrict__ cur_positions = (const GT *)bufs.positions_cur;
-    const int2 off = block_offset(a.frame);
-    const int ox = x + kBlock / 2 - off.x, oy = y + kBlock / 2 - off.y;
-    const int64_t g = (int64_t)(ox / kBlock) + (int64_t)(oy / kBlock) * a.blocks_x;
     const float3 wp = load3(cur_positions, lp);
     const float3 n = load3(cur_normals, lp);
     const float feat[kFeatures] = {1.f,  n.x,  n.y,         n.z,         wp.x,
@@ -214,10 +228,13 @@ __device__ __forceinline__ float3 stage3_color(const StageArgs &a, const StageBu
 }
 
 template <typename GT>
-__device__ __forceinline__ void stage3_body(const StageArgs &a, const StageBuffers &bufs, int x, int y) {
-    if (x >= a.width || y >= a.height) return;
+__device__ __forceinline__ void stage3_body(const StageArgs &a, const StageBuffers &bufs, int local) {
+    int x, y;
+    int64_t g;
+    block_tile(a, local, x, y, g);
+    if (x < 0 || y < 0 || x >= a.width || y >= a.height) return;
     const int64_t lp = (int64_t)y * a.width + x;
-    const float3 c = stage3_color<GT>(a, bufs, x, y, lp);
+    const float3 c = stage3_color<GT>(a, bufs, lp, g);
     float *__restrict__ output = (float *)bufs.filtered;
     output[lp * 3 + 0] = c.x;
     output[lp * 3 + 1] = c.y;
@@ -226,9 +243,7 @@ __device__ __forceinline__ void stage3_body(const StageArgs &a, const StageBuffe
 
 template <typename GT>
 __global__ __launch_bounds__(256) void k_weighted_sum(const StageArgs a, const StageBuffers b) {
-    int x, y;
-    image_tile(a, x, y);
-    stage3_body<GT>(a, b, x, y);
+    stage3_body<GT>(a, b, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // ---------------------------------------------------------------------------
@@ -285,9 +300,10 @@ __device__ __forceinline__ void stage4_from(const StageArgs &a, const StageBuffe
     ac.z = blend_alpha * fc.z + omb * pc.z;
     store3(acc_out, lp, ac);
     const float ar = albedo[lp * 3 + 0], ag = albedo[lp * 3 + 1], ab = albedo[lp * 3 + 2];
-    tone_mapped[lp * 3 + 0] = fminf(fmaxf(powf(fmaxf(0.f, ar * ac.x), 0.454545f), 0.f), 1.f);
-    tone_mapped[lp * 3 + 1] = fminf(fmaxf(powf(fmaxf(0.f, ag * ac.y), 0.454545f), 0.f), 1.f);
-    tone_mapped[lp * 3 + 2] = fminf(fmaxf(powf(fmaxf(0.f, ab * ac.z), 0.454545f), 0.f), 1.f);
+    // clamp(pow(max(0, albedo * colour), 0.454545f), 0, 1), bmfr.cl:851-856
+    tone_mapped[lp * 3 + 0] = tone_curve(ar * ac.x);
+    tone_mapped[lp * 3 + 1] = tone_curve(ag * ac.y);
+    tone_mapped[lp * 3 + 2] = tone_curve(ab * ac.z);
 }
 
 template <typename GT>
@@ -298,20 +314,22 @@ __device__ __forceinline__ void stage4_body(const StageArgs &a, const StageBuffe
     stage4_from<GT>(a, bufs, lp, make_float3(filtered[lp * 3 + 0], filtered[lp * 3 + 1], filtered[lp * 3 + 2]));
 }
 
-// Stages 3 and 4 fused for the chained paths: the filtered colour stays in
-// registers (the `filtered` buffer is not written); same arithmetic.
+// Stages 3 and 4 fused for the chained paths, on stage 3's launch shape: the
+// filtered colour stays in registers (the `filtered` buffer is not
+// written); same arithmetic.
 template <typename GT>
-__device__ __forceinline__ void stage34_body(const StageArgs &a, const StageBuffers &bufs, int x, int y) {
-    if (x >= a.width || y >= a.height) return;
+__device__ __forceinline__ void stage34_body(const StageArgs &a, const StageBuffers &bufs, int local) {
+    int x, y;
+    int64_t g;
+    block_tile(a, local, x, y, g);
+    if (x < 0 || y < 0 || x >= a.width || y >= a.height) return;
     const int64_t lp = (int64_t)y * a.width + x;
-    stage4_from<GT>(a, bufs, lp, stage3_color<GT>(a, bufs, x, y, lp));
+    stage4_from<GT>(a, bufs, lp, stage3_color<GT>(a, bufs, lp, g));
 }
 
 template <typename GT>
 __global__ __launch_bounds__(256) void k_weighted_sum_filtered(const StageArgs a, const StageBuffers b) {
-    int x, y;
-    image_tile(a, x, y);
-    stage34_body<GT>(a, b, x, y);
+    stage34_body<GT>(a, b, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 template <typename GT>
@@ -334,13 +352,17 @@ __device__ __forceinline__ float3 ycocg_to_rgb(float3 c) {
 }
 
 // The 64x4 work-group tile's 3x3 neighbourhoods are read from LDS: the tile
-// plus a one-pixel halo (66 x 6 pixels) is loaded once, each in-image pixel
-// exactly once, so no value differs from reading it directly.
+// plus a one-pixel halo (66 x 6 texels) is loaded once and converted to
+// YCoCg once per texel (the same operations as per neighbour, so the same
+// bits).  Texels outside the image hold NaN: fminf/fmaxf return the other
+// operand for a NaN, which is exactly the reference's skipping of
+// out-of-image neighbours (bmfr.cl:898-918), and the centre texel is always
+// in the image.  Min and max are exact, so their order does not matter.
 constexpr int kTaaTileW = 66, kTaaTileH = 6;
 
 template <typename GT>
 __device__ __forceinline__ void stage5_body(const StageArgs &a, const StageBuffers &bufs, int x, int y) {
-    __shared__ float tile[3][kTaaTileH][kTaaTileW];
+    __shared__ float4 tile[kTaaTileH][kTaaTileW];
     const float2 *__restrict__ prev_pixel = (const float2 *)bufs.prev_pixel;
     const float *__restrict__ new_frame = (const float *)bufs.tone_mapped;
     GT *__restrict__ result = (GT *)bufs.result_cur;
@@ -348,46 +370,54 @@ __device__ __forceinline__ void stage5_body(const StageArgs &a, const StageBuffe
     const int W = a.width, H = a.height;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     const int x0 = x - tx, y0 = y - ty;
+    // the pixel's own loads are issued before the tile's, so that they are
+    // in flight during the tile load and the barrier
+    const bool inside = x < W && y < H;
+    const int64_t lp = (int64_t)y * W + x;
+    float2 pf = make_float2(0.f, 0.f);
+    float3 mine = make_float3(0.f, 0.f, 0.f);
+    if (inside) {
+        pf = prev_pixel[lp];
+        mine = make_float3(new_frame[lp * 3 + 0], new_frame[lp * 3 + 1], new_frame[lp * 3 + 2]);
+    }
     for (int e = threadIdx.x; e < kTaaTileW * kTaaTileH; e += 256) {
         const int ry = e / kTaaTileW, rx = e - ry * kTaaTileW;
         const int gx = x0 - 1 + rx, gy = y0 - 1 + ry;
+        float4 t = make_float4(__builtin_nanf(""), __builtin_nanf(""), __builtin_nanf(""), 0.f);
         if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
             const int64_t ls = (int64_t)gy * W + gx;
-            tile[0][ry][rx] = new_frame[ls * 3 + 0];
-            tile[1][ry][rx] = new_frame[ls * 3 + 1];
-            tile[2][ry][rx] = new_frame[ls * 3 + 2];
+            const float3 yc = rgb_to_ycocg(make_float3(new_frame[ls * 3 + 0], new_frame[ls * 3 + 1],
+                                                       new_frame[ls * 3 + 2]));
+            t = make_float4(yc.x, yc.y, yc.z, 0.f);
         }
+        tile[ry][rx] = t;
     }
     __syncthreads();
-    if (x >= W || y >= H) return;
-    const int64_t lp = (int64_t)y * W + x;
-    const float3 mine = make_float3(tile[0][ty + 1][tx + 1], tile[1][ty + 1][tx + 1], tile[2][ty + 1][tx + 1]);
-    const float2 pf = prev_pixel[lp];
+    if (!inside) return;
     const int ix = cvt_rtn(pf.x), iy = cvt_rtn(pf.y);
     if (a.frame == 0 || ix < -1 || iy < -1 || ix >= W || iy >= H) {
         store3(result, lp, mine);
         return;
     }
-    float3 mnb = make_float3(INFINITY, INFINITY, INFINITY), mnc = mnb;
-    float3 mxb = make_float3(-INFINITY, -INFINITY, -INFINITY), mxc = mxb;
+    float4 nb[3][3];
 #pragma unroll
-    for (int dy = -1; dy < 2; ++dy) {
+    for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
-        for (int dx = -1; dx < 2; ++dx) {
-            const int sx = x + dx, sy = y + dy;
-            if (sx >= 0 && sy >= 0 && sx < W && sy < H) {
-                const float3 sc = make_float3(tile[0][ty + 1 + dy][tx + 1 + dx], tile[1][ty + 1 + dy][tx + 1 + dx],
-                                              tile[2][ty + 1 + dy][tx + 1 + dx]);
-                const float3 yc = rgb_to_ycocg(sc);
-                if (dx == 0 || dy == 0) {
-                    mnc = make_float3(fminf(mnc.x, yc.x), fminf(mnc.y, yc.y), fminf(mnc.z, yc.z));
-                    mxc = make_float3(fmaxf(mxc.x, yc.x), fmaxf(mxc.y, yc.y), fmaxf(mxc.z, yc.z));
-                }
-                mnb = make_float3(fminf(mnb.x, yc.x), fminf(mnb.y, yc.y), fminf(mnb.z, yc.z));
-                mxb = make_float3(fmaxf(mxb.x, yc.x), fmaxf(mxb.y, yc.y), fmaxf(mxb.z, yc.z));
-            }
-        }
-    }
+        for (int dx = 0; dx < 3; ++dx) nb[dy][dx] = tile[ty + dy][tx + dx];
+    // cross: centre and its 4-neighbours; box: all 9
+    float3 mnc, mxc, mnb, mxb;
+    mnc.x = fminf(fminf(fminf(nb[0][1].x, nb[1][0].x), fminf(nb[1][1].x, nb[1][2].x)), nb[2][1].x);
+    mnc.y = fminf(fminf(fminf(nb[0][1].y, nb[1][0].y), fminf(nb[1][1].y, nb[1][2].y)), nb[2][1].y);
+    mnc.z = fminf(fminf(fminf(nb[0][1].z, nb[1][0].z), fminf(nb[1][1].z, nb[1][2].z)), nb[2][1].z);
+    mxc.x = fmaxf(fmaxf(fmaxf(nb[0][1].x, nb[1][0].x), fmaxf(nb[1][1].x, nb[1][2].x)), nb[2][1].x);
+    mxc.y = fmaxf(fmaxf(fmaxf(nb[0][1].y, nb[1][0].y), fmaxf(nb[1][1].y, nb[1][2].y)), nb[2][1].y);
+    mxc.z = fmaxf(fmaxf(fmaxf(nb[0][1].z, nb[1][0].z), fmaxf(nb[1][1].z, nb[1][2].z)), nb[2][1].z);
+    mnb.x = fminf(fminf(mnc.x, fminf(nb[0][0].x, nb[0][2].x)), fminf(nb[2][0].x, nb[2][2].x));
+    mnb.y = fminf(fminf(mnc.y, fminf(nb[0][0].y, nb[0][2].y)), fminf(nb[2][0].y, nb[2][2].y));
+    mnb.z = fminf(fminf(mnc.z, fminf(nb[0][0].z, nb[0][2].z)), fminf(nb[2][0].z, nb[2][2].z));
+    mxb.x = fmaxf(fmaxf(mxc.x, fmaxf(nb[0][0].x, nb[0][2].x)), fmaxf(nb[2][0].x, nb[2][2].x));
+    mxb.y = fmaxf(fmaxf(mxc.y, fmaxf(nb[0][0].y, nb[0][2].y)), fmaxf(nb[2][0].y, nb[2][2].y));
+    mxb.z = fmaxf(fmaxf(mxc.z, fmaxf(nb[0][0].z, nb[0][2].z)), fmaxf(nb[2][0].z, nb[2][2].z));
     float3 pc = make_float3(0.f, 0.f, 0.f);
     float tw = 0.f;
     const float fx = pf.x - (float)ix, fy = pf.y - (float)iy;
@@ -505,22 +535,28 @@ __global__ __launch_bounds__(256) void kb_accumulate_noisy_data(const BatchLaunc
                         local * 256 + threadIdx.x, true);
 }
 
+// Stages 3 and 3+4 run over the padded tiles (block-uniform weights), 4 and
+// 5 over the image tiles.
 template <int STAGE, typename GT>
 __global__ __launch_bounds__(256) void kb_image_stage(const BatchLaunch L) {
-    const int32_t *starts = L.starts + kStartsImage * (kMaxBatch + 1);
+    constexpr bool kBlockTiles = STAGE == 3 || STAGE == 34;
+    const int32_t *starts = L.starts + (kBlockTiles ? kStartsPadded : kStartsImage) * (kMaxBatch + 1);
     const int unit = xcd_tile(blockIdx.x, gridDim.x);
     const int s = find_stream(starts, L.n, unit);
     StageArgs a;
     StageBuffers b;
     resolve_batch(L, s, a, b);
     const int local = unit - starts[s];
-    const int tiles_x = (a.width + 63) / 64;
-    const int ty = local / tiles_x, tx = local - ty * tiles_x;
-    const int x = tx * 64 + (threadIdx.x & 63), y = ty * 4 + (threadIdx.x >> 6);
-    if constexpr (STAGE == 3) stage3_body<GT>(a, b, x, y);
-    else if constexpr (STAGE == 4) stage4_body<GT>(a, b, x, y);
-    else if constexpr (STAGE == 34) stage34_body<GT>(a, b, x, y);
-    else stage5_body<GT>(a, b, x, y);
+    if constexpr (kBlockTiles) {
+        if constexpr (STAGE == 3) stage3_body<GT>(a, b, local);
+        else stage34_body<GT>(a, b, local);
+    } else {
+        const int tiles_x = (a.width + 63) / 64;
+        const int ty = local / tiles_x, tx = local - ty * tiles_x;
+        const int x = tx * 64 + (threadIdx.x & 63), y = ty * 4 + (threadIdx.x >> 6);
+        if constexpr (STAGE == 4) stage4_body<GT>(a, b, x, y);
+        else stage5_body<GT>(a, b, x, y);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -556,12 +592,15 @@ hipError_t launch_noise_table(const StageArgs &a, const StageBuffers &b, hipStre
 static inline dim3 image_grid(const StageArgs &a) {
     return dim3(((a.width + 63) / 64) * ((a.height + 3) / 4));
 }
+static inline dim3 block_grid(const StageArgs &a) {
+    return dim3((a.margin_width / 32) * (a.margin_height / 8));
+}
 
 hipError_t launch_weighted_sum(const StageArgs &a, const StageBuffers &b, hipStream_t s) {
     if (a.storage_f16)
-        hipLaunchKernelGGL((k_weighted_sum<__half>), image_grid(a), dim3(256), 0, s, a, b);
+        hipLaunchKernelGGL((k_weighted_sum<__half>), block_grid(a), dim3(256), 0, s, a, b);
     else
-        hipLaunchKernelGGL((k_weighted_sum<float>), image_grid(a), dim3(256), 0, s, a, b);
+        hipLaunchKernelGGL((k_weighted_sum<float>), block_grid(a), dim3(256), 0, s, a, b);
     return hipGetLastError();
 }
 
@@ -575,9 +614,9 @@ hipError_t launch_accumulate_filtered_data(const StageArgs &a, const StageBuffer
 
 hipError_t launch_weighted_sum_filtered(const StageArgs &a, const StageBuffers &b, hipStream_t s) {
     if (a.storage_f16)
-        hipLaunchKernelGGL((k_weighted_sum_filtered<__half>), image_grid(a), dim3(256), 0, s, a, b);
+        hipLaunchKernelGGL((k_weighted_sum_filtered<__half>), block_grid(a), dim3(256), 0, s, a, b);
     else
-        hipLaunchKernelGGL((k_weighted_sum_filtered<float>), image_grid(a), dim3(256), 0, s, a, b);
+        hipLaunchKernelGGL((k_weighted_sum_filtered<float>), block_grid(a), dim3(256), 0, s, a, b);
     return hipGetLastError();
 }
 

@@ -18,6 +18,8 @@ namespace bmfr {
 // Stage 1: accumulate_noisy_data, opencl/bmfr.cl:287-485.
 // One work-item of the padded grid (gx, gy); `lin` is its linear index,
 // used by the first 9216 work-items to also write the frame's noise table.
+// Called by all 256 work-items of a 32x8 tile (gx = 32 tx + t % 32,
+// gy = 8 ty + t / 32, t = threadIdx.x): the body has a barrier.
 template <typename GT, typename TT>
 __device__ __forceinline__ void stage1_body(const StageArgs &a, const StageBuffers &bufs, int gx, int gy,
                                             int lin, bool write_noise) {
@@ -74,24 +76,44 @@ __device__ __forceinline__ void stage1_body(const StageArgs &a, const StageBuffe
         const float fx = ppf_x - (float)ipx, fy = ppf_y - (float)ipy;
         const float omx = 1.f - fx, omy = 1.f - fy;
         const float wts[4] = {omx * omy, fx * omy, omx * fy, fx * fy};
-        float total_weight = 0.f;
+        // All four taps' history is loaded up front (one memory round trip
+        // instead of three dependent ones: position, then normal, then
+        // colour and spp); a tap outside the image reads the pixel's own
+        // texel and is never used.  The tests and sums below keep the
+        // reference's order (bmfr.cl:374-419).
+        bool tap_in[4];
+        int64_t tap_ls[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {  // bmfr.cl:374-419
+        for (int i = 0; i < 4; ++i) {
             const int sx = wrap_add(ipx, i & 1);
             const int sy = wrap_add(ipy, i >> 1);
-            if (sx >= 0 && sy >= 0 && sx < W && sy < H) {
-                const int64_t ls = (int64_t)wrap_mad(sy, W, sx);
-                const float3 pp = load3(prev_positions, ls);
+            tap_in[i] = sx >= 0 && sy >= 0 && sx < W && sy < H;
+            tap_ls[i] = tap_in[i] ? (int64_t)wrap_mad(sy, W, sx) : lp;
+        }
+        float3 tpp[4], tpn[4], tpc[4];
+        float tsp[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            tpp[i] = load3(prev_positions, tap_ls[i]);
+            tpn[i] = load3(prev_normals, tap_ls[i]);
+            tpc[i] = load3(prev_noisy, tap_ls[i]);
+            tsp[i] = (float)prev_spp[tap_ls[i]];
+        }
+        float total_weight = 0.f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (tap_in[i]) {
+                const float3 pp = tpp[i];
                 const float dx = pp.x - wp.x, dy = pp.y - wp.y, dz = pp.z - wp.z;
                 const float pd = (dx * dx + dy * dy) + dz * dz;
                 if (pd < a.position_limit_squared) {
-                    const float3 pn = load3(prev_normals, ls);
+                    const float3 pn = tpn[i];
                     const float nx = pn.x - nrm.x, ny = pn.y - nrm.y, nz = pn.z - nrm.z;
                     const float nd = (nx * nx + ny * ny) + nz * nz;
                     if (nd < a.normal_limit_squared) {
                         store_accept |= 1u << i;
-                        sample_spp = sample_spp + wts[i] * (float)prev_spp[ls];
-                        const float3 pc = load3(prev_noisy, ls);
+                        sample_spp = sample_spp + wts[i] * tsp[i];
+                        const float3 pc = tpc[i];
                         prev_col.x = prev_col.x + wts[i] * pc.x;
                         prev_col.y = prev_col.y + wts[i] * pc.y;
                         prev_col.z = prev_col.z + wts[i] * pc.z;
@@ -140,18 +162,33 @@ __device__ __forceinline__ void stage1_body(const StageArgs &a, const StageBuffe
                                       nc.x,
                                       nc.y,
                                       nc.z};
-    const int xin = gx & (kBlock - 1), yin = gy & (kBlock - 1);
-    const int64_t blk = (int64_t)(gy / kBlock) * a.blocks_x + gx / kBlock;
-    TT *dst = tmp_data + blk * (kBufferCount * kBlockPixels) + yin * kBlock + xin;
+    // The work-group's 32x8 tile is rows yin0..yin0+7 of one block, and
+    // those rows of a feature are 256 contiguous elements of tmp_data: the
+    // features go through LDS and out as 16-byte stores (1.6 per work-item
+    // instead of 13 narrow ones).
+    __shared__ __align__(16) TT feat_tile[kBufferCount][256];
 #pragma unroll
     for (int f = 0; f < kBufferCount; ++f) {
         float v = feat[f];
         if (v != v) v = 0.f;
         if constexpr (sizeof(TT) == 2) {
             v = fmaxf(fminf(v, 65504.f), -65504.f);
-            dst[f * kBlockPixels] = __float2half_rn(v);
+            feat_tile[f][threadIdx.x] = __float2half_rn(v);
         } else {
-            dst[f * kBlockPixels] = v;
+            feat_tile[f][threadIdx.x] = v;
+        }
+    }
+    __syncthreads();
+    {
+        const int yin0 = (gy - (int)(threadIdx.x >> 5)) & (kBlock - 1);
+        const int64_t blk = (int64_t)(gy / kBlock) * a.blocks_x + gx / kBlock;
+        TT *dst = tmp_data + blk * (kBufferCount * kBlockPixels) + yin0 * kBlock;
+        constexpr int kVec = 16 / (int)sizeof(TT);             // elements per store
+        constexpr int kPerFeature = 256 / kVec;
+        for (int c = threadIdx.x; c < kBufferCount * kPerFeature; c += 256) {
+            const int f = c / kPerFeature, e = (c - f * kPerFeature) * kVec;
+            *reinterpret_cast<uint4 *>(dst + f * kBlockPixels + e) =
+                *reinterpret_cast<const uint4 *>(&feat_tile[f][e]);
         }
     }
     // bmfr.cl:478-484 (and the spp of bmfr.cl:442, which mirrored
@@ -248,49 +285,77 @@ __global__ __launch_bounds__(256) void k_weighted_sum(const StageArgs a, const S
 
 // ---------------------------------------------------------------------------
 // Stage 4: accumulate_filtered_data, opencl/bmfr.cl:761-857.
-// Stage 4 of in-image pixel lp given its filtered colour fc.
+// Split in two so that the history taps are in flight while stage 3's sum is
+// computed: stage4_gather issues every load of pixel lp (a tap that the
+// accept bits or the range check rule out reads the pixel's own texel and
+// is never used), stage4_finish does the reference's arithmetic in its order.
 template <typename GT>
-__device__ __forceinline__ void stage4_from(const StageArgs &a, const StageBuffers &bufs, int64_t lp, float3 fc) {
+struct Stage4Gather {
+    float3 tap[4];
+    float3 albedo;
+    float wts[4];
+    uint32_t use;    // bit i: tap i contributes
+    uint32_t spp;
+};
+
+template <typename GT>
+__device__ __forceinline__ Stage4Gather<GT> stage4_gather(const StageArgs &a, const StageBuffers &bufs, int64_t lp) {
     const float2 *__restrict__ prev_pixel = (const float2 *)bufs.prev_pixel;
     const uint8_t *__restrict__ accept_bools = (const uint8_t *)bufs.accept;
     const float *__restrict__ albedo = (const float *)bufs.albedo;
-    float *__restrict__ tone_mapped = (float *)bufs.tone_mapped;
     const uint8_t *__restrict__ cur_spp = (const uint8_t *)bufs.spp_cur;
     const GT *__restrict__ acc_prev = (const GT *)bufs.out_prev;
-    GT *__restrict__ acc_out = (GT *)bufs.out_cur;
     const int W = a.width;
+    Stage4Gather<GT> g;
+    const uint32_t acc = a.frame > 0 ? (uint32_t)accept_bools[lp] : 0u;
+    const float2 pf = prev_pixel[lp];
+    g.spp = cur_spp[lp];
+    g.albedo = make_float3(albedo[lp * 3 + 0], albedo[lp * 3 + 1], albedo[lp * 3 + 2]);
+    const int ix = cvt_rtn(pf.x), iy = cvt_rtn(pf.y);
+    const float fx = pf.x - (float)ix, fy = pf.y - (float)iy;
+    const float omx = 1.f - fx, omy = 1.f - fy;
+    g.wts[0] = omx * omy;
+    g.wts[1] = fx * omy;
+    g.wts[2] = omx * fy;
+    g.wts[3] = fx * fy;
+    g.use = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int64_t ls = (int64_t)wrap_add(wrap_mad(wrap_add(iy, i >> 1), W, ix), i & 1);
+        // Stage 1 sets a bit only for an in-image tap at the same floor();
+        // the range check keeps inconsistent inputs (e.g. written through
+        // the debug API) from reading outside.
+        const bool use = (acc & (1u << i)) && ls >= 0 && ls < (int64_t)W * a.height;
+        g.use |= use ? 1u << i : 0u;
+        g.tap[i] = load3(acc_prev, use ? ls : lp);
+    }
+    return g;
+}
+
+template <typename GT>
+__device__ __forceinline__ void stage4_finish(const StageArgs &a, const StageBuffers &bufs, int64_t lp, float3 fc,
+                                              const Stage4Gather<GT> &g) {
+    float *__restrict__ tone_mapped = (float *)bufs.tone_mapped;
+    GT *__restrict__ acc_out = (GT *)bufs.out_cur;
     float3 pc = make_float3(0.f, 0.f, 0.f);
     float blend_alpha = 1.f;
-    if (a.frame > 0) {
-        const uint32_t acc = accept_bools[lp];
-        if (acc > 0) {
-            const float2 pf = prev_pixel[lp];
-            const int ix = cvt_rtn(pf.x), iy = cvt_rtn(pf.y);
-            const float fx = pf.x - (float)ix, fy = pf.y - (float)iy;
-            const float omx = 1.f - fx, omy = 1.f - fy;
-            const float wts[4] = {omx * omy, fx * omy, omx * fy, fx * fy};
-            float total_weight = 0.f;
+    if (g.use) {
+        float total_weight = 0.f;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int64_t ls = (int64_t)wrap_add(wrap_mad(wrap_add(iy, i >> 1), W, ix), i & 1);
-                // Stage 1 sets a bit only for an in-image tap at the same
-                // floor(); the range check keeps inconsistent inputs (e.g.
-                // written through the debug API) from reading outside.
-                if ((acc & (1u << i)) && ls >= 0 && ls < (int64_t)W * a.height) {
-                    total_weight = total_weight + wts[i];
-                    const float3 hv = load3(acc_prev, ls);
-                    pc.x = pc.x + wts[i] * hv.x;
-                    pc.y = pc.y + wts[i] * hv.y;
-                    pc.z = pc.z + wts[i] * hv.z;
-                }
+        for (int i = 0; i < 4; ++i) {
+            if (g.use & (1u << i)) {
+                total_weight = total_weight + g.wts[i];
+                pc.x = pc.x + g.wts[i] * g.tap[i].x;
+                pc.y = pc.y + g.wts[i] * g.tap[i].y;
+                pc.z = pc.z + g.wts[i] * g.tap[i].z;
             }
-            if (total_weight > 0.f) {
-                blend_alpha = 1.f / (float)cur_spp[lp];
-                blend_alpha = fmaxf(blend_alpha, a.second_blend_alpha);
-                pc.x = pc.x / total_weight;
-                pc.y = pc.y / total_weight;
-                pc.z = pc.z / total_weight;
-            }
+        }
+        if (total_weight > 0.f) {
+            blend_alpha = 1.f / (float)g.spp;
+            blend_alpha = fmaxf(blend_alpha, a.second_blend_alpha);
+            pc.x = pc.x / total_weight;
+            pc.y = pc.y / total_weight;
+            pc.z = pc.z / total_weight;
         }
     }
     const float omb = 1.f - blend_alpha;
@@ -299,11 +364,10 @@ __device__ __forceinline__ void stage4_from(const StageArgs &a, const StageBuffe
     ac.y = blend_alpha * fc.y + omb * pc.y;
     ac.z = blend_alpha * fc.z + omb * pc.z;
     store3(acc_out, lp, ac);
-    const float ar = albedo[lp * 3 + 0], ag = albedo[lp * 3 + 1], ab = albedo[lp * 3 + 2];
     // clamp(pow(max(0, albedo * colour), 0.454545f), 0, 1), bmfr.cl:851-856
-    tone_mapped[lp * 3 + 0] = tone_curve(ar * ac.x);
-    tone_mapped[lp * 3 + 1] = tone_curve(ag * ac.y);
-    tone_mapped[lp * 3 + 2] = tone_curve(ab * ac.z);
+    tone_mapped[lp * 3 + 0] = tone_curve(g.albedo.x * ac.x);
+    tone_mapped[lp * 3 + 1] = tone_curve(g.albedo.y * ac.y);
+    tone_mapped[lp * 3 + 2] = tone_curve(g.albedo.z * ac.z);
 }
 
 template <typename GT>
@@ -311,7 +375,9 @@ __device__ __forceinline__ void stage4_body(const StageArgs &a, const StageBuffe
     if (x >= a.width || y >= a.height) return;
     const int64_t lp = (int64_t)y * a.width + x;
     const float *__restrict__ filtered = (const float *)bufs.filtered;
-    stage4_from<GT>(a, bufs, lp, make_float3(filtered[lp * 3 + 0], filtered[lp * 3 + 1], filtered[lp * 3 + 2]));
+    const Stage4Gather<GT> gth = stage4_gather<GT>(a, bufs, lp);
+    stage4_finish<GT>(a, bufs, lp, make_float3(filtered[lp * 3 + 0], filtered[lp * 3 + 1], filtered[lp * 3 + 2]),
+                      gth);
 }
 
 // Stages 3 and 4 fused for the chained paths, on stage 3's launch shape: the
@@ -324,7 +390,8 @@ __device__ __forceinline__ void stage34_body(const StageArgs &a, const StageBuff
     block_tile(a, local, x, y, g);
     if (x < 0 || y < 0 || x >= a.width || y >= a.height) return;
     const int64_t lp = (int64_t)y * a.width + x;
-    stage4_from<GT>(a, bufs, lp, stage3_color<GT>(a, bufs, lp, g));
+    const Stage4Gather<GT> gth = stage4_gather<GT>(a, bufs, lp);
+    stage4_finish<GT>(a, bufs, lp, stage3_color<GT>(a, bufs, lp, g), gth);
 }
 
 template <typename GT>
@@ -373,13 +440,21 @@ __device__ __forceinline__ void stage5_body(const StageArgs &a, const StageBuffe
     // the pixel's own loads are issued before the tile's, so that they are
     // in flight during the tile load and the barrier
     const bool inside = x < W && y < H;
-    const int64_t lp = (int64_t)y * W + x;
-    float2 pf = make_float2(0.f, 0.f);
-    float3 mine = make_float3(0.f, 0.f, 0.f);
-    if (inside) {
-        pf = prev_pixel[lp];
-        mine = make_float3(new_frame[lp * 3 + 0], new_frame[lp * 3 + 1], new_frame[lp * 3 + 2]);
-    }
+    const int64_t lp = inside ? (int64_t)y * W + x : 0;
+    const float2 pf = prev_pixel[lp];
+    const float3 mine = make_float3(new_frame[lp * 3 + 0], new_frame[lp * 3 + 1], new_frame[lp * 3 + 2]);
+    // bilinear history taps, issued as soon as prev_pixel is in (before the
+    // tile load and the barrier); a tap that the tests of bmfr.cl:922-962
+    // rule out reads the pixel's own texel and is never used
+    const int ix = cvt_rtn(pf.x), iy = cvt_rtn(pf.y);
+    const bool history = a.frame != 0 && ix >= -1 && iy >= -1 && ix < W && iy < H;
+    const int64_t r0 = (int64_t)iy * W, r1 = ((int64_t)iy + 1) * W;  // 64-bit: iy may saturate
+    const bool use[4] = {history && iy >= 0 && ix >= 0, history && iy >= 0 && ix < W - 1,
+                         history && iy < H - 1 && ix >= 0, history && iy < H - 1 && ix < W - 1};
+    const int64_t tap_ls[4] = {r0 + ix, r0 + ix + 1, r1 + ix, r1 + ix + 1};
+    float3 tap[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) tap[i] = load3(prev_result, use[i] ? tap_ls[i] : lp);
     for (int e = threadIdx.x; e < kTaaTileW * kTaaTileH; e += 256) {
         const int ry = e / kTaaTileW, rx = e - ry * kTaaTileW;
         const int gx = x0 - 1 + rx, gy = y0 - 1 + ry;
@@ -394,8 +469,7 @@ __device__ __forceinline__ void stage5_body(const StageArgs &a, const StageBuffe
     }
     __syncthreads();
     if (!inside) return;
-    const int ix = cvt_rtn(pf.x), iy = cvt_rtn(pf.y);
-    if (a.frame == 0 || ix < -1 || iy < -1 || ix >= W || iy >= H) {
+    if (!history) {
         store3(result, lp, mine);
         return;
     }
@@ -422,33 +496,14 @@ __device__ __forceinline__ void stage5_body(const StageArgs &a, const StageBuffe
     float tw = 0.f;
     const float fx = pf.x - (float)ix, fy = pf.y - (float)iy;
     const float omx = 1.f - fx, omy = 1.f - fy;
-    const int64_t r0 = (int64_t)iy * W, r1 = (int64_t)(iy + 1) * W;
-    if (iy >= 0) {
-        if (ix >= 0) {
-            const float w = omx * omy;
-            const float3 v = load3(prev_result, r0 + ix);
-            pc.x = pc.x + w * v.x; pc.y = pc.y + w * v.y; pc.z = pc.z + w * v.z;
-            tw = tw + w;
-        }
-        if (ix < W - 1) {
-            const float w = fx * omy;
-            const float3 v = load3(prev_result, r0 + ix + 1);
-            pc.x = pc.x + w * v.x; pc.y = pc.y + w * v.y; pc.z = pc.z + w * v.z;
-            tw = tw + w;
-        }
-    }
-    if (iy < H - 1) {
-        if (ix >= 0) {
-            const float w = omx * fy;
-            const float3 v = load3(prev_result, r1 + ix);
-            pc.x = pc.x + w * v.x; pc.y = pc.y + w * v.y; pc.z = pc.z + w * v.z;
-            tw = tw + w;
-        }
-        if (ix < W - 1) {
-            const float w = fx * fy;
-            const float3 v = load3(prev_result, r1 + ix + 1);
-            pc.x = pc.x + w * v.x; pc.y = pc.y + w * v.y; pc.z = pc.z + w * v.z;
-            tw = tw + w;
+    const float wts[4] = {omx * omy, fx * omy, omx * fy, fx * fy};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (use[i]) {
+            pc.x = pc.x + wts[i] * tap[i].x;
+            pc.y = pc.y + wts[i] * tap[i].y;
+            pc.z = pc.z + wts[i] * tap[i].z;
+            tw = tw + wts[i];
         }
     }
     pc.x = pc.x / tw;  // unguarded like bmfr.cl:962

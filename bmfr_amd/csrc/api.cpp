@@ -149,7 +149,7 @@ struct bmfr_stream_s {
     hipEvent_t markers[kMarkers]{};
     hipEvent_t sync_ev = nullptr;   // orders this stream against batch launches
     bool recorded[kStages]{};
-    bmfr::FitterConfig fitter{16, 1};
+    bmfr::FitterConfig fitter{16, 4};  // one wave per block, four blocks per work-group
     bool fuse = true;       // bmfr_run_frame runs stages 3+4 as one kernel
 
     int cur(int a_slot) const { return swapped ? a_slot : a_slot + 1; }
@@ -718,7 +718,7 @@ struct bmfr_batch_s {
     hipEvent_t ev_start[kStages]{}, ev_end[kStages]{};
     bool recorded = false;
     bool recorded_total_only = false;
-    bmfr::FitterConfig fitter{16, 1};
+    bmfr::FitterConfig fitter{16, 4};
     int grid_cap = 0;
     bool use_graph = false;
     hipGraphExec_t graph = nullptr;
@@ -1010,6 +1010,7 @@ int32_t bmfr_batch_autotune(bmfr_batch b, int32_t reps, char *log, size_t log_le
         cands.push_back(c);
     for (int mult : {4, 8}) {  // persistent grids: a few waves per SIMD
         cands.push_back({16, 1, cus * 4 * mult / 4});
+        cands.push_back({16, 4, cus * mult / 4});
         cands.push_back({8, 2, cus * mult / 4});
     }
     std::string text = "fitter autotune (us per batched launch, min of reps):";
@@ -1107,8 +1108,8 @@ struct bmfr_server_s {
     int ring = 0;
     hipEvent_t step_start[kRing]{}, step_end[kRing]{};
     int64_t steps_launched = 0;
-    bmfr::FitterConfig fitter{16, 1};
-    int grid_cap = 1024;
+    bmfr::FitterConfig fitter{16, 4};
+    int grid_cap = 512;   // work-groups of 4 waves: 2 waves per SIMD on 256 CUs
 };
 
 extern "C" {

@@ -162,6 +162,57 @@ struct BlockFlag {
     float threshold;
 };
 
+// Back substitution, bmfr.cl:658-699, by one wave.  Lane L = 3*c + ch holds
+// column c (0..10) of R for colour channel ch: R[r][c] = elem(c, r) (the
+// factorised column c at row r, r < c, wave-uniform), R[c][c] = norms[c];
+// column 10 is Q^T b = rows 0..9 of colour column 10+ch.  Weights are
+// written by lane 30 + ch.
+template <typename Elem>
+__device__ __forceinline__ void back_substitution(Elem elem, const float (&norms)[kFeatures],
+                                                  float *__restrict__ weights, bool active, int lane) {
+    const int cl = lane / 3;
+    const int ch = lane - 3 * cl;
+    float rc[kFeatures];
+#pragma unroll
+    for (int r = 0; r < kFeatures; ++r) rc[r] = 0.f;
+#pragma unroll
+    for (int c = 0; c < kFeatures; ++c) {
+#pragma unroll
+        for (int r = 0; r < kFeatures; ++r) {
+            float val;
+            if (r < c) val = elem(c, r);
+            else if (r == c) val = norms[c];
+            else val = 0.f;
+            if (cl == c) rc[r] = val;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < kFeatures; ++r) {
+        const float v0 = elem(10, r);
+        const float v1 = elem(11, r);
+        const float v2 = elem(12, r);
+        if (cl == 10) rc[r] = ch == 0 ? v0 : (ch == 1 ? v1 : v2);
+    }
+#pragma unroll
+    for (int i = kREdge - 2; i >= 0; --i) {
+        const float divider = __shfl(rc[i], 3 * i + ch);
+        if (cl >= i) rc[i] = rc[i] / divider;
+        float acc = __shfl(rc[i], 30 + ch);
+#pragma unroll
+        for (int j = i + 1; j < kREdge - 1; ++j) acc = acc - __shfl(rc[i], 3 * j + ch);
+        if (cl == 10) rc[i] = acc;
+        const float wi = __shfl(rc[i], 30 + ch);
+        if (cl == i) {
+#pragma unroll
+            for (int r = 0; r <= i; ++r) rc[r] = rc[r] * wi;
+        }
+    }
+    if (active && cl == 10 && ch < 3) {
+#pragma unroll
+        for (int r = 0; r < kFeatures; ++r) weights[r * 3 + ch] = rc[r];
+    }
+}
+
 // One 32x32 block: RPL rows per lane, BPG blocks per work-group, TT tmp
 // element type, HALF_ROUND: round every stored element to binary16 like the
 // reference's USE_HALF_PRECISION_IN_TMP_DATA=1 (bmfr.cl:255-258, 540, 651).
@@ -410,54 +461,199 @@ __device__ __forceinline__ void fitter_block(const TT *__restrict__ src, float *
     }
 
     // ---- back substitution, bmfr.cl:658-699, in wave 0 ----
-    // Lane L = 3*c + ch holds column c (0..10) of R for colour channel ch:
-    // R[r][c] = a[c] at row r (r < c), R[c][c] = norms[c]; column 10 is
-    // Q^T b = rows 0..9 of colour column 10+ch.
-    if (wave == 0) {
-        const int cl = lane / 3;
-        const int ch = lane - 3 * cl;
-        float rc[kFeatures];
+    if (wave == 0)
+        back_substitution([&](int c, int r) { return readlane(a[c][r % V], r / V); }, norms, weights, active,
+                          lane);
+}
+
+// ---- One-wave fitter for binary16 tmp_data: the default configuration -------
+// The same operations as fitter_block with RPL = 16 (one wave per block, 16
+// rows per lane, plain FP32 sums), scheduled for one wave: every pivot step
+// forms all of its trailing dot products, each with two partial sums (even
+// and odd rows of a lane), and reduces them step-major (one DPP step for all
+// columns before the next), and the pivot loop is unrolled at compile time
+// so that each step reduces exactly its 12 - col columns.  Independent DPP
+// chains then fill each other's hazard slots instead of s_nop, and the
+// wave-uniform results stay in SGPRs.  Every stored element is rounded to
+// binary16 like the reference's stores (bmfr.cl:540, 651); the feature noise
+// is added in FP64 right after the load (bmfr.cl:623-627), so its loads are
+// in flight during the scaling phase.
+struct Rows16 {
+    float a[kBufferCount][16];
+    __device__ __forceinline__ float get(int f, int k) const { return a[f][k]; }
+    __device__ __forceinline__ void set(int f, int k, float v) { a[f][k] = round_half(v); }
+};
+
+// N independent 64-lane reductions, step-major; results are wave-uniform and
+// each equals wave_reduce<Op> of its input.
+template <typename Op, int N>
+__device__ __forceinline__ void wave_reduce_n(float (&v)[N]) {
 #pragma unroll
-        for (int r = 0; r < kFeatures; ++r) rc[r] = 0.f;
+    for (int i = 0; i < N; ++i) v[i] = Op::f(v[i], dpp_all<0xB1>(v[i]));
 #pragma unroll
-        for (int c = 0; c < kFeatures; ++c) {
+    for (int i = 0; i < N; ++i) v[i] = Op::f(v[i], dpp_all<0x4E>(v[i]));
 #pragma unroll
-            for (int r = 0; r < kFeatures; ++r) {
-                float val;
-                if (r < c) val = readlane(a[c][r % V], r / V);
-                else if (r == c) val = norms[c];
-                else val = 0.f;
-                if (cl == c) rc[r] = val;
-            }
+    for (int i = 0; i < N; ++i) v[i] = Op::f(v[i], dpp_all<0x141>(v[i]));
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = Op::f(v[i], dpp_all<0x140>(v[i]));
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = Op::f(v[i], (dpp<0x142, 0xA>(Op::identity, v[i])));
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = Op::f(v[i], (dpp<0x143, 0xC>(Op::identity, v[i])));
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = readlane(v[i], 63);
+}
+
+constexpr int kH16V = 8;   // rows per 16-byte load: element k is row (k/8)*512 + 8*lane + k%8
+__device__ __forceinline__ int h16_row(int k, int lane) { return (k / kH16V) * (64 * kH16V) + lane * kH16V + (k % kH16V); }
+
+// Pivot step COL: norm, u, the 12 - COL trailing dot products, update.
+template <int COL>
+__device__ __forceinline__ void h16_pivot(Rows16 &m, float (&norms)[kFeatures], int lane) {
+    constexpr int NV = kBufferCount - 1 - COL;
+    const float xc = readlane(m.get(COL, COL % kH16V), COL / kH16V);
+    float r1[1];
+    {
+        float s0 = 0.f, s1 = 0.f;
+#pragma unroll
+        for (int k = 0; k < 16; k += 2) {
+            const float x0 = h16_row(k, lane) > COL ? m.get(COL, k) : 0.f;
+            const float x1 = h16_row(k + 1, lane) > COL ? m.get(COL, k + 1) : 0.f;
+            s0 = __builtin_fmaf(x0, x0, s0);
+            s1 = __builtin_fmaf(x1, x1, s1);
         }
+        r1[0] = s0 + s1;
+    }
+    wave_reduce_n<OpAdd, 1>(r1);
+    const float sigma = r1[0];
+    const float norm = sqrtf(sigma + xc * xc);
+    const float uc = xc - norm;
+    const float uu = sigma + uc * uc;
+    norms[COL] = norm;
+    float u[16];
 #pragma unroll
-        for (int r = 0; r < kFeatures; ++r) {
-            const float v0 = readlane(a[10][r % V], r / V);
-            const float v1 = readlane(a[11][r % V], r / V);
-            const float v2 = readlane(a[12][r % V], r / V);
-            if (cl == 10) rc[r] = ch == 0 ? v0 : (ch == 1 ? v1 : v2);
+    for (int k = 0; k < 16; ++k) {
+        const int row = h16_row(k, lane);
+        u[k] = row > COL ? m.get(COL, k) : (row == COL ? uc : 0.f);
+    }
+    float d[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        const int j = COL + 1 + i;
+        float s0 = 0.f, s1 = 0.f;
+#pragma unroll
+        for (int k = 0; k < 16; k += 2) {
+            s0 = __builtin_fmaf(u[k], m.get(j, k), s0);
+            s1 = __builtin_fmaf(u[k + 1], m.get(j, k + 1), s1);
         }
+        d[i] = s0 + s1;
+    }
+    wave_reduce_n<OpAdd, NV>(d);
+    const float inv_uu = 1.f / uu;
 #pragma unroll
-        for (int i = kREdge - 2; i >= 0; --i) {
-            const float divider = __shfl(rc[i], 3 * i + ch);
-            if (cl >= i) rc[i] = rc[i] / divider;
-            float acc = __shfl(rc[i], 30 + ch);
+    for (int i = 0; i < NV; ++i) {
+        const int j = COL + 1 + i;
+        const float fct = (2.f * d[i]) * inv_uu;
 #pragma unroll
-            for (int j = i + 1; j < kREdge - 1; ++j) acc = acc - __shfl(rc[i], 3 * j + ch);
-            if (cl == 10) rc[i] = acc;
-            const float wi = __shfl(rc[i], 30 + ch);
-            if (cl == i) {
-#pragma unroll
-                for (int r = 0; r <= i; ++r) rc[r] = rc[r] * wi;
-            }
-        }
-        if (active && cl == 10 && ch < 3) {
-            float *w = weights;
-#pragma unroll
-            for (int r = 0; r < kFeatures; ++r) w[r * 3 + ch] = rc[r];
-        }
+        for (int k = 0; k < 16; ++k) m.set(j, k, __builtin_fmaf(-u[k], fct, m.get(j, k)));
     }
 }
+
+template <int COL>
+__device__ __forceinline__ void h16_pivots(Rows16 &m, float (&norms)[kFeatures], int lane) {
+    if constexpr (COL < kFeatures) {
+        h16_pivot<COL>(m, norms, lane);
+        h16_pivots<COL + 1>(m, norms, lane);
+    }
+}
+
+__device__ __forceinline__ void fitter_block_h16(const __half *__restrict__ src, float *__restrict__ mins_maxs,
+                                                 float *__restrict__ weights, const double *__restrict__ noise,
+                                                 bool active) {
+    const int lane = threadIdx.x & 63;
+    Rows16 m;
+#pragma unroll
+    for (int f = 0; f < kBufferCount; ++f)
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+            load_rows<__half, kH16V>(src + f * kBlockPixels + q * (64 * kH16V) + lane * kH16V, &m.a[f][q * kH16V]);
+
+    // ---- scaling, bmfr.cl:511-542 (max taken as -min(-x): exact) ----
+    float bmin[kScaled], bmax[kScaled];
+    {
+        float v[2 * kScaled];
+#pragma unroll
+        for (int i = 0; i < kScaled; ++i) {
+            float mn = INFINITY, nmx = INFINITY;
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                const float x = m.get(kNotScaled + i, k);
+                mn = fminf(mn, x);
+                nmx = fminf(nmx, -x);
+            }
+            v[i] = mn;
+            v[kScaled + i] = nmx;
+        }
+        wave_reduce_n<OpMin, 2 * kScaled>(v);
+#pragma unroll
+        for (int i = 0; i < kScaled; ++i) {
+            bmin[i] = v[i];
+            bmax[i] = -v[kScaled + i];
+        }
+    }
+    if (active && lane == 0) {
+#pragma unroll
+        for (int i = 0; i < kScaled; ++i) {
+            mins_maxs[2 * i + 0] = bmin[i];
+            mins_maxs[2 * i + 1] = bmax[i];
+        }
+    }
+    // (v-min)/(max-min) as Markstein's corrected quotient (see fitter_block)
+#pragma unroll
+    for (int i = 0; i < kScaled; ++i) {
+        const float d = bmax[i] - bmin[i];
+        const bool divide = fabsf(d) > 1.0f;
+        const float y = 1.0f / d;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const float num = m.get(kNotScaled + i, k) - bmin[i];
+            float sc = num;
+            if (divide) {
+                const float q = num * y;
+                const float r = __builtin_fmaf(-q, d, num);
+                sc = __builtin_fmaf(r, y, q);
+            }
+            m.set(kNotScaled + i, k, sc);
+        }
+    }
+    // ---- feature noise of the first pass, in FP64, rounded once to float ----
+#pragma unroll
+    for (int f = 1; f < kFeatures; ++f) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            const double *nz = noise + (f - 1) * kBlockPixels + q * (64 * kH16V) + lane * kH16V;
+#pragma unroll
+            for (int e = 0; e < kH16V; e += 2) {
+                const double2 x = *reinterpret_cast<const double2 *>(nz + e);
+                m.a[f][q * kH16V + e] = (float)((double)m.a[f][q * kH16V + e] + x.x);
+                m.a[f][q * kH16V + e + 1] = (float)((double)m.a[f][q * kH16V + e + 1] + x.y);
+            }
+        }
+    }
+
+    // ---- Householder QR over the 10 feature columns, bmfr.cl:549-656 ----
+    float norms[kFeatures];
+    h16_pivots<0>(m, norms, lane);
+
+    // ---- back substitution, bmfr.cl:658-699 ----
+    back_substitution([&](int c, int r) { return readlane(m.get(c, r % kH16V), r / kH16V); }, norms, weights,
+                      active, lane);
+}
+
+// The one-wave body serves binary16 tmp_data with RPL = 16 and the plain FP32
+// sums; every other combination runs fitter_block.
+template <int RPL, typename TT, bool HALF_ROUND, bool COMP>
+constexpr bool kPackedH16 = RPL == 16 && sizeof(TT) == 2 && HALF_ROUND && !COMP;
 
 template <int RPL, int BPG, typename TT, bool HALF_ROUND, bool COMP>
 __global__ __launch_bounds__(64 * (16 / RPL) * BPG) void k_fitter(const FitterParams p) {
@@ -466,10 +662,16 @@ __global__ __launch_bounds__(64 * (16 / RPL) * BPG) void k_fitter(const FitterPa
     const int blk_raw = blockIdx.x * BPG + sub;
     const bool active = blk_raw < p.blocks;
     const int blk = active ? blk_raw : p.blocks - 1;
-    fitter_block<RPL, BPG, TT, HALF_ROUND, COMP>(
-        reinterpret_cast<const TT *>(p.tmp) + (int64_t)blk * (kBufferCount * kBlockPixels),
-        p.mins_maxs + (int64_t)blk * (kScaled * 2), p.weights + (int64_t)blk * (kFeatures * 3),
-        p.noise, BlockFlag{p.flags ? p.flags + blk : nullptr, p.threshold}, active, red);
+    if constexpr (kPackedH16<RPL, TT, HALF_ROUND, COMP>) {
+        fitter_block_h16(reinterpret_cast<const __half *>(p.tmp) + (int64_t)blk * (kBufferCount * kBlockPixels),
+                         p.mins_maxs + (int64_t)blk * (kScaled * 2), p.weights + (int64_t)blk * (kFeatures * 3),
+                         p.noise, active);
+    } else {
+        fitter_block<RPL, BPG, TT, HALF_ROUND, COMP>(
+            reinterpret_cast<const TT *>(p.tmp) + (int64_t)blk * (kBufferCount * kBlockPixels),
+            p.mins_maxs + (int64_t)blk * (kScaled * 2), p.weights + (int64_t)blk * (kFeatures * 3),
+            p.noise, BlockFlag{p.flags ? p.flags + blk : nullptr, p.threshold}, active, red);
+    }
 }
 
 // Persistent batched fitter: the work-groups stride over the concatenated
@@ -490,11 +692,17 @@ __global__ __launch_bounds__(64 * (16 / RPL) * BPG) void kb_fitter(const BatchLa
         const int blk = u - starts[s];
         const BatchStream &bs = L.streams[s];
         int32_t *fl = (int32_t *)bs.flags;
-        fitter_block<RPL, BPG, TT, HALF_ROUND, COMP>(
-            reinterpret_cast<const TT *>(bs.tmp_data) + (int64_t)blk * (kBufferCount * kBlockPixels),
-            (float *)bs.mins_maxs + (int64_t)blk * (kScaled * 2),
-            (float *)bs.weights + (int64_t)blk * (kFeatures * 3), (const double *)bs.noise,
-            BlockFlag{fl ? fl + blk : nullptr, bs.args.refactor_threshold}, active, red);
+        if constexpr (kPackedH16<RPL, TT, HALF_ROUND, COMP>) {
+            fitter_block_h16(reinterpret_cast<const __half *>(bs.tmp_data) + (int64_t)blk * (kBufferCount * kBlockPixels),
+                             (float *)bs.mins_maxs + (int64_t)blk * (kScaled * 2),
+                             (float *)bs.weights + (int64_t)blk * (kFeatures * 3), (const double *)bs.noise, active);
+        } else {
+            fitter_block<RPL, BPG, TT, HALF_ROUND, COMP>(
+                reinterpret_cast<const TT *>(bs.tmp_data) + (int64_t)blk * (kBufferCount * kBlockPixels),
+                (float *)bs.mins_maxs + (int64_t)blk * (kScaled * 2),
+                (float *)bs.weights + (int64_t)blk * (kFeatures * 3), (const double *)bs.noise,
+                BlockFlag{fl ? fl + blk : nullptr, bs.args.refactor_threshold}, active, red);
+        }
         // A block ends with round 20 in LDS buffer 0 and the next one starts
         // with round 0 in the same buffer: wait until every wave has read it.
         if constexpr (16 / RPL > 1) __syncthreads();

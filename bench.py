@@ -392,6 +392,11 @@ def cpu_baseline(frames=8):
                       f"1280x720 scene; mean of frames 1..{frames - 1}"}
 
 
+def progress(msg):
+    """One line per finished phase on stderr (the JSON line stays alone on stdout)."""
+    print(f"bench.py: {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -414,6 +419,7 @@ def main():
 
     totals, t, wall, s = run_plain(args)
     ms_frame = float(totals.mean())
+    progress(f"plain 1280x720: {ms_frame:.4f} ms/frame over {len(totals)} frames")
     stage_ms = {name: float(t[:, i].mean()) for i, name in enumerate(STAGES)}
     stage_ms["weighted_sum"] = 0.0  # fused into accumulate_filtered_data
     P, Q, nB = s.pixels, s.padded_pixels, s.blocks
@@ -428,17 +434,21 @@ def main():
     if not args.no_batched:
         batched = run_batched(args.batched_steps, 5)
         batched["speedup_vs_plain_720p"] = round(ms_frame / batched["ms_per_720p_frame_equiv"], 2)
+        progress(f"batched: {batched['step_ms']} ms/step, x{batched['speedup_vs_plain_720p']} vs plain")
     precision = None
     if not args.no_precision:
         precision = run_precision(args.precision_frames)
+        progress(f"precision 1920x1080: {precision['ms_per_frame']} ms/frame")
     served = None
     if not args.no_served:
         served = run_served(args.serve_seconds, args.serve_budget_gib)
+        progress(f"served: p99 {served['latency_ms_p99']} ms, {served['mpixel_per_s']} Mpixel/s")
     copy_gbs = copy_peak()
     roof["copy_peak_gbs"] = round(copy_gbs, 1)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline()
+        progress(f"cpu baseline: {cpu['value']:.1f} ms/frame on {cpu['cores']} threads")
     out = {
         "metric": "720p ms/frame plain vs batched; p99 ms vs streams; fitter % LDS/HBM peak",
         "value": round(ms_frame, 4),

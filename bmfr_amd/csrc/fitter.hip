@@ -20,694 +20,9 @@
 // so skipping them leaves mins_maxs and weights unchanged.  The factorised
 // matrix is not written back to tmp_data: nothing reads it after the fitter
 // (opencl/bmfr.cpp:449-476).
-#include "device_common.h"
-#include "kernels.h"
+#include "fitter_body.h"
 
 namespace bmfr {
-
-template <int CTRL, int ROW_MASK = 0xf>
-__device__ __forceinline__ float dpp(float old, float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(old), __float_as_int(v), CTRL,
-                                                      ROW_MASK, 0xf, false));
-}
-// Full-mask DPP move: every lane is written, so the compiler can fold it
-// into the consuming VALU op (v_add_f32_dpp / v_min_f32_dpp).
-template <int CTRL>
-__device__ __forceinline__ float dpp_all(float v) {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xf, 0xf, true));
-}
-
-__device__ __forceinline__ float readlane(float v, int lane) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
-}
-
-struct OpAdd {
-    static constexpr float identity = 0.f;
-    __device__ __forceinline__ static float f(float a, float b) { return a + b; }
-};
-struct OpMin {
-    static constexpr float identity = __builtin_huge_valf();
-    __device__ __forceinline__ static float f(float a, float b) { return fminf(a, b); }
-};
-
-// 64-lane reduction; the result is wave-uniform.
-template <typename Op>
-__device__ __forceinline__ float wave_reduce(float v) {
-    v = Op::f(v, dpp_all<0xB1>(v));    // quad_perm [1,0,3,2]
-    v = Op::f(v, dpp_all<0x4E>(v));    // quad_perm [2,3,0,1]
-    v = Op::f(v, dpp_all<0x141>(v));   // row_half_mirror
-    v = Op::f(v, dpp_all<0x140>(v));   // row_mirror
-    v = Op::f(v, dpp<0x142, 0xA>(Op::identity, v));  // row_bcast15 -> rows 1, 3
-    v = Op::f(v, dpp<0x143, 0xC>(Op::identity, v));  // row_bcast31 -> rows 2, 3
-    return readlane(v, 63);
-}
-
-// Reduce NV values over the W waves of one block.  `red` points at this
-// block's [W][16] slot of the round's LDS buffer.
-template <int W, int NV, typename Op, int RS = 16>
-__device__ __forceinline__ void block_reduce(float (&v)[NV], float *red, int wave, int lane) {
-#pragma unroll
-    for (int i = 0; i < NV; ++i) v[i] = wave_reduce<Op>(v[i]);
-    if constexpr (W > 1) {
-        if (lane == 0) {
-#pragma unroll
-            for (int i = 0; i < NV; ++i) red[wave * RS + i] = v[i];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < NV; ++i) {
-            float acc = red[i];
-#pragma unroll
-            for (int w = 1; w < W; ++w) acc = Op::f(acc, red[w * RS + i]);
-            v[i] = acc;
-        }
-    }
-}
-
-// Error-free transformations for the compensated (BMFR_QR_KAHAN) sums.
-__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
-    s = a + b;
-    const float z = s - a;
-    e = (a - (s - z)) + (b - z);
-}
-// s + c += x * y with the product's and the sum's rounding errors kept in c.
-__device__ __forceinline__ void dot2_acc(float &s, float &c, float x, float y) {
-    const float p = x * y;
-    const float pe = __builtin_fmaf(x, y, -p);
-    float t, e;
-    two_sum(s, p, t, e);
-    s = t;
-    c = c + (e + pe);
-}
-// 64-lane reduction of a compensated pair; the result (s + c) is uniform.
-__device__ __forceinline__ float wave_reduce_pair(float s, float c) {
-    float t, e;
-#define BMFR_PAIR_STEP(S2, C2)  \
-    {                           \
-        const float s2 = (S2);  \
-        const float c2 = (C2);  \
-        two_sum(s, s2, t, e);   \
-        s = t;                  \
-        c = c + (c2 + e);       \
-    }
-    BMFR_PAIR_STEP(dpp_all<0xB1>(s), dpp_all<0xB1>(c));
-    BMFR_PAIR_STEP(dpp_all<0x4E>(s), dpp_all<0x4E>(c));
-    BMFR_PAIR_STEP(dpp_all<0x141>(s), dpp_all<0x141>(c));
-    BMFR_PAIR_STEP(dpp_all<0x140>(s), dpp_all<0x140>(c));
-    BMFR_PAIR_STEP((dpp<0x142, 0xA>(0.f, s)), (dpp<0x142, 0xA>(0.f, c)));
-    BMFR_PAIR_STEP((dpp<0x143, 0xC>(0.f, s)), (dpp<0x143, 0xC>(0.f, c)));
-#undef BMFR_PAIR_STEP
-    return readlane(s, 63) + readlane(c, 63);
-}
-
-template <typename TT, int V>
-__device__ __forceinline__ void load_rows(const TT *p, float *out) {
-    if constexpr (sizeof(TT) == 4) {
-        static_assert(V == 4, "fp32 rows load as float4");
-        const float4 x = *reinterpret_cast<const float4 *>(p);
-        out[0] = x.x; out[1] = x.y; out[2] = x.z; out[3] = x.w;
-    } else if constexpr (V == 4) {
-        const uint2 x = *reinterpret_cast<const uint2 *>(p);
-        const __half2 h0 = *reinterpret_cast<const __half2 *>(&x.x);
-        const __half2 h1 = *reinterpret_cast<const __half2 *>(&x.y);
-        out[0] = __low2float(h0); out[1] = __high2float(h0);
-        out[2] = __low2float(h1); out[3] = __high2float(h1);
-    } else {
-        static_assert(V == 8, "fp16 rows load as 4 or 8 halves");
-        const uint4 x = *reinterpret_cast<const uint4 *>(p);
-        const uint32_t w[4] = {x.x, x.y, x.z, x.w};
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const __half2 h = *reinterpret_cast<const __half2 *>(&w[i]);
-            out[2 * i] = __low2float(h);
-            out[2 * i + 1] = __high2float(h);
-        }
-    }
-}
-
-struct FitterParams {
-    const void *tmp;
-    float *mins_maxs;
-    float *weights;
-    const double *noise;   // the frame's noise table (kNoiseTable terms)
-    int32_t *flags;        // per block: 1 = re-factor in FP64 (null: not wanted)
-    float threshold;       // flag when min |R_ii| < threshold * max |R_ii|
-    int32_t blocks;
-    int32_t frame;
-};
-
-// Per-block outputs of the FP32 fitter besides weights and mins_maxs.
-struct BlockFlag {
-    int32_t *flag;
-    float threshold;
-};
-
-// Back substitution, bmfr.cl:658-699, by one wave.  Lane L = 3*c + ch holds
-// column c (0..10) of R for colour channel ch: R[r][c] = elem(c, r) (the
-// factorised column c at row r, r < c, wave-uniform), R[c][c] = norms[c];
-// column 10 is Q^T b = rows 0..9 of colour column 10+ch.  Weights are
-// written by lane 30 + ch.
-template <typename Elem>
-__device__ __forceinline__ void back_substitution(Elem elem, const float (&norms)[kFeatures],
-                                                  float *__restrict__ weights, bool active, int lane) {
-    const int cl = lane / 3;
-    const int ch = lane - 3 * cl;
-    float rc[kFeatures];
-#pragma unroll
-    for (int r = 0; r < kFeatures; ++r) rc[r] = 0.f;
-#pragma unroll
-    for (int c = 0; c < kFeatures; ++c) {
-#pragma unroll
-        for (int r = 0; r < kFeatures; ++r) {
-            float val;
-            if (r < c) val = elem(c, r);
-            else if (r == c) val = norms[c];
-            else val = 0.f;
-            if (cl == c) rc[r] = val;
-        }
-    }
-#pragma unroll
-    for (int r = 0; r < kFeatures; ++r) {
-        const float v0 = elem(10, r);
-        const float v1 = elem(11, r);
-        const float v2 = elem(12, r);
-        if (cl == 10) rc[r] = ch == 0 ? v0 : (ch == 1 ? v1 : v2);
-    }
-#pragma unroll
-    for (int i = kREdge - 2; i >= 0; --i) {
-        const float divider = __shfl(rc[i], 3 * i + ch);
-        if (cl >= i) rc[i] = rc[i] / divider;
-        float acc = __shfl(rc[i], 30 + ch);
-#pragma unroll
-        for (int j = i + 1; j < kREdge - 1; ++j) acc = acc - __shfl(rc[i], 3 * j + ch);
-        if (cl == 10) rc[i] = acc;
-        const float wi = __shfl(rc[i], 30 + ch);
-        if (cl == i) {
-#pragma unroll
-            for (int r = 0; r <= i; ++r) rc[r] = rc[r] * wi;
-        }
-    }
-    if (active && cl == 10 && ch < 3) {
-#pragma unroll
-        for (int r = 0; r < kFeatures; ++r) weights[r * 3 + ch] = rc[r];
-    }
-}
-
-// One 32x32 block: RPL rows per lane, BPG blocks per work-group, TT tmp
-// element type, HALF_ROUND: round every stored element to binary16 like the
-// reference's USE_HALF_PRECISION_IN_TMP_DATA=1 (bmfr.cl:255-258, 540, 651).
-// `red` is this work-group's LDS exchange area; `active` false recomputes a
-// valid block (keeping barriers uniform) without storing.
-// COMP: compensated norm and dot sums (BMFR_QR_KAHAN).  `flag` (optional)
-// receives 1 when min |R_ii| < threshold * max |R_ii|.
-constexpr int red_slots(bool comp) { return comp ? 32 : 16; }
-
-template <int RPL, int BPG, typename TT, bool HALF_ROUND, bool COMP>
-__device__ __forceinline__ void fitter_block(const TT *__restrict__ src, float *__restrict__ mins_maxs,
-                                             float *__restrict__ weights, const double *__restrict__ noise,
-                                             BlockFlag flag, bool active,
-                                             float (&red)[2][BPG][16 / RPL][red_slots(COMP)]) {
-    constexpr int RS = red_slots(COMP);
-    constexpr int W = 16 / RPL;            // waves per block
-    constexpr int T = 64 * W;              // lanes per block
-    constexpr int V = (RPL < (int)(16 / sizeof(TT))) ? RPL : (int)(16 / sizeof(TT));
-    constexpr int Q = RPL / V;             // vector loads per feature
-
-    const int tid = threadIdx.x;
-    const int sub = tid / T;
-    const int t = tid - sub * T;
-    const int lane = tid & 63;
-    const int wave = t >> 6;
-    struct {
-        const double *noise;
-    } p = {noise};
-
-    // Row held in element k of this lane: rows t*V..t*V+V-1 of each T*V strip.
-    auto row_of = [&](int k) { return (k / V) * (T * V) + t * V + (k % V); };
-
-    float a[kBufferCount][RPL];
-#pragma unroll
-    for (int f = 0; f < kBufferCount; ++f)
-#pragma unroll
-        for (int q = 0; q < Q; ++q) load_rows<TT, V>(src + f * kBlockPixels + q * (T * V) + t * V, &a[f][q * V]);
-
-    int round = 0;
-    // ---- scaling, bmfr.cl:511-542 (max taken as -min(-x): exact) ----
-    float bmin[kScaled], bmax[kScaled];
-    {
-        float v[2 * kScaled];
-#pragma unroll
-        for (int i = 0; i < kScaled; ++i) {
-            float mn = INFINITY, nmx = INFINITY;
-#pragma unroll
-            for (int k = 0; k < RPL; ++k) {
-                mn = fminf(mn, a[kNotScaled + i][k]);
-                nmx = fminf(nmx, -a[kNotScaled + i][k]);
-            }
-            v[i] = mn;
-            v[kScaled + i] = nmx;
-        }
-        block_reduce<W, 2 * kScaled, OpMin, RS>(v, &red[round & 1][sub][0][0], wave, lane);
-        ++round;
-#pragma unroll
-        for (int i = 0; i < kScaled; ++i) {
-            bmin[i] = v[i];
-            bmax[i] = -v[kScaled + i];
-        }
-    }
-    if (active && t == 0) {
-        float *mm = mins_maxs;
-#pragma unroll
-        for (int i = 0; i < kScaled; ++i) {
-            mm[2 * i + 0] = bmin[i];
-            mm[2 * i + 1] = bmax[i];
-        }
-    }
-    // scale (bmfr.cl:200-205).  (v-min)/(max-min) is evaluated as Markstein's
-    // correctly rounded quotient: y = RN(1/d) once per feature, then
-    // q = RN(a*y), r = RN(a - q*d) (exact), q' = RN(q + r*y) = RN(a/d).
-#pragma unroll
-    for (int i = 0; i < kScaled; ++i) {
-        const float d = bmax[i] - bmin[i];
-        const bool divide = fabsf(d) > 1.0f;
-        const float y = 1.0f / d;
-#pragma unroll
-        for (int k = 0; k < RPL; ++k) {
-            const float num = a[kNotScaled + i][k] - bmin[i];
-            float sc = num;
-            if (divide) {
-                const float q = num * y;
-                const float r = __builtin_fmaf(-q, d, num);
-                sc = __builtin_fmaf(r, y, q);
-            }
-            a[kNotScaled + i][k] = HALF_ROUND ? round_half(sc) : sc;
-        }
-    }
-    // ---- feature noise of the first pass, bmfr.cl:623-627: value + noise in
-    // FP64, rounded once to float (NOISE_AMOUNT is a double literal) ----
-#pragma unroll
-    for (int f = 1; f < kFeatures; ++f) {
-        const double *nz = p.noise + (f - 1) * kBlockPixels;
-#pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            double n[V];
-#pragma unroll
-            for (int e = 0; e < V; e += 2) {
-                const double2 x = *reinterpret_cast<const double2 *>(nz + q * (T * V) + t * V + e);
-                n[e] = x.x;
-                n[e + 1] = x.y;
-            }
-#pragma unroll
-            for (int e = 0; e < V; ++e)
-                a[f][q * V + e] = (float)((double)a[f][q * V + e] + n[e]);
-        }
-    }
-
-    // ---- Householder QR over the 10 feature columns, bmfr.cl:549-656 ----
-    float norms[kFeatures];
-#pragma unroll
-    for (int col = 0; col < kFeatures; ++col) {
-        // Pivot row `col` sits in lane col/V (wave 0), element col%V.
-        float r1[1];
-        float xc = readlane(a[col][col % V], col / V);
-        if constexpr (!COMP) {
-            float s = 0.f;
-#pragma unroll
-            for (int k = 0; k < RPL; ++k) {
-                const float x = row_of(k) > col ? a[col][k] : 0.f;
-                s = __builtin_fmaf(x, x, s);
-            }
-            r1[0] = wave_reduce<OpAdd>(s);
-        } else {
-            float s = 0.f, c = 0.f;
-#pragma unroll
-            for (int k = 0; k < RPL; ++k) {
-                const float x = row_of(k) > col ? a[col][k] : 0.f;
-                dot2_acc(s, c, x, x);
-            }
-            r1[0] = wave_reduce_pair(s, c);
-        }
-        if constexpr (W > 1) {
-            float *rd = &red[round & 1][sub][0][0];
-            if (lane == 0) {
-                rd[wave * RS + 0] = r1[0];
-                if (wave == 0) rd[1] = xc;
-            }
-            __syncthreads();
-            float acc = rd[0], cc = 0.f;
-#pragma unroll
-            for (int w = 1; w < W; ++w) {
-                if constexpr (COMP) {
-                    float t, e;
-                    two_sum(acc, rd[w * RS], t, e);
-                    acc = t;
-                    cc = cc + e;
-                } else {
-                    acc = acc + rd[w * RS];
-                }
-            }
-            r1[0] = acc + cc;
-            xc = rd[1];
-        }
-        ++round;
-        const float sigma = r1[0];
-        const float norm = sqrtf(sigma + xc * xc);
-        const float uc = xc - norm;
-        const float uu = sigma + uc * uc;
-        norms[col] = norm;
-
-        float u[RPL];
-#pragma unroll
-        for (int k = 0; k < RPL; ++k) {
-            const int row = row_of(k);
-            u[k] = row > col ? a[col][k] : (row == col ? uc : 0.f);
-        }
-        constexpr int kMaxTrail = kBufferCount - 1;
-        float d[kMaxTrail];
-#pragma unroll
-        for (int j = 0; j < kMaxTrail; ++j) d[j] = 0.f;
-        {
-            const int nv = kBufferCount - 1 - col;
-            if constexpr (!COMP) {
-#pragma unroll
-                for (int j = col + 1; j < kBufferCount; ++j) {
-                    float s = 0.f;
-#pragma unroll
-                    for (int k = 0; k < RPL; ++k) s = __builtin_fmaf(u[k], a[j][k], s);
-                    d[j - col - 1] = wave_reduce<OpAdd>(s);
-                }
-            } else {
-#pragma unroll
-                for (int j = col + 1; j < kBufferCount; ++j) {
-                    float s = 0.f, c = 0.f;
-#pragma unroll
-                    for (int k = 0; k < RPL; ++k) dot2_acc(s, c, u[k], a[j][k]);
-                    d[j - col - 1] = wave_reduce_pair(s, c);
-                }
-            }
-            if constexpr (W > 1) {
-                float *rd = &red[round & 1][sub][0][0];
-                if (lane == 0) {
-#pragma unroll
-                    for (int i = 0; i < kMaxTrail; ++i)
-                        if (i < nv) rd[wave * RS + i] = d[i];
-                }
-                __syncthreads();
-#pragma unroll
-                for (int i = 0; i < kMaxTrail; ++i) {
-                    if (i < nv) {
-                        float acc = rd[i], cc = 0.f;
-#pragma unroll
-                        for (int w = 1; w < W; ++w) {
-                            if constexpr (COMP) {
-                                float t, e;
-                                two_sum(acc, rd[w * RS + i], t, e);
-                                acc = t;
-                                cc = cc + e;
-                            } else {
-                                acc = acc + rd[w * RS + i];
-                            }
-                        }
-                        d[i] = acc + cc;
-                    }
-                }
-            }
-            ++round;
-        }
-        // a_j -= u * (2 u.a_j / uu): one reciprocal per pivot and a fused
-        // multiply-add per element (the reference divides per element,
-        // bmfr.cl:650; the difference is within a couple of ulp).
-        const float inv_uu = 1.f / uu;
-#pragma unroll
-        for (int j = col + 1; j < kBufferCount; ++j) {
-            const float fct = (2.f * d[j - col - 1]) * inv_uu;
-#pragma unroll
-            for (int k = 0; k < RPL; ++k) {
-                const float v = __builtin_fmaf(-u[k], fct, a[j][k]);
-                a[j][k] = HALF_ROUND ? round_half(v) : v;
-            }
-        }
-    }
-
-    // conditioning flag for the FP64 re-factorisation (BMFR_QR_KAHAN_FP64)
-    if (flag.flag && active && t == 0) {
-        float rmin = norms[0], rmax = norms[0];
-#pragma unroll
-        for (int i = 1; i < kFeatures; ++i) {
-            rmin = fminf(rmin, fabsf(norms[i]));
-            rmax = fmaxf(rmax, fabsf(norms[i]));
-        }
-        *flag.flag = (rmin < flag.threshold * rmax || !(rmin == rmin)) ? 1 : 0;
-    }
-
-    // ---- back substitution, bmfr.cl:658-699, in wave 0 ----
-    if (wave == 0)
-        back_substitution([&](int c, int r) { return readlane(a[c][r % V], r / V); }, norms, weights, active,
-                          lane);
-}
-
-// ---- One-wave fitter for binary16 tmp_data: the default configuration -------
-// The same operations as fitter_block with RPL = 16 (one wave per block, 16
-// rows per lane, plain FP32 sums), scheduled for one wave: every pivot step
-// forms all of its trailing dot products, each with two partial sums (even
-// and odd rows of a lane), and reduces them step-major (one DPP step for all
-// columns before the next), and the pivot loop is unrolled at compile time
-// so that each step reduces exactly its 12 - col columns.  Independent DPP
-// chains then fill each other's hazard slots instead of s_nop, and the
-// wave-uniform results stay in SGPRs.  Every stored element is rounded to
-// binary16 like the reference's stores (bmfr.cl:540, 651); the feature noise
-// is added in FP64 right after the load (bmfr.cl:623-627), so its loads are
-// in flight during the scaling phase.
-struct Rows16 {
-    float a[kBufferCount][16];
-    __device__ __forceinline__ float get(int f, int k) const { return a[f][k]; }
-    __device__ __forceinline__ void set(int f, int k, float v) { a[f][k] = round_half(v); }
-};
-
-// N independent 64-lane reductions, step-major; results are wave-uniform and
-// each equals wave_reduce<Op> of its input.
-template <typename Op, int N>
-__device__ __forceinline__ void wave_reduce_n(float (&v)[N]) {
-#pragma unroll
-    for (int i = 0; i < N; ++i) v[i] = Op::f(v[i], dpp_all<0xB1>(v[i]));
-#pragma unroll
-    for (int i = 0; i < N; ++i) v[i] = Op::f(v[i], dpp_all<0x4E>(v[i]));
-#pragma unroll
-    for (int i = 0; i < N; ++i) v[i] = Op::f(v[i], dpp_all<0x141>(v[i]));
-#pragma unroll
-    for (int i = 0; i < N; ++i) v[i] = Op::f(v[i], dpp_all<0x140>(v[i]));
-#pragma unroll
-    for (int i = 0; i < N; ++i) v[i] = Op::f(v[i], (dpp<0x142, 0xA>(Op::identity, v[i])));
-#pragma unroll
-    for (int i = 0; i < N; ++i) v[i] = Op::f(v[i], (dpp<0x143, 0xC>(Op::identity, v[i])));
-#pragma unroll
-    for (int i = 0; i < N; ++i) v[i] = readlane(v[i], 63);
-}
-
-constexpr int kH16V = 8;   // rows per 16-byte load: element k is row (k/8)*512 + 8*lane + k%8
-__device__ __forceinline__ int h16_row(int k, int lane) { return (k / kH16V) * (64 * kH16V) + lane * kH16V + (k % kH16V); }
-
-// Pivot step COL: norm, u, the 12 - COL trailing dot products, update.
-template <int COL>
-__device__ __forceinline__ void h16_pivot(Rows16 &m, float (&norms)[kFeatures], int lane) {
-    constexpr int NV = kBufferCount - 1 - COL;
-    const float xc = readlane(m.get(COL, COL % kH16V), COL / kH16V);
-    float r1[1];
-    {
-        float s0 = 0.f, s1 = 0.f;
-#pragma unroll
-        for (int k = 0; k < 16; k += 2) {
-            const float x0 = h16_row(k, lane) > COL ? m.get(COL, k) : 0.f;
-            const float x1 = h16_row(k + 1, lane) > COL ? m.get(COL, k + 1) : 0.f;
-            s0 = __builtin_fmaf(x0, x0, s0);
-            s1 = __builtin_fmaf(x1, x1, s1);
-        }
-        r1[0] = s0 + s1;
-    }
-    wave_reduce_n<OpAdd, 1>(r1);
-    const float sigma = r1[0];
-    const float norm = sqrtf(sigma + xc * xc);
-    const float uc = xc - norm;
-    const float uu = sigma + uc * uc;
-    norms[COL] = norm;
-    float u[16];
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        const int row = h16_row(k, lane);
-        u[k] = row > COL ? m.get(COL, k) : (row == COL ? uc : 0.f);
-    }
-    float d[NV];
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-        const int j = COL + 1 + i;
-        float s0 = 0.f, s1 = 0.f;
-#pragma unroll
-        for (int k = 0; k < 16; k += 2) {
-            s0 = __builtin_fmaf(u[k], m.get(j, k), s0);
-            s1 = __builtin_fmaf(u[k + 1], m.get(j, k + 1), s1);
-        }
-        d[i] = s0 + s1;
-    }
-    wave_reduce_n<OpAdd, NV>(d);
-    const float inv_uu = 1.f / uu;
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-        const int j = COL + 1 + i;
-        const float fct = (2.f * d[i]) * inv_uu;
-#pragma unroll
-        for (int k = 0; k < 16; ++k) m.set(j, k, __builtin_fmaf(-u[k], fct, m.get(j, k)));
-    }
-}
-
-template <int COL>
-__device__ __forceinline__ void h16_pivots(Rows16 &m, float (&norms)[kFeatures], int lane) {
-    if constexpr (COL < kFeatures) {
-        h16_pivot<COL>(m, norms, lane);
-        h16_pivots<COL + 1>(m, norms, lane);
-    }
-}
-
-__device__ __forceinline__ void fitter_block_h16(const __half *__restrict__ src, float *__restrict__ mins_maxs,
-                                                 float *__restrict__ weights, const double *__restrict__ noise,
-                                                 bool active) {
-    const int lane = threadIdx.x & 63;
-    Rows16 m;
-#pragma unroll
-    for (int f = 0; f < kBufferCount; ++f)
-#pragma unroll
-        for (int q = 0; q < 2; ++q)
-            load_rows<__half, kH16V>(src + f * kBlockPixels + q * (64 * kH16V) + lane * kH16V, &m.a[f][q * kH16V]);
-
-    // ---- scaling, bmfr.cl:511-542 (max taken as -min(-x): exact) ----
-    float bmin[kScaled], bmax[kScaled];
-    {
-        float v[2 * kScaled];
-#pragma unroll
-        for (int i = 0; i < kScaled; ++i) {
-            float mn = INFINITY, nmx = INFINITY;
-#pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                const float x = m.get(kNotScaled + i, k);
-                mn = fminf(mn, x);
-                nmx = fminf(nmx, -x);
-            }
-            v[i] = mn;
-            v[kScaled + i] = nmx;
-        }
-        wave_reduce_n<OpMin, 2 * kScaled>(v);
-#pragma unroll
-        for (int i = 0; i < kScaled; ++i) {
-            bmin[i] = v[i];
-            bmax[i] = -v[kScaled + i];
-        }
-    }
-    if (active && lane == 0) {
-#pragma unroll
-        for (int i = 0; i < kScaled; ++i) {
-            mins_maxs[2 * i + 0] = bmin[i];
-            mins_maxs[2 * i + 1] = bmax[i];
-        }
-    }
-    // (v-min)/(max-min) as Markstein's corrected quotient (see fitter_block)
-#pragma unroll
-    for (int i = 0; i < kScaled; ++i) {
-        const float d = bmax[i] - bmin[i];
-        const bool divide = fabsf(d) > 1.0f;
-        const float y = 1.0f / d;
-#pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            const float num = m.get(kNotScaled + i, k) - bmin[i];
-            float sc = num;
-            if (divide) {
-                const float q = num * y;
-                const float r = __builtin_fmaf(-q, d, num);
-                sc = __builtin_fmaf(r, y, q);
-            }
-            m.set(kNotScaled + i, k, sc);
-        }
-    }
-    // ---- feature noise of the first pass, in FP64, rounded once to float ----
-#pragma unroll
-    for (int f = 1; f < kFeatures; ++f) {
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-            const double *nz = noise + (f - 1) * kBlockPixels + q * (64 * kH16V) + lane * kH16V;
-#pragma unroll
-            for (int e = 0; e < kH16V; e += 2) {
-                const double2 x = *reinterpret_cast<const double2 *>(nz + e);
-                m.a[f][q * kH16V + e] = (float)((double)m.a[f][q * kH16V + e] + x.x);
-                m.a[f][q * kH16V + e + 1] = (float)((double)m.a[f][q * kH16V + e + 1] + x.y);
-            }
-        }
-    }
-
-    // ---- Householder QR over the 10 feature columns, bmfr.cl:549-656 ----
-    float norms[kFeatures];
-    h16_pivots<0>(m, norms, lane);
-
-    // ---- back substitution, bmfr.cl:658-699 ----
-    back_substitution([&](int c, int r) { return readlane(m.get(c, r % kH16V), r / kH16V); }, norms, weights,
-                      active, lane);
-}
-
-// The one-wave body serves binary16 tmp_data with RPL = 16 and the plain FP32
-// sums; every other combination runs fitter_block.
-template <int RPL, typename TT, bool HALF_ROUND, bool COMP>
-constexpr bool kPackedH16 = RPL == 16 && sizeof(TT) == 2 && HALF_ROUND && !COMP;
-
-template <int RPL, int BPG, typename TT, bool HALF_ROUND, bool COMP>
-__global__ __launch_bounds__(64 * (16 / RPL) * BPG) void k_fitter(const FitterParams p) {
-    __shared__ float red[2][BPG][16 / RPL][red_slots(COMP)];
-    const int sub = threadIdx.x / (64 * (16 / RPL));
-    const int blk_raw = blockIdx.x * BPG + sub;
-    const bool active = blk_raw < p.blocks;
-    const int blk = active ? blk_raw : p.blocks - 1;
-    if constexpr (kPackedH16<RPL, TT, HALF_ROUND, COMP>) {
-        fitter_block_h16(reinterpret_cast<const __half *>(p.tmp) + (int64_t)blk * (kBufferCount * kBlockPixels),
-                         p.mins_maxs + (int64_t)blk * (kScaled * 2), p.weights + (int64_t)blk * (kFeatures * 3),
-                         p.noise, active);
-    } else {
-        fitter_block<RPL, BPG, TT, HALF_ROUND, COMP>(
-            reinterpret_cast<const TT *>(p.tmp) + (int64_t)blk * (kBufferCount * kBlockPixels),
-            p.mins_maxs + (int64_t)blk * (kScaled * 2), p.weights + (int64_t)blk * (kFeatures * 3),
-            p.noise, BlockFlag{p.flags ? p.flags + blk : nullptr, p.threshold}, active, red);
-    }
-}
-
-// Persistent batched fitter: the work-groups stride over the concatenated
-// block list of every stream in the batch (edge blocks included).  The trip
-// count depends only on blockIdx, so barriers stay uniform.
-template <int RPL, int BPG, typename TT, bool HALF_ROUND, bool COMP>
-__global__ __launch_bounds__(64 * (16 / RPL) * BPG) void kb_fitter(const BatchLaunch L) {
-    __shared__ float red[2][BPG][16 / RPL][red_slots(COMP)];
-    const int32_t *starts = L.starts + kStartsBlocks * (kMaxBatch + 1);
-    const int total = starts[L.n];
-    const int sub = threadIdx.x / (64 * (16 / RPL));
-    for (int g = blockIdx.x; g * BPG < total; g += gridDim.x) {
-        const int raw = g * BPG + sub;
-        const bool active = raw < total;
-        const int u = active ? raw : total - 1;
-        int s = 0;
-        while (s + 1 < L.n && starts[s + 1] <= u) ++s;
-        const int blk = u - starts[s];
-        const BatchStream &bs = L.streams[s];
-        int32_t *fl = (int32_t *)bs.flags;
-        if constexpr (kPackedH16<RPL, TT, HALF_ROUND, COMP>) {
-            fitter_block_h16(reinterpret_cast<const __half *>(bs.tmp_data) + (int64_t)blk * (kBufferCount * kBlockPixels),
-                             (float *)bs.mins_maxs + (int64_t)blk * (kScaled * 2),
-                             (float *)bs.weights + (int64_t)blk * (kFeatures * 3), (const double *)bs.noise, active);
-        } else {
-            fitter_block<RPL, BPG, TT, HALF_ROUND, COMP>(
-                reinterpret_cast<const TT *>(bs.tmp_data) + (int64_t)blk * (kBufferCount * kBlockPixels),
-                (float *)bs.mins_maxs + (int64_t)blk * (kScaled * 2),
-                (float *)bs.weights + (int64_t)blk * (kFeatures * 3), (const double *)bs.noise,
-                BlockFlag{fl ? fl + blk : nullptr, bs.args.refactor_threshold}, active, red);
-        }
-        // A block ends with round 20 in LDS buffer 0 and the next one starts
-        // with round 0 in the same buffer: wait until every wave has read it.
-        if constexpr (16 / RPL > 1) __syncthreads();
-    }
-}
 
 // ---- FP64 re-factorisation (BMFR_QR_KAHAN_FP64) ------------------------------
 // One 256-thread work-group per flagged block (unflagged blocks exit at once).
@@ -858,59 +173,6 @@ __global__ __launch_bounds__(256) void kb_refactor(const BatchLaunch L) {
                                    (const double *)bs.noise, (float *)bs.weights + (int64_t)blk * (kFeatures * 3));
 }
 
-template <int RPL, int BPG, bool COMP>
-static hipError_t launch_rpl(const FitterParams &fp, bool tmp_f16, hipStream_t s) {
-    constexpr int threads = 64 * (16 / RPL) * BPG;
-    const dim3 grid((fp.blocks + BPG - 1) / BPG);
-    if (tmp_f16)
-        hipLaunchKernelGGL((k_fitter<RPL, BPG, __half, true, COMP>), grid, dim3(threads), 0, s, fp);
-    else
-        hipLaunchKernelGGL((k_fitter<RPL, BPG, float, false, COMP>), grid, dim3(threads), 0, s, fp);
-    return hipGetLastError();
-}
-
-template <int RPL, int BPG, bool COMP>
-static hipError_t launch_batch_rpl(const BatchLaunch &L, int total, bool tmp_f16, int grid_cap,
-                                   hipStream_t s) {
-    constexpr int threads = 64 * (16 / RPL) * BPG;
-    int groups = (total + BPG - 1) / BPG;
-    if (grid_cap > 0 && groups > grid_cap) groups = grid_cap;
-    if (tmp_f16)
-        hipLaunchKernelGGL((kb_fitter<RPL, BPG, __half, true, COMP>), dim3(groups), dim3(threads), 0, s, L);
-    else
-        hipLaunchKernelGGL((kb_fitter<RPL, BPG, float, false, COMP>), dim3(groups), dim3(threads), 0, s, L);
-    return hipGetLastError();
-}
-
-template <bool COMP>
-static hipError_t dispatch_single(const FitterParams &fp, bool h, const FitterConfig &cfg, hipStream_t s) {
-    switch (cfg.rows_per_lane * 10 + cfg.blocks_per_group) {
-        case 41: return launch_rpl<4, 1, COMP>(fp, h, s);
-        case 42: return launch_rpl<4, 2, COMP>(fp, h, s);
-        case 81: return launch_rpl<8, 1, COMP>(fp, h, s);
-        case 82: return launch_rpl<8, 2, COMP>(fp, h, s);
-        case 161: return launch_rpl<16, 1, COMP>(fp, h, s);
-        case 162: return launch_rpl<16, 2, COMP>(fp, h, s);
-        case 164: return launch_rpl<16, 4, COMP>(fp, h, s);
-        default: return hipErrorInvalidValue;
-    }
-}
-
-template <bool COMP>
-static hipError_t dispatch_batch(const BatchLaunch &L, int total, bool h, const FitterConfig &cfg,
-                                 int grid_cap, hipStream_t s) {
-    switch (cfg.rows_per_lane * 10 + cfg.blocks_per_group) {
-        case 41: return launch_batch_rpl<4, 1, COMP>(L, total, h, grid_cap, s);
-        case 42: return launch_batch_rpl<4, 2, COMP>(L, total, h, grid_cap, s);
-        case 81: return launch_batch_rpl<8, 1, COMP>(L, total, h, grid_cap, s);
-        case 82: return launch_batch_rpl<8, 2, COMP>(L, total, h, grid_cap, s);
-        case 161: return launch_batch_rpl<16, 1, COMP>(L, total, h, grid_cap, s);
-        case 162: return launch_batch_rpl<16, 2, COMP>(L, total, h, grid_cap, s);
-        case 164: return launch_batch_rpl<16, 4, COMP>(L, total, h, grid_cap, s);
-        default: return hipErrorInvalidValue;
-    }
-}
-
 static hipError_t launch_refactor_all(const StageArgs &a, const StageBuffers &b, const int32_t *flags,
                                       float *out, hipStream_t s) {
     RefactorParams rp;
@@ -925,6 +187,24 @@ static hipError_t launch_refactor_all(const StageArgs &a, const StageBuffers &b,
     return hipGetLastError();
 }
 
+hipError_t fitter_single_plain(const FitterParams &fp, bool tmp_f16, const FitterConfig &cfg, hipStream_t s);
+hipError_t fitter_single_comp(const FitterParams &fp, bool tmp_f16, const FitterConfig &cfg, hipStream_t s);
+hipError_t fitter_batch_plain(const BatchLaunch &L, int total, bool tmp_f16, const FitterConfig &cfg, int grid_cap,
+                              hipStream_t s);
+hipError_t fitter_batch_comp(const BatchLaunch &L, int total, bool tmp_f16, const FitterConfig &cfg, int grid_cap,
+                             hipStream_t s);
+
+hipError_t fitter_dispatch_single(const FitterParams &fp, bool tmp_f16, bool comp, const FitterConfig &cfg,
+                                  hipStream_t s) {
+    return comp ? fitter_single_comp(fp, tmp_f16, cfg, s) : fitter_single_plain(fp, tmp_f16, cfg, s);
+}
+
+hipError_t fitter_dispatch_batch(const BatchLaunch &L, int total, bool tmp_f16, bool comp,
+                                 const FitterConfig &cfg, int grid_cap, hipStream_t s) {
+    return comp ? fitter_batch_comp(L, total, tmp_f16, cfg, grid_cap, s)
+                : fitter_batch_plain(L, total, tmp_f16, cfg, grid_cap, s);
+}
+
 hipError_t launch_fitter(const StageArgs &a, const StageBuffers &b, const FitterConfig &cfg,
                          int32_t qr_mode, hipStream_t s) {
     FitterParams fp;
@@ -937,7 +217,7 @@ hipError_t launch_fitter(const StageArgs &a, const StageBuffers &b, const Fitter
     fp.blocks = a.blocks;
     fp.frame = a.frame;
     const bool h = a.tmp_f16 != 0;
-    hipError_t e = qr_mode >= 1 ? dispatch_single<true>(fp, h, cfg, s) : dispatch_single<false>(fp, h, cfg, s);
+    hipError_t e = fitter_dispatch_single(fp, h, qr_mode >= 1, cfg, s);
     if (e != hipSuccess || qr_mode != 2) return e;
     return launch_refactor_all(a, b, (const int32_t *)b.flags, (float *)b.weights, s);
 }
@@ -949,8 +229,7 @@ hipError_t launch_fitter_fp64_all(const StageArgs &a, const StageBuffers &b, flo
 hipError_t launch_batch_fitter(const BatchLaunch &L, int total, bool t16, const FitterConfig &cfg,
                                int grid_cap, int32_t qr_mode, hipStream_t s) {
     if (total <= 0) return hipErrorInvalidValue;
-    hipError_t e = qr_mode >= 1 ? dispatch_batch<true>(L, total, t16, cfg, grid_cap, s)
-                                : dispatch_batch<false>(L, total, t16, cfg, grid_cap, s);
+    hipError_t e = fitter_dispatch_batch(L, total, t16, qr_mode >= 1, cfg, grid_cap, s);
     if (e != hipSuccess || qr_mode != 2) return e;
     if (t16) hipLaunchKernelGGL((kb_refactor<__half, true>), dim3(total), dim3(256), 0, s, L);
     else hipLaunchKernelGGL((kb_refactor<float, false>), dim3(total), dim3(256), 0, s, L);

@@ -46,6 +46,25 @@ def stage_bytes(P, Q, nB, s=4, t=2):
 STAGES = ["accumulate_noisy_data", "fitter", "weighted_sum", "accumulate_filtered_data", "taa"]
 
 
+def chain_kernels(fuse, P, Q, nB, s=4):
+    """The kernels of the chained frame at a fuse level: name -> (index of
+    the stage whose events bracket it, algorithmic bytes per launch).  A fused
+    kernel keeps the intermediates it folds in on chip: `filtered` (12 B/px
+    written and read), and at level 2 `tone_mapped` (the same) and the
+    second read of prev_pixel (8 B/px)."""
+    sb = stage_bytes(P, Q, nB, s)
+    k = {"accumulate_noisy_data": (0, sb["accumulate_noisy_data"]), "fitter": (1, sb["fitter"])}
+    if fuse == 0:
+        k.update({n: (i, sb[n]) for i, n in enumerate(STAGES) if i >= 2})
+    elif fuse == 1:
+        k["weighted_sum+accumulate_filtered_data"] = (
+            3, sb["weighted_sum"] + sb["accumulate_filtered_data"] - 24 * P)
+        k["taa"] = (4, sb["taa"])
+    else:
+        k["filter_taa"] = (3, sb["weighted_sum"] + sb["accumulate_filtered_data"] + sb["taa"] - 56 * P)
+    return k
+
+
 def run_plain(args):
     """Timed frames record events only at the first stage's start and the
     last stage's end (nothing between kernels); a second pass with events
@@ -53,6 +72,7 @@ def run_plain(args):
     import bmfr_amd
     p = bmfr_amd.default_params(W, H)
     st = bmfr_amd.Stream(p)
+    st.set_fused(args.fuse)
     if args.fitter_config:
         st.set_fitter_config(*[int(x) for x in args.fitter_config.split(",")])
     sc = bmfr_amd.Scene(SEED, SPHERES, 1e-3, W, H)
@@ -136,7 +156,7 @@ def run_batched(steps, warmup):
         "ms_per_720p_frame_equiv": round(step_ms * p720 / pixels, 5),
         "aggregate_frames_per_s": round(n / (step_ms * 1e-3), 1),
         "mpixel_per_s": round(pixels / (step_ms * 1e-3) / 1e6, 1),
-        "stage_ms": {k: round(float(v), 4) for k, v in zip(STAGES, stage[:5])},
+        "stage_ms": {k: round(float(stage[i]), 4) for k, (i, _) in chain_kernels(2, 1, 1, 1).items()},
         "graph": True, "autotune": tune,
     }
 
@@ -403,6 +423,7 @@ def main():
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--fitter-config", default="")
+    ap.add_argument("--fuse", type=int, default=1, help="stages 3-5: 0 separate, 1 3+4 fused, 2 3+4+5 fused")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=40)
     ap.add_argument("--no-batched", action="store_true")
@@ -420,11 +441,10 @@ def main():
     totals, t, wall, s = run_plain(args)
     ms_frame = float(totals.mean())
     progress(f"plain 1280x720: {ms_frame:.4f} ms/frame over {len(totals)} frames")
-    stage_ms = {name: float(t[:, i].mean()) for i, name in enumerate(STAGES)}
-    stage_ms["weighted_sum"] = 0.0  # fused into accumulate_filtered_data
     P, Q, nB = s.pixels, s.padded_pixels, s.blocks
-    sb = stage_bytes(P, Q, nB)
-    sb["accumulate_filtered_data"] += sb.pop("weighted_sum") - 2 * 12 * P  # no filtered round trip
+    kern = chain_kernels(args.fuse, P, Q, nB)
+    stage_ms = {k: float(t[:, i].mean()) for k, (i, _) in kern.items()}
+    sb = {k: b for k, (_, b) in kern.items()}
     stage_gbs = {k: sb[k] / (stage_ms[k] * 1e-3) / 1e9 for k in sb}
     dom = max(sb, key=lambda k: stage_ms[k])
     roof = {"kernel": dom, "bound": "hbm", "achieved": round(stage_gbs[dom], 1),
@@ -464,7 +484,7 @@ def main():
         "data": "synthetic: seeded procedural plane + 24 spheres, moving camera, rendered on the GPU",
         "config": {"workload": "plain BMFR single stream 1280x720 (984 blocks/frame), FP32 "
                                "G-buffers, FP16 tmp_data", "width": W, "height": H,
-                   "blocks": nB, "timed_frames": args.steps},
+                   "blocks": nB, "timed_frames": args.steps, "fuse_level": args.fuse},
         "mpixel_per_s": round(P / (ms_frame * 1e-3) / 1e6, 1),
         "frames_per_s_wall": round(args.steps / wall, 1),
         "ms_per_frame_with_stage_events": round(float(t[:, 5].mean()), 4),

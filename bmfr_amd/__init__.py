@@ -28,6 +28,7 @@ STORAGE_F16 = 1
 QR_FP32 = 0
 QR_KAHAN = 1
 QR_KAHAN_FP64 = 2
+FUSE_NONE, FUSE_34, FUSE_345 = 0, 1, 2   # bmfr_fuse_level
 
 # enum bmfr_buffer: id -> (name, element dtype key, elements per pixel kind)
 BUFFERS = {
@@ -178,8 +179,12 @@ class Stream:
         o = _f32(pixel_offset, 2)
         _check(self._lib.bmfr_run_frame(self._h, frame, _fp(m), _fp(o)))
 
-    def set_fused(self, enable=True):
-        _check(self._lib.bmfr_set_fused(self._h, 1 if enable else 0))
+    def set_fused(self, level=True):
+        """How run_frame runs stages 3-5: FUSE_NONE (False), FUSE_34, or
+        FUSE_345 (True, the default)."""
+        if isinstance(level, bool):
+            level = FUSE_345 if level else FUSE_NONE
+        _check(self._lib.bmfr_set_fused(self._h, int(level)))
 
     def set_fitter_config(self, rows_per_lane, blocks_per_group):
         _check(self._lib.bmfr_set_fitter_config(self._h, rows_per_lane, blocks_per_group))
@@ -292,6 +297,9 @@ class Batch:
     def set_graph(self, enable=True):
         _check(self._lib.bmfr_batch_set_graph(self._h, 1 if enable else 0))
 
+    def set_fused(self, level=2):
+        _check(self._lib.bmfr_batch_set_fused(self._h, int(level)))
+
     def set_fitter_config(self, rows_per_lane, blocks_per_group, grid_cap=0):
         _check(self._lib.bmfr_batch_set_fitter_config(self._h, rows_per_lane, blocks_per_group,
                                                       grid_cap))
@@ -377,6 +385,9 @@ class Server:
 
     def synchronize(self):
         _check(self._lib.bmfr_server_synchronize(self._h))
+
+    def set_fused(self, level=2):
+        _check(self._lib.bmfr_server_set_fused(self._h, int(level)))
 
 
 class Scene:

@@ -89,6 +89,7 @@ SIGNATURES = [
                                  c_int32]),
     ("bmfr_batch_synchronize", c_int32, [ctypes.c_void_p]),
     ("bmfr_batch_set_graph", c_int32, [ctypes.c_void_p, c_int32]),
+    ("bmfr_batch_set_fused", c_int32, [ctypes.c_void_p, c_int32]),
     ("bmfr_batch_set_fitter_config", c_int32, [ctypes.c_void_p, c_int32, c_int32, c_int32]),
     ("bmfr_batch_stage_times", c_int32, [ctypes.c_void_p, c_float_p]),
     ("bmfr_batch_autotune", c_int32, [ctypes.c_void_p, c_int32, ctypes.c_char_p, ctypes.c_size_t]),
@@ -107,6 +108,7 @@ SIGNATURES = [
     ("bmfr_server_step_query", c_int32, [ctypes.c_void_p, c_int32, ctypes.POINTER(c_int32),
                                          c_float_p]),
     ("bmfr_server_synchronize", c_int32, [ctypes.c_void_p]),
+    ("bmfr_server_set_fused", c_int32, [ctypes.c_void_p, c_int32]),
     ("bmfr_scene_create", c_int32, [ctypes.c_uint64, c_int32, ctypes.c_float, c_int32, c_int32,
                                     c_int32, ctypes.POINTER(ctypes.c_void_p)]),
     ("bmfr_scene_destroy", c_int32, [ctypes.c_void_p]),

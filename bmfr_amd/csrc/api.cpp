@@ -150,7 +150,7 @@ struct bmfr_stream_s {
     hipEvent_t sync_ev = nullptr;   // orders this stream against batch launches
     bool recorded[kStages]{};
     bmfr::FitterConfig fitter{16, 4};  // one wave per block, four blocks per work-group
-    bool fuse = true;       // bmfr_run_frame runs stages 3+4 as one kernel
+    int fuse = BMFR_FUSE_34;   // how bmfr_run_frame runs stages 3-5 (bmfr_fuse_level)
 
     int cur(int a_slot) const { return swapped ? a_slot : a_slot + 1; }
     int prev(int a_slot) const { return swapped ? a_slot + 1 : a_slot; }
@@ -553,24 +553,36 @@ int32_t bmfr_end_frame(bmfr_stream s) {
 int32_t bmfr_run_frame(bmfr_stream s, int32_t frame, const float *matrix, const float *pixel_offset) {
     int32_t st = bmfr_accumulate_noisy_data(s, frame, matrix, pixel_offset);
     if (!st) st = bmfr_fitter(s, frame);
-    if (!st && !s->fuse) st = bmfr_weighted_sum(s, frame);
-    if (!st && !s->fuse) st = bmfr_accumulate_filtered_data(s, frame);
-    if (!st && s->fuse) {
-        // weighted_sum fused into accumulate_filtered_data: stage 2's events
-        // bracket nothing (0 ms), stage 3's bracket the fused kernel
+    if (st) return st;
+    if (s->fuse == BMFR_FUSE_NONE) {
+        st = bmfr_weighted_sum(s, frame);
+        if (!st) st = bmfr_accumulate_filtered_data(s, frame);
+        if (!st) st = bmfr_taa(s, frame);
+    } else {
+        // fused stages: the events of a stage folded into a later kernel
+        // bracket nothing (0 ms); stage 4's (accumulate_filtered_data) bracket
+        // the fused kernel
         const bmfr::StageArgs a = s->args(frame);
         const bmfr::StageBuffers b = s->buffers();
         st = timed_stage(s, 2, [&] { return hipSuccess; });
-        if (!st) st = timed_stage(s, 3, [&] { return bmfr::launch_weighted_sum_filtered(a, b, s->stream); });
+        if (s->fuse == BMFR_FUSE_34) {
+            if (!st) st = timed_stage(s, 3, [&] { return bmfr::launch_weighted_sum_filtered(a, b, s->stream); });
+            if (!st) st = bmfr_taa(s, frame);
+        } else {
+            if (!st) st = timed_stage(s, 3, [&] { return bmfr::launch_filter_taa(a, b, s->stream); });
+            if (!st) st = timed_stage(s, 4, [&] { return hipSuccess; });
+            if (!st) s->last_result = s->cur(S_RESULT_A);
+        }
     }
-    if (!st) st = bmfr_taa(s, frame);
     if (!st) st = bmfr_end_frame(s);
     return st;
 }
 
-int32_t bmfr_set_fused(bmfr_stream s, int32_t enable) {
+int32_t bmfr_set_fused(bmfr_stream s, int32_t level) {
     if (int32_t st = check_stream(s)) return st;
-    s->fuse = enable != 0;
+    if (level < BMFR_FUSE_NONE || level > BMFR_FUSE_345)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "fuse level %d is not one of 0, 1, 2", level);
+    s->fuse = level;
     return BMFR_OK;
 }
 
@@ -720,6 +732,7 @@ struct bmfr_batch_s {
     bool recorded_total_only = false;
     bmfr::FitterConfig fitter{16, 4};
     int grid_cap = 0;
+    int fuse = BMFR_FUSE_345;
     bool use_graph = false;
     hipGraphExec_t graph = nullptr;
     // pinned staging of the per-step parameters, one slot per group and ring entry
@@ -750,11 +763,15 @@ int32_t batch_enqueue_stages(bmfr_batch b, bool timed) {
                 case 2:  // weighted_sum fused into stage 4 (0 ms)
                     break;
                 case 3:
-                    e = bmfr::launch_batch_weighted_sum_filtered(L, g.starts[bmfr::kStartsPadded][L.n],
-                                                                 g.g16, b->stream);
+                    if (b->fuse == BMFR_FUSE_345)
+                        e = bmfr::launch_batch_filter_taa(L, g.starts[bmfr::kStartsBlocks][L.n], g.g16, b->stream);
+                    else
+                        e = bmfr::launch_batch_weighted_sum_filtered(L, g.starts[bmfr::kStartsPadded][L.n],
+                                                                     g.g16, b->stream);
                     break;
-                default:
-                    e = bmfr::launch_batch_taa(L, g.starts[bmfr::kStartsImage][L.n], g.g16, b->stream);
+                default:  // taa, unless fused into stage 4 (0 ms)
+                    if (b->fuse != BMFR_FUSE_345)
+                        e = bmfr::launch_batch_taa(L, g.starts[bmfr::kStartsImage][L.n], g.g16, b->stream);
                     break;
             }
             if (e != hipSuccess) return fail(BMFR_ERR_HIP, "batched stage %d launch failed: %s", st, hipGetErrorString(e));
@@ -882,6 +899,18 @@ int32_t bmfr_batch_destroy(bmfr_batch b) {
     if (b->h_steps) (void)hipHostFree(b->h_steps);
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
+    return BMFR_OK;
+}
+
+int32_t bmfr_batch_set_fused(bmfr_batch b, int32_t level) {
+    if (!b) return fail(BMFR_ERR_INVALID_ARGUMENT, "batch is null");
+    if (level != BMFR_FUSE_34 && level != BMFR_FUSE_345)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "batches run fuse level 1 or 2, not %d", level);
+    b->fuse = level;
+    if (b->graph) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
     return BMFR_OK;
 }
 
@@ -1110,6 +1139,7 @@ struct bmfr_server_s {
     int64_t steps_launched = 0;
     bmfr::FitterConfig fitter{16, 4};
     int grid_cap = 512;   // work-groups of 4 waves: 2 waves per SIMD on 256 CUs
+    int fuse = BMFR_FUSE_345;
 };
 
 extern "C" {
@@ -1291,8 +1321,17 @@ int32_t bmfr_server_step(bmfr_server sv, int32_t count, const bmfr_stream *strea
                 case 0: e = bmfr::launch_batch_accumulate_noisy_data(L, sts[0][L.n], g16, t16, sv->stream); break;
                 case 1: e = bmfr::launch_batch_fitter(L, sts[2][L.n], t16, sv->fitter, sv->grid_cap, qr, sv->stream); break;
                 case 2: e = hipSuccess; break;  // fused into stage 4
-                case 3: e = bmfr::launch_batch_weighted_sum_filtered(L, sts[bmfr::kStartsPadded][L.n], g16, sv->stream); break;
-                default: e = bmfr::launch_batch_taa(L, sts[bmfr::kStartsImage][L.n], g16, sv->stream); break;
+                case 3:
+                    e = sv->fuse == BMFR_FUSE_345
+                            ? bmfr::launch_batch_filter_taa(L, sts[bmfr::kStartsBlocks][L.n], g16, sv->stream)
+                            : bmfr::launch_batch_weighted_sum_filtered(L, sts[bmfr::kStartsPadded][L.n], g16,
+                                                                       sv->stream);
+                    break;
+                default:
+                    e = sv->fuse == BMFR_FUSE_345
+                            ? hipSuccess
+                            : bmfr::launch_batch_taa(L, sts[bmfr::kStartsImage][L.n], g16, sv->stream);
+                    break;
             }
             if (e != hipSuccess) return fail(BMFR_ERR_HIP, "server stage %d launch failed: %s", st, hipGetErrorString(e));
         }
@@ -1325,6 +1364,14 @@ int32_t bmfr_server_step_query(bmfr_server sv, int32_t step_id, int32_t *done, f
     if (q != hipSuccess) return fail(BMFR_ERR_HIP, "step query failed: %s", hipGetErrorString(q));
     *done = 1;
     if (ms) HIP_TRY(hipEventElapsedTime(ms, sv->step_start[step_id], sv->step_end[step_id]));
+    return BMFR_OK;
+}
+
+int32_t bmfr_server_set_fused(bmfr_server sv, int32_t level) {
+    if (!sv) return fail(BMFR_ERR_INVALID_ARGUMENT, "server is null");
+    if (level != BMFR_FUSE_34 && level != BMFR_FUSE_345)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "server steps run fuse level 1 or 2, not %d", level);
+    sv->fuse = level;
     return BMFR_OK;
 }
 

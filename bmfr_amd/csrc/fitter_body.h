@@ -21,6 +21,7 @@
 // so skipping them leaves mins_maxs and weights unchanged.  The factorised
 // matrix is not written back to tmp_data: nothing reads it after the fitter
 // (opencl/bmfr.cpp:449-476).
+
 #include "device_common.h"
 #include "kernels.h"
 
@@ -184,6 +185,14 @@ struct BlockFlag {
     int32_t *flag;
     float threshold;
 };
+
+// Correctly rounded a / d from y = RN(1/d): q = RN(a*y), r = RN(a - q*d)
+// (exact by FMA), RN(q + r*y) (Markstein's corrected quotient).
+__device__ __forceinline__ float div_by(float a, float d, float y) {
+    const float q = a * y;
+    const float r = __builtin_fmaf(-q, d, a);
+    return __builtin_fmaf(r, y, q);
+}
 
 // Back substitution, bmfr.cl:658-699, by one wave.  Lane L = 3*c + ch holds
 // column c (0..10) of R for colour channel ch: R[r][c] = elem(c, r) (the
@@ -601,12 +610,15 @@ __device__ __forceinline__ void fast_pivot(FastRows<RPL, TT, HALF_ROUND> &m, flo
     for (int i = 0; i < NV; ++i) e[1 + i] = m.pivot_row(COL + 1 + i, COL);
     fast_exchange<L::W, OpAdd>(v, e, &red[(COL + 1) & 1][sub][0][0], wave, lane);
 
+    // v_sqrt_f32 and v_rcp_f32 (1 ulp) for the pivot's norm and 1/uu: the
+    // reference's sqrt and per-element division are not correctly rounded
+    // either, and the trailing update is checked by tolerance
     const float sigma = v[0], xc = e[0];
-    const float norm = sqrtf(sigma + xc * xc);
+    const float norm = __builtin_amdgcn_sqrtf(sigma + xc * xc);
     const float uc = xc - norm;
     const float uu = sigma + uc * uc;
     norms[COL] = norm;
-    const float inv_uu = 1.f / uu;
+    const float inv_uu = __builtin_amdgcn_rcpf(uu);
     const float u_kc = t == COL / V ? uc : x[KC];
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
@@ -683,22 +695,16 @@ __device__ __forceinline__ void fitter_block_fast(const TT *__restrict__ src, fl
     // scale (bmfr.cl:200-205).  (v-min)/(max-min) is evaluated as Markstein's
     // correctly rounded quotient: y = RN(1/d) once per feature, then
     // q = RN(a*y), r = RN(a - q*d) (exact), q' = RN(q + r*y) = RN(a/d).
+    // Without the division (|max - min| <= 1) the same three operations with
+    // d = y = 1 return the numerator exactly.
 #pragma unroll
     for (int i = 0; i < kScaled; ++i) {
-        const float d = bmax[i] - bmin[i];
-        const bool divide = fabsf(d) > 1.0f;
-        const float y = 1.0f / d;
+        const float dd = bmax[i] - bmin[i];
+        const bool divide = fabsf(dd) > 1.0f;
+        const float d = divide ? dd : 1.f;
+        const float y = divide ? 1.0f / dd : 1.f;
 #pragma unroll
-        for (int k = 0; k < RPL; ++k) {
-            const float num = m.a[kNotScaled + i][k] - bmin[i];
-            float sc = num;
-            if (divide) {
-                const float q = num * y;
-                const float r = __builtin_fmaf(-q, d, num);
-                sc = __builtin_fmaf(r, y, q);
-            }
-            m.put(kNotScaled + i, k, sc);
-        }
+        for (int k = 0; k < RPL; ++k) m.put(kNotScaled + i, k, div_by(m.a[kNotScaled + i][k] - bmin[i], d, y));
     }
     // ---- feature noise of the first pass (bmfr.cl:623-627): the value plus
     // the FP64 noise term, rounded once to float (NOISE_AMOUNT is a double

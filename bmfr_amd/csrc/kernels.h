@@ -88,6 +88,8 @@ hipError_t launch_batch_weighted_sum(const BatchLaunch &L, int total, bool g16, 
 hipError_t launch_batch_accumulate_filtered_data(const BatchLaunch &L, int total, bool g16,
                                                  hipStream_t s);
 hipError_t launch_batch_taa(const BatchLaunch &L, int total, bool g16, hipStream_t s);
+// stages 3+4+5 fused, one work-group per 32x32 block (total = block count)
+hipError_t launch_batch_filter_taa(const BatchLaunch &L, int total, bool g16, hipStream_t s);
 
 // Up to four independent byte copies in one launch (16-byte aligned bases).
 struct CopyJob {
@@ -106,6 +108,7 @@ hipError_t launch_fitter(const StageArgs &a, const StageBuffers &b, const Fitter
 hipError_t launch_weighted_sum(const StageArgs &a, const StageBuffers &b, hipStream_t s);
 hipError_t launch_accumulate_filtered_data(const StageArgs &a, const StageBuffers &b, hipStream_t s);
 hipError_t launch_taa(const StageArgs &a, const StageBuffers &b, hipStream_t s);
+hipError_t launch_filter_taa(const StageArgs &a, const StageBuffers &b, hipStream_t s);
 // Stages 3 and 4 in one kernel (the filtered colour is not written).
 hipError_t launch_weighted_sum_filtered(const StageArgs &a, const StageBuffers &b, hipStream_t s);
 hipError_t launch_batch_weighted_sum_filtered(const BatchLaunch &L, int total, bool g16, hipStream_t s);

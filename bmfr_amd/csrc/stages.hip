@@ -240,14 +240,10 @@ __device__ __forceinline__ void block_tile(const StageArgs &a, int local, int &x
 // Filtered colour of in-image pixel lp in block g: sum of block weights
 // times the pixel's (scaled) features, negatives clamped (bmfr.cl:712-750).
 template <typename GT>
-__device__ __forceinline__ float3 stage3_color(const StageArgs &a, const StageBuffers &bufs, int64_t lp,
-                                              int64_t g) {
+__device__ __forceinline__ float3 stage3_color_of(const StageArgs &a, const StageBuffers &bufs, float3 wp, float3 n,
+                                                 int64_t g) {
     const float *__restrict__ weights = (const float *)bufs.weights;
     const float *__restrict__ mins_maxs = (const float *)bufs.mins_maxs;
-    const GT *__restrict__ cur_normals = (const GT *)bufs.normals_cur;
-    const GT *__restrict__ cur_positions = (const GT *)bufs.positions_cur;
-    const float3 wp = load3(cur_positions, lp);
-    const float3 n = load3(cur_normals, lp);
     const float feat[kFeatures] = {1.f,  n.x,  n.y,         n.z,         wp.x,
                                    wp.y, wp.z, wp.x * wp.x, wp.y * wp.y, wp.z * wp.z};
     const float *w = weights + g * (kFeatures * 3);
@@ -262,6 +258,13 @@ __device__ __forceinline__ float3 stage3_color(const StageArgs &a, const StageBu
         cz = cz + w[f * 3 + 2] * v;
     }
     return make_float3(cx < 0.f ? 0.f : cx, cy < 0.f ? 0.f : cy, cz < 0.f ? 0.f : cz);
+}
+
+template <typename GT>
+__device__ __forceinline__ float3 stage3_color(const StageArgs &a, const StageBuffers &bufs, int64_t lp,
+                                              int64_t g) {
+    return stage3_color_of<GT>(a, bufs, load3((const GT *)bufs.positions_cur, lp),
+                               load3((const GT *)bufs.normals_cur, lp), g);
 }
 
 template <typename GT>
@@ -296,6 +299,7 @@ struct Stage4Gather {
     float wts[4];
     uint32_t use;    // bit i: tap i contributes
     uint32_t spp;
+    float2 pf;       // prev_pixel (stage 5 reads it too)
 };
 
 template <typename GT>
@@ -309,6 +313,7 @@ __device__ __forceinline__ Stage4Gather<GT> stage4_gather(const StageArgs &a, co
     Stage4Gather<GT> g;
     const uint32_t acc = a.frame > 0 ? (uint32_t)accept_bools[lp] : 0u;
     const float2 pf = prev_pixel[lp];
+    g.pf = pf;
     g.spp = cur_spp[lp];
     g.albedo = make_float3(albedo[lp * 3 + 0], albedo[lp * 3 + 1], albedo[lp * 3 + 2]);
     const int ix = cvt_rtn(pf.x), iy = cvt_rtn(pf.y);
@@ -332,11 +337,10 @@ __device__ __forceinline__ Stage4Gather<GT> stage4_gather(const StageArgs &a, co
     return g;
 }
 
+// The accumulated colour of a pixel from its filtered colour `fc` and its
+// gathered history (bmfr.cl:801-849).
 template <typename GT>
-__device__ __forceinline__ void stage4_finish(const StageArgs &a, const StageBuffers &bufs, int64_t lp, float3 fc,
-                                              const Stage4Gather<GT> &g) {
-    float *__restrict__ tone_mapped = (float *)bufs.tone_mapped;
-    GT *__restrict__ acc_out = (GT *)bufs.out_cur;
+__device__ __forceinline__ float3 stage4_blend(const StageArgs &a, float3 fc, const Stage4Gather<GT> &g) {
     float3 pc = make_float3(0.f, 0.f, 0.f);
     float blend_alpha = 1.f;
     if (g.use) {
@@ -363,11 +367,24 @@ __device__ __forceinline__ void stage4_finish(const StageArgs &a, const StageBuf
     ac.x = blend_alpha * fc.x + omb * pc.x;
     ac.y = blend_alpha * fc.y + omb * pc.y;
     ac.z = blend_alpha * fc.z + omb * pc.z;
-    store3(acc_out, lp, ac);
-    // clamp(pow(max(0, albedo * colour), 0.454545f), 0, 1), bmfr.cl:851-856
-    tone_mapped[lp * 3 + 0] = tone_curve(g.albedo.x * ac.x);
-    tone_mapped[lp * 3 + 1] = tone_curve(g.albedo.y * ac.y);
-    tone_mapped[lp * 3 + 2] = tone_curve(g.albedo.z * ac.z);
+    return ac;
+}
+
+// clamp(pow(max(0, albedo * colour), 0.454545f), 0, 1), bmfr.cl:851-856
+__device__ __forceinline__ float3 tone_map(float3 albedo, float3 ac) {
+    return make_float3(tone_curve(albedo.x * ac.x), tone_curve(albedo.y * ac.y), tone_curve(albedo.z * ac.z));
+}
+
+template <typename GT>
+__device__ __forceinline__ void stage4_finish(const StageArgs &a, const StageBuffers &bufs, int64_t lp, float3 fc,
+                                              const Stage4Gather<GT> &g) {
+    float *__restrict__ tone_mapped = (float *)bufs.tone_mapped;
+    const float3 ac = stage4_blend<GT>(a, fc, g);
+    store3((GT *)bufs.out_cur, lp, ac);
+    const float3 t = tone_map(g.albedo, ac);
+    tone_mapped[lp * 3 + 0] = t.x;
+    tone_mapped[lp * 3 + 1] = t.y;
+    tone_mapped[lp * 3 + 2] = t.z;
 }
 
 template <typename GT>
@@ -427,57 +444,51 @@ __device__ __forceinline__ float3 ycocg_to_rgb(float3 c) {
 // in the image.  Min and max are exact, so their order does not matter.
 constexpr int kTaaTileW = 66, kTaaTileH = 6;
 
+// The bilinear previous result at prev_pixel (bmfr.cl:879-890, 922-928):
+// its four taps are issued as soon as prev_pixel is known; a tap that the
+// tests of bmfr.cl:929-962 rule out reads the pixel's own texel and is never
+// used.
 template <typename GT>
-__device__ __forceinline__ void stage5_body(const StageArgs &a, const StageBuffers &bufs, int x, int y) {
-    __shared__ float4 tile[kTaaTileH][kTaaTileW];
-    const float2 *__restrict__ prev_pixel = (const float2 *)bufs.prev_pixel;
-    const float *__restrict__ new_frame = (const float *)bufs.tone_mapped;
-    GT *__restrict__ result = (GT *)bufs.result_cur;
+struct TaaHistory {
+    float3 tap[4];
+    bool use[4];
+    bool history;   // frame > 0 and floor(prev_pixel) in [-1, W) x [-1, H)
+    int ix, iy;
+};
+
+template <typename GT>
+__device__ __forceinline__ TaaHistory<GT> taa_history(const StageArgs &a, const StageBuffers &bufs, int64_t lp,
+                                                      float2 pf) {
     const GT *__restrict__ prev_result = (const GT *)bufs.result_prev;
     const int W = a.width, H = a.height;
-    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-    const int x0 = x - tx, y0 = y - ty;
-    // the pixel's own loads are issued before the tile's, so that they are
-    // in flight during the tile load and the barrier
-    const bool inside = x < W && y < H;
-    const int64_t lp = inside ? (int64_t)y * W + x : 0;
-    const float2 pf = prev_pixel[lp];
-    const float3 mine = make_float3(new_frame[lp * 3 + 0], new_frame[lp * 3 + 1], new_frame[lp * 3 + 2]);
-    // bilinear history taps, issued as soon as prev_pixel is in (before the
-    // tile load and the barrier); a tap that the tests of bmfr.cl:922-962
-    // rule out reads the pixel's own texel and is never used
-    const int ix = cvt_rtn(pf.x), iy = cvt_rtn(pf.y);
-    const bool history = a.frame != 0 && ix >= -1 && iy >= -1 && ix < W && iy < H;
+    TaaHistory<GT> h;
+    h.ix = cvt_rtn(pf.x);
+    h.iy = cvt_rtn(pf.y);
+    const int ix = h.ix, iy = h.iy;
+    h.history = a.frame != 0 && ix >= -1 && iy >= -1 && ix < W && iy < H;
     const int64_t r0 = (int64_t)iy * W, r1 = ((int64_t)iy + 1) * W;  // 64-bit: iy may saturate
-    const bool use[4] = {history && iy >= 0 && ix >= 0, history && iy >= 0 && ix < W - 1,
-                         history && iy < H - 1 && ix >= 0, history && iy < H - 1 && ix < W - 1};
+    h.use[0] = h.history && iy >= 0 && ix >= 0;
+    h.use[1] = h.history && iy >= 0 && ix < W - 1;
+    h.use[2] = h.history && iy < H - 1 && ix >= 0;
+    h.use[3] = h.history && iy < H - 1 && ix < W - 1;
     const int64_t tap_ls[4] = {r0 + ix, r0 + ix + 1, r1 + ix, r1 + ix + 1};
-    float3 tap[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) tap[i] = load3(prev_result, use[i] ? tap_ls[i] : lp);
-    for (int e = threadIdx.x; e < kTaaTileW * kTaaTileH; e += 256) {
-        const int ry = e / kTaaTileW, rx = e - ry * kTaaTileW;
-        const int gx = x0 - 1 + rx, gy = y0 - 1 + ry;
-        float4 t = make_float4(__builtin_nanf(""), __builtin_nanf(""), __builtin_nanf(""), 0.f);
-        if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
-            const int64_t ls = (int64_t)gy * W + gx;
-            const float3 yc = rgb_to_ycocg(make_float3(new_frame[ls * 3 + 0], new_frame[ls * 3 + 1],
-                                                       new_frame[ls * 3 + 2]));
-            t = make_float4(yc.x, yc.y, yc.z, 0.f);
-        }
-        tile[ry][rx] = t;
-    }
-    __syncthreads();
-    if (!inside) return;
-    if (!history) {
-        store3(result, lp, mine);
-        return;
-    }
+    for (int i = 0; i < 4; ++i) h.tap[i] = load3(prev_result, h.use[i] ? tap_ls[i] : lp);
+    return h;
+}
+
+// TAA of one pixel (bmfr.cl:884-973): `mine` is its tone-mapped colour, `nb`
+// the top-left texel of its 3x3 YCoCg neighbourhood in an LDS tile of row
+// stride STRIDE (NaN outside the image).
+template <int STRIDE, typename GT>
+__device__ __forceinline__ float3 taa_resolve(const StageArgs &a, float2 pf, float3 mine, const TaaHistory<GT> &h,
+                                              const float4 *nbp) {
+    if (!h.history) return mine;
     float4 nb[3][3];
 #pragma unroll
     for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
-        for (int dx = 0; dx < 3; ++dx) nb[dy][dx] = tile[ty + dy][tx + dx];
+        for (int dx = 0; dx < 3; ++dx) nb[dy][dx] = nbp[dy * STRIDE + dx];
     // cross: centre and its 4-neighbours; box: all 9
     float3 mnc, mxc, mnb, mxb;
     mnc.x = fminf(fminf(fminf(nb[0][1].x, nb[1][0].x), fminf(nb[1][1].x, nb[1][2].x)), nb[2][1].x);
@@ -494,15 +505,15 @@ __device__ __forceinline__ void stage5_body(const StageArgs &a, const StageBuffe
     mxb.z = fmaxf(fmaxf(mxc.z, fmaxf(nb[0][0].z, nb[0][2].z)), fmaxf(nb[2][0].z, nb[2][2].z));
     float3 pc = make_float3(0.f, 0.f, 0.f);
     float tw = 0.f;
-    const float fx = pf.x - (float)ix, fy = pf.y - (float)iy;
+    const float fx = pf.x - (float)h.ix, fy = pf.y - (float)h.iy;
     const float omx = 1.f - fx, omy = 1.f - fy;
     const float wts[4] = {omx * omy, fx * omy, omx * fy, fx * fy};
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        if (use[i]) {
-            pc.x = pc.x + wts[i] * tap[i].x;
-            pc.y = pc.y + wts[i] * tap[i].y;
-            pc.z = pc.z + wts[i] * tap[i].z;
+        if (h.use[i]) {
+            pc.x = pc.x + wts[i] * h.tap[i].x;
+            pc.y = pc.y + wts[i] * h.tap[i].y;
+            pc.z = pc.z + wts[i] * h.tap[i].z;
             tw = tw + wts[i];
         }
     }
@@ -520,7 +531,123 @@ __device__ __forceinline__ void stage5_body(const StageArgs &a, const StageBuffe
     r.x = ta * mine.x + omt * prgb.x;
     r.y = ta * mine.y + omt * prgb.y;
     r.z = ta * mine.z + omt * prgb.z;
-    store3(result, lp, r);
+    return r;
+}
+
+__device__ __forceinline__ float4 ycocg_texel(float3 rgb) {
+    const float3 yc = rgb_to_ycocg(rgb);
+    return make_float4(yc.x, yc.y, yc.z, 0.f);
+}
+
+template <typename GT>
+__device__ __forceinline__ void stage5_body(const StageArgs &a, const StageBuffers &bufs, int x, int y) {
+    __shared__ float4 tile[kTaaTileH][kTaaTileW];
+    const float2 *__restrict__ prev_pixel = (const float2 *)bufs.prev_pixel;
+    const float *__restrict__ new_frame = (const float *)bufs.tone_mapped;
+    const int W = a.width, H = a.height;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int x0 = x - tx, y0 = y - ty;
+    // the pixel's own loads and its history taps are issued before the
+    // tile's, so that they are in flight during the tile load and the barrier
+    const bool inside = x < W && y < H;
+    const int64_t lp = inside ? (int64_t)y * W + x : 0;
+    const float2 pf = prev_pixel[lp];
+    const float3 mine = make_float3(new_frame[lp * 3 + 0], new_frame[lp * 3 + 1], new_frame[lp * 3 + 2]);
+    const TaaHistory<GT> h = taa_history<GT>(a, bufs, lp, pf);
+    for (int e = threadIdx.x; e < kTaaTileW * kTaaTileH; e += 256) {
+        const int ry = e / kTaaTileW, rx = e - ry * kTaaTileW;
+        const int gx = x0 - 1 + rx, gy = y0 - 1 + ry;
+        float4 t = make_float4(__builtin_nanf(""), __builtin_nanf(""), __builtin_nanf(""), 0.f);
+        if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
+            const int64_t ls = (int64_t)gy * W + gx;
+            t = ycocg_texel(make_float3(new_frame[ls * 3 + 0], new_frame[ls * 3 + 1], new_frame[ls * 3 + 2]));
+        }
+        tile[ry][rx] = t;
+    }
+    __syncthreads();
+    if (!inside) return;
+    store3((GT *)bufs.result_cur, lp, taa_resolve<kTaaTileW, GT>(a, pf, mine, h, &tile[ty][tx]));
+}
+
+// ---------------------------------------------------------------------------
+// Stages 3, 4 and 5 fused for the chained paths: one work-group per padded
+// 32x32 block (whose image pixels share one set of weights and bounds).
+// Phase 1 computes the filtered, accumulated and tone-mapped colour of the
+// block's image pixels, 4 per thread, stores the accumulated colour and puts
+// the tone-mapped colour, in YCoCg, into an LDS tile; it also recomputes the
+// one-pixel ring around the block (pixels of the neighbouring blocks, whose
+// weights and bounds are vector loads), which only goes into the tile.
+// Phase 2 runs TAA on the block's pixels from that tile.  Neither `filtered`
+// nor `tone_mapped` is written; each pixel's arithmetic is that of the
+// separate stages, so every result has the same bits.
+constexpr int kRing = kBlock + 2;                 // 34x34 tile with the ring
+constexpr int kRingTexels = 4 * kBlock + 4;       // 132 ring texels
+
+template <typename GT>
+__device__ __forceinline__ void stage345_body(const StageArgs &a, const StageBuffers &bufs, int blk) {
+    __shared__ float4 tile[kRing][kRing];
+    const int W = a.width, H = a.height;
+    const int2 off = block_offset(a.frame);
+    const int by = blk / a.blocks_x, bx = blk - by * a.blocks_x;
+    // image pixel of the block's padded-grid corner (bmfr.cl:707-710 inverted)
+    const int x0 = bx * kBlock - kBlock / 2 + off.x;
+    const int y0 = by * kBlock - kBlock / 2 + off.y;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const float4 nan4 = make_float4(__builtin_nanf(""), __builtin_nanf(""), __builtin_nanf(""), 0.f);
+    float3 mine[4];
+    float2 pf[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int x = x0 + tx, y = y0 + ty + 8 * r;
+        float4 t = nan4;
+        pf[r] = make_float2(0.f, 0.f);
+        mine[r] = make_float3(0.f, 0.f, 0.f);
+        if (x >= 0 && y >= 0 && x < W && y < H) {
+            const int64_t lp = (int64_t)y * W + x;
+            const Stage4Gather<GT> gth = stage4_gather<GT>(a, bufs, lp);
+            const float3 ac = stage4_blend<GT>(a, stage3_color<GT>(a, bufs, lp, blk), gth);
+            store3((GT *)bufs.out_cur, lp, ac);
+            mine[r] = tone_map(gth.albedo, ac);
+            pf[r] = gth.pf;
+            t = ycocg_texel(mine[r]);
+        }
+        tile[ty + 8 * r + 1][tx + 1] = t;
+    }
+    if (threadIdx.x < kRingTexels) {
+        const int h = threadIdx.x;
+        int rx, ry;   // ring position in -1..32
+        if (h < kRing) { rx = h - 1; ry = -1; }
+        else if (h < 2 * kRing) { rx = h - kRing - 1; ry = kBlock; }
+        else if (h < 2 * kRing + kBlock) { rx = -1; ry = h - 2 * kRing; }
+        else { rx = kBlock; ry = h - 2 * kRing - kBlock; }
+        const int x = x0 + rx, y = y0 + ry;
+        float4 t = nan4;
+        if (x >= 0 && y >= 0 && x < W && y < H) {
+            const int64_t lp = (int64_t)y * W + x;
+            // the pixel's own block (bmfr.cl:715-718)
+            const int64_t g = (int64_t)((y + kBlock / 2 - off.y) / kBlock) * a.blocks_x +
+                              (x + kBlock / 2 - off.x) / kBlock;
+            const Stage4Gather<GT> gth = stage4_gather<GT>(a, bufs, lp);
+            const float3 ac = stage4_blend<GT>(a, stage3_color<GT>(a, bufs, lp, g), gth);
+            t = ycocg_texel(tone_map(gth.albedo, ac));
+        }
+        tile[ry + 1][rx + 1] = t;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int x = x0 + tx, y = y0 + ty + 8 * r;
+        if (x >= 0 && y >= 0 && x < W && y < H) {
+            const int64_t lp = (int64_t)y * W + x;
+            const TaaHistory<GT> h = taa_history<GT>(a, bufs, lp, pf[r]);
+            store3((GT *)bufs.result_cur, lp, taa_resolve<kRing, GT>(a, pf[r], mine[r], h, &tile[ty + 8 * r][tx]));
+        }
+    }
+}
+
+template <typename GT>
+__global__ __launch_bounds__(256) void k_filter_taa(const StageArgs a, const StageBuffers b) {
+    stage345_body<GT>(a, b, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 template <typename GT>
@@ -595,14 +722,17 @@ __global__ __launch_bounds__(256) void kb_accumulate_noisy_data(const BatchLaunc
 template <int STAGE, typename GT>
 __global__ __launch_bounds__(256) void kb_image_stage(const BatchLaunch L) {
     constexpr bool kBlockTiles = STAGE == 3 || STAGE == 34;
-    const int32_t *starts = L.starts + (kBlockTiles ? kStartsPadded : kStartsImage) * (kMaxBatch + 1);
+    const int32_t *starts =
+        L.starts + (STAGE == 345 ? kStartsBlocks : kBlockTiles ? kStartsPadded : kStartsImage) * (kMaxBatch + 1);
     const int unit = xcd_tile(blockIdx.x, gridDim.x);
     const int s = find_stream(starts, L.n, unit);
     StageArgs a;
     StageBuffers b;
     resolve_batch(L, s, a, b);
     const int local = unit - starts[s];
-    if constexpr (kBlockTiles) {
+    if constexpr (STAGE == 345) {
+        stage345_body<GT>(a, b, local);
+    } else if constexpr (kBlockTiles) {
         if constexpr (STAGE == 3) stage3_body<GT>(a, b, local);
         else stage34_body<GT>(a, b, local);
     } else {
@@ -675,6 +805,14 @@ hipError_t launch_weighted_sum_filtered(const StageArgs &a, const StageBuffers &
     return hipGetLastError();
 }
 
+hipError_t launch_filter_taa(const StageArgs &a, const StageBuffers &b, hipStream_t s) {
+    if (a.storage_f16)
+        hipLaunchKernelGGL((k_filter_taa<__half>), dim3(a.blocks), dim3(256), 0, s, a, b);
+    else
+        hipLaunchKernelGGL((k_filter_taa<float>), dim3(a.blocks), dim3(256), 0, s, a, b);
+    return hipGetLastError();
+}
+
 hipError_t launch_taa(const StageArgs &a, const StageBuffers &b, hipStream_t s) {
     if (a.storage_f16)
         hipLaunchKernelGGL((k_taa<__half>), image_grid(a), dim3(256), 0, s, a, b);
@@ -716,6 +854,9 @@ hipError_t launch_batch_weighted_sum_filtered(const BatchLaunch &L, int total, b
 }
 hipError_t launch_batch_taa(const BatchLaunch &L, int total, bool g16, hipStream_t s) {
     return launch_batch_image<5>(L, total, g16, s);
+}
+hipError_t launch_batch_filter_taa(const BatchLaunch &L, int total, bool g16, hipStream_t s) {
+    return launch_batch_image<345>(L, total, g16, s);
 }
 
 }  // namespace bmfr

@@ -170,12 +170,22 @@ int32_t bmfr_accumulate_filtered_data(bmfr_stream stream, int32_t frame_number);
 int32_t bmfr_taa(bmfr_stream stream, int32_t frame_number);
 /* Swaps the double buffers (opencl/bmfr.cpp:482-484). */
 int32_t bmfr_end_frame(bmfr_stream stream);
-/* All five stages in order, then bmfr_end_frame.  By default stages 3 and 4
- * run as one kernel here (same arithmetic; the intermediate filtered buffer
- * is then not written, and bmfr_stage_times reports 0 ms for stage 3 with
- * the fused time under stage 4); bmfr_set_fused(stream, 0) runs them
- * separately.  Batches and server steps always fuse them. */
-int32_t bmfr_set_fused(bmfr_stream stream, int32_t enable);
+/* How bmfr_run_frame, batches and server steps run stages 3-5.  Every level
+ * gives the same bits; a fused kernel does not write the intermediates it
+ * keeps on chip (filtered, and for BMFR_FUSE_345 also tone_mapped), and
+ * bmfr_stage_times reports 0 ms for the stages folded into it, with the
+ * fused kernel's time under stage 4 (accumulate_filtered_data).  The five
+ * separate entry points always run the five separate kernels. */
+typedef enum bmfr_fuse_level {
+    BMFR_FUSE_NONE = 0,  /* weighted_sum, accumulate_filtered_data, taa */
+    BMFR_FUSE_34 = 1,    /* weighted_sum inside accumulate_filtered_data, then taa
+                            (default of a stream) */
+    BMFR_FUSE_345 = 2    /* all three in one kernel per 32x32 block, with the
+                            one-pixel TAA ring recomputed (default of batches
+                            and server steps) */
+} bmfr_fuse_level;
+int32_t bmfr_set_fused(bmfr_stream stream, int32_t level);
+/* All five stages in order, then bmfr_end_frame (opencl/bmfr.cpp:429-484). */
 int32_t bmfr_run_frame(bmfr_stream stream, int32_t frame_number,
                        const float *prev_frame_camera_matrix, const float *pixel_offset);
 
@@ -230,6 +240,8 @@ int32_t bmfr_batch_run(bmfr_batch batch, const int32_t *frames, const float *mat
 int32_t bmfr_batch_synchronize(bmfr_batch batch);
 /* hipGraph capture of the five batched launches, replayed each step. */
 int32_t bmfr_batch_set_graph(bmfr_batch batch, int32_t enable);
+/* BMFR_FUSE_34 or BMFR_FUSE_345 (default) for the batch's stages 3-5. */
+int32_t bmfr_batch_set_fused(bmfr_batch batch, int32_t level);
 /* Fitter shape for the batch; grid_cap > 0 bounds the persistent grid. */
 int32_t bmfr_batch_set_fitter_config(bmfr_batch batch, int32_t rows_per_lane,
                                      int32_t blocks_per_group, int32_t grid_cap);
@@ -264,6 +276,8 @@ int32_t bmfr_server_step(bmfr_server server, int32_t count, const bmfr_stream *s
                          int32_t *step_id);
 int32_t bmfr_server_step_query(bmfr_server server, int32_t step_id, int32_t *done, float *gpu_ms);
 int32_t bmfr_server_synchronize(bmfr_server server);
+/* BMFR_FUSE_34 or BMFR_FUSE_345 (default) for later steps' stages 3-5. */
+int32_t bmfr_server_set_fused(bmfr_server server, int32_t level);
 
 /* ---- procedural scene (replaces the EXR dataset, opencl/bmfr.cpp:145-172) ---- */
 typedef struct bmfr_scene_s *bmfr_scene;

@@ -27,14 +27,21 @@ def _scenes(mix):
     return [scene(w, h, seed=0xB3F40000 + 3 + k) for (w, h, _, _, k) in mix]
 
 
-def _buffers(st):
-    return {n: st.read_buffer(n) for n in ("result_prev", "out_prev", "noisy_acc_prev", "spp_prev",
-                                            "weights", "mins_maxs", "prev_pixel", "accept", "tone_mapped")}
+def _buffers(st, fuse):
+    names = ["result_prev", "out_prev", "noisy_acc_prev", "spp_prev", "weights", "mins_maxs",
+             "prev_pixel", "accept"]
+    if fuse == bmfr_amd.FUSE_34:  # the 3+4+5 kernel keeps tone_mapped on chip
+        names.append("tone_mapped")
+    return {n: st.read_buffer(n) for n in names}
 
 
-def _run(mix, frames, batched, graph=False, eager_between=False):
+def _run(mix, frames, batched, graph=False, eager_between=False, fuse=bmfr_amd.FUSE_345):
     sts, scs = _streams(mix), _scenes(mix)
+    for st in sts:
+        st.set_fused(fuse)
     b = bmfr_amd.Batch(sts) if batched else None
+    if b:
+        b.set_fused(fuse)
     if graph:
         b.set_graph(True)
     try:
@@ -56,7 +63,7 @@ def _run(mix, frames, batched, graph=False, eager_between=False):
         out = []
         for st in sts:
             st.synchronize()
-            d = _buffers(st)
+            d = _buffers(st, fuse)
             d["download"] = st.download_result()
             out.append(d)
         return out
@@ -72,9 +79,10 @@ def _same(a, b):
         np.nan_to_num(a), np.nan_to_num(b))
 
 
-def test_batch_equals_each_stream_alone(gpu):
-    alone = _run(MIX, 5, batched=False)
-    mixed = _run(MIX, 5, batched=True)
+@pytest.mark.parametrize("fuse", [bmfr_amd.FUSE_34, bmfr_amd.FUSE_345])
+def test_batch_equals_each_stream_alone(gpu, fuse):
+    alone = _run(MIX, 5, batched=False, fuse=fuse)
+    mixed = _run(MIX, 5, batched=True, fuse=fuse)
     for k, (a, m) in enumerate(zip(alone, mixed)):
         for name in a:
             assert _same(a[name], m[name]), (k, name)

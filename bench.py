@@ -219,31 +219,59 @@ SERVE_RES = [(640, 360), (1280, 720), (1920, 1080), (2560, 1440)]
 SERVE_P = [0.2, 0.5, 0.25, 0.05]
 
 
+def poisson_arrivals(rng, t_open, t_close, rate):
+    arr, t = [], t_open
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t >= t_close:
+            return arr
+        arr.append(t)
+
+
 def run_served(duration, budget_gib, seed=0xB3F40000 + 4):
     """64 render streams opened and closed over `duration` seconds of wall
     time; per stream a resolution from SERVE_RES with probabilities SERVE_P,
-    Poisson frame arrivals at 30 or 60 Hz and FP16 or FP32 storage.  Streams
-    are admitted by the server's HBM budget (a FIFO waits for memory), and
-    every step batches the oldest pending frame of each stream with work
-    (continuous batching, at most two steps in flight).  Latency = host time
-    at which a frame's step is seen complete minus the frame's arrival."""
-    import bmfr_amd
+    Poisson frame arrivals at 30 or 60 Hz and FP16 or FP32 storage."""
     rng = np.random.default_rng(seed)
-    n = 64
     specs = []
-    for k in range(n):
+    for k in range(64):
         w, h = SERVE_RES[rng.choice(4, p=SERVE_P)]
         t_open = rng.uniform(0.0, 0.8 * duration)
         t_close = min(duration, t_open + rng.uniform(0.15, 0.4) * duration)
         rate = 30.0 if rng.random() < 0.5 else 60.0
         f16 = bool(rng.random() < 0.5)
-        arr, t = [], t_open
-        while True:
-            t += rng.exponential(1.0 / rate)
-            if t >= t_close:
-                break
-            arr.append(t)
-        specs.append(dict(w=w, h=h, open=t_open, close=t_close, rate=rate, f16=f16, arrivals=arr))
+        specs.append(dict(w=w, h=h, open=t_open, close=t_close, rate=rate, f16=f16,
+                          arrivals=poisson_arrivals(rng, t_open, t_close, rate)))
+    return serve(specs, duration, budget_gib, seed)
+
+
+def run_serve_sweep(counts, duration, seed=0xB3F40000 + 5):
+    """p50/p99 frame latency against the number of concurrent streams: N
+    streams of 1280x720 (FP32 storage), all open for `duration` seconds, each
+    with Poisson arrivals at 60 Hz; latency of the frames arriving after the
+    first second (all N streams are admitted and allocated by then)."""
+    out = []
+    for n in counts:
+        rng = np.random.default_rng(seed + n)
+        specs = [dict(w=1280, h=720, open=0.0, close=duration, rate=60.0, f16=False,
+                      arrivals=poisson_arrivals(rng, 0.0, duration, 60.0)) for _ in range(n)]
+        r = serve(specs, duration, 64.0, seed + n, settle=1.0)
+        out.append({k: r[k] for k in ("streams", "admitted", "frames", "aggregate_frames_per_s", "mpixel_per_s",
+                                      "latency_ms_p50", "latency_ms_p99", "mean_streams_per_step",
+                                      "step_gpu_ms_mean")})
+        progress(f"serve sweep: {n} streams, p99 {r['latency_ms_p99']} ms, "
+                 f"{r['mean_streams_per_step']} streams/step")
+    return out
+
+
+def serve(specs, duration, budget_gib, seed, settle=0.0):
+    """Streams are admitted by the server's HBM budget (a FIFO waits for
+    memory), and every step batches the oldest pending frame of each stream
+    with work (continuous batching, at most two steps in flight).  Latency =
+    host time at which a frame's step is seen complete minus the frame's
+    arrival, for frames arriving at or after `settle` seconds."""
+    import bmfr_amd
+    n = len(specs)
     sv = bmfr_amd.Server(int(budget_gib * (1 << 30)))
     state = [dict(stream=None, scene=None, next=0, pending=[], frame=0, done=False) for _ in specs]
     waiting, admitted, waited = [], 0, 0
@@ -298,7 +326,8 @@ def run_served(duration, budget_gib, seed=0xB3F40000 + 4):
                 tnow = time.perf_counter() - t0
                 step_ms.append(ms)
                 for (k, t_arr, px) in members:
-                    latencies.append(tnow - t_arr)
+                    if t_arr >= settle:
+                        latencies.append(tnow - t_arr)
                     frames_done += 1
                     pixels_done += px
             else:
@@ -430,6 +459,9 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=30.0)
     ap.add_argument("--serve-budget-gib", type=float, default=4.0)
     ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--serve-sweep", default="1,16,64,128",
+                    help="stream counts of the latency-vs-streams sweep (empty: skip)")
+    ap.add_argument("--serve-sweep-seconds", type=float, default=5.0)
     ap.add_argument("--precision-frames", type=int, default=60)
     ap.add_argument("--no-precision", action="store_true")
     args = ap.parse_args()
@@ -463,6 +495,9 @@ def main():
     if not args.no_served:
         served = run_served(args.serve_seconds, args.serve_budget_gib)
         progress(f"served: p99 {served['latency_ms_p99']} ms, {served['mpixel_per_s']} Mpixel/s")
+    sweep = None
+    if not args.no_served and args.serve_sweep:
+        sweep = run_serve_sweep([int(x) for x in args.serve_sweep.split(",")], args.serve_sweep_seconds)
     copy_gbs = copy_peak()
     roof["copy_peak_gbs"] = round(copy_gbs, 1)
     cpu = None
@@ -493,6 +528,7 @@ def main():
         "roofline": roof,
         "batched": batched,
         "served": served,
+        "latency_vs_streams": sweep,
         "precision": precision,
         "cpu_baseline": cpu,
     }

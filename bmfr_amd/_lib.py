@@ -116,6 +116,13 @@ SIGNATURES = [
     ("bmfr_scene_render_host", c_int32, [ctypes.c_void_p, c_int32, c_float_p, c_float_p,
                                          c_float_p, c_float_p]),
     ("bmfr_scene_render_to_stream", c_int32, [ctypes.c_void_p, c_int32, ctypes.c_void_p]),
+    ("bmfr_exr_info", c_int32, [ctypes.c_char_p, ctypes.POINTER(c_int32), ctypes.POINTER(c_int32),
+                                ctypes.POINTER(c_int32)]),
+    ("bmfr_exr_read_rgb", c_int32, [ctypes.c_char_p, c_int32, c_int32, c_float_p]),
+    ("bmfr_exr_write_rgb", c_int32, [ctypes.c_char_p, c_int32, c_int32, c_float_p, ctypes.c_int64, c_int32]),
+    ("bmfr_png_write_rgb", c_int32, [ctypes.c_char_p, c_int32, c_int32, c_float_p, ctypes.c_int64]),
+    ("bmfr_camera_matrices_load", c_int32, [ctypes.c_char_p, c_int32, c_float_p, c_float_p, c_float_p,
+                                            c_float_p]),
 ]
 
 _lib = None

@@ -20,13 +20,16 @@
 #include <vector>
 
 #include "bmfr_hip.h"
+#include "host_error.h"
 #include "kernels.h"
 #include "scene.h"
 #include "scene_launch.h"
 
-namespace {
+namespace bmfr_host {
 
+namespace {
 thread_local std::string g_last_error;
+}
 
 int32_t fail(int32_t code, const char *fmt, ...) {
     char buf[512];
@@ -37,6 +40,14 @@ int32_t fail(int32_t code, const char *fmt, ...) {
     g_last_error = buf;
     return code;
 }
+
+const char *last_error() { return g_last_error.c_str(); }
+
+}  // namespace bmfr_host
+
+namespace {
+
+using bmfr_host::fail;
 
 #define HIP_TRY(expr)                                                                       \
     do {                                                                                    \
@@ -286,7 +297,7 @@ const char *bmfr_status_string(int32_t status) {
     }
 }
 
-const char *bmfr_last_error(void) { return g_last_error.c_str(); }
+const char *bmfr_last_error(void) { return bmfr_host::last_error(); }
 
 int32_t bmfr_device_count(void) {
     int n = 0;

@@ -299,6 +299,37 @@ int32_t bmfr_scene_render_host(bmfr_scene scene, int32_t frame, float *albedo,
  * (albedo, normals, positions, noisy colour), enqueued on the stream. */
 int32_t bmfr_scene_render_to_stream(bmfr_scene scene, int32_t frame, bmfr_stream stream);
 
+/* ---- dataset-format IO (the reference's OpenImageIO and camera header use) ----
+ * Host-only helpers so that the published scenes run when a user supplies
+ * them: the reference reads color/shading_normal/world_position/albedo NN.exr
+ * as 3-channel float (opencl/bmfr.cpp:145-172), compiles the dataset's
+ * camera_matrices.h in (opencl/bmfr.cpp:43-47, 226-227, 440-444) and writes
+ * outputNN.png cropped to the image size (opencl/bmfr.cpp:519-547). */
+/* Size and channel count of a single-part scanline OpenEXR file. */
+int32_t bmfr_exr_info(const char *path, int32_t *width, int32_t *height, int32_t *channels);
+/* Channels R, G, B (or a single Y) of a scanline OpenEXR file with NONE,
+ * RLE, ZIPS or ZIP compression and HALF, FLOAT or UINT pixels, as
+ * interleaved float RGB with row stride `width` (as read_image_file's
+ * conversion to float, opencl/bmfr.cpp:145-164).  Other compressions and
+ * tiled or multi-part files are refused with BMFR_ERR_INVALID_ARGUMENT. */
+int32_t bmfr_exr_read_rgb(const char *path, int32_t width, int32_t height, float *rgb);
+/* Writes interleaved float RGB (row stride `row_stride` pixels) as a FLOAT
+ * B, G, R OpenEXR file; compression 0 (NONE) or 3 (ZIP). */
+int32_t bmfr_exr_write_rgb(const char *path, int32_t width, int32_t height, const float *rgb,
+                           int64_t row_stride, int32_t compression);
+/* Writes interleaved float RGB (row stride `row_stride` pixels) as an 8-bit
+ * RGB PNG, round(clamp(v, 0, 1) * 255) per channel (opencl/bmfr.cpp:519-547;
+ * the reference passes a WORKSET_WIDTH stride for IMAGE_WIDTH rows, which
+ * this does not reproduce). */
+int32_t bmfr_png_write_rgb(const char *path, int32_t width, int32_t height, const float *rgb,
+                           int64_t row_stride);
+/* Parses the dataset's camera_matrices.h: camera_matrices[frames][4][4]
+ * (row-vector view-projection, 16 floats per frame), pixel_offsets[frames][2]
+ * and the scalars position_limit_squared and normal_limit_squared (C
+ * declarations or #defines).  Outputs may be null. */
+int32_t bmfr_camera_matrices_load(const char *path, int32_t frames, float *matrices, float *offsets,
+                                  float *position_limit_squared, float *normal_limit_squared);
+
 #ifdef __cplusplus
 }
 #endif

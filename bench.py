@@ -106,7 +106,7 @@ def run_plain(args):
 BATCH_MIX = [(640, 360)] * 4 + [(1280, 720)] * 6 + [(1920, 1080)] * 4 + [(2560, 1440)] * 2
 
 
-def run_batched(steps, warmup):
+def run_batched(steps, warmup, lanes=2):
     """16 concurrent streams of mixed resolutions, one frame each per step:
     every stage is one launch over all of them (the fitter a persistent
     launch over all 22,988 blocks), replayed from a captured hipGraph.  The
@@ -116,6 +116,7 @@ def run_batched(steps, warmup):
     sts = [bmfr_amd.Stream(bmfr_amd.default_params(w, h)) for (w, h) in BATCH_MIX]
     scs = [bmfr_amd.Scene(0xB3F40000 + 3 + k, SPHERES, 1e-3, w, h) for k, (w, h) in enumerate(BATCH_MIX)]
     b = bmfr_amd.Batch(sts)
+    b.set_lanes(lanes)
     n = len(sts)
     pixels = sum(w * h for (w, h) in BATCH_MIX)
     blocks = sum(st.sizes.blocks for st in sts)
@@ -157,7 +158,7 @@ def run_batched(steps, warmup):
         "aggregate_frames_per_s": round(n / (step_ms * 1e-3), 1),
         "mpixel_per_s": round(pixels / (step_ms * 1e-3) / 1e6, 1),
         "stage_ms": {k: round(float(stage[i]), 4) for k, (i, _) in chain_kernels(2, 1, 1, 1).items()},
-        "graph": True, "autotune": tune,
+        "graph": True, "pipeline_lanes": lanes, "autotune": tune,
     }
 
 
@@ -456,6 +457,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=40)
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batch-lanes", type=int, default=2)
     ap.add_argument("--serve-seconds", type=float, default=30.0)
     ap.add_argument("--serve-budget-gib", type=float, default=4.0)
     ap.add_argument("--no-served", action="store_true")
@@ -484,7 +486,7 @@ def main():
             "traffic": measured_traffic(dom), "algorithmic_bytes": int(sb[dom])}
     batched = None
     if not args.no_batched:
-        batched = run_batched(args.batched_steps, 5)
+        batched = run_batched(args.batched_steps, 5, args.batch_lanes)
         batched["speedup_vs_plain_720p"] = round(ms_frame / batched["ms_per_720p_frame_equiv"], 2)
         progress(f"batched: {batched['step_ms']} ms/step, x{batched['speedup_vs_plain_720p']} vs plain")
     precision = None

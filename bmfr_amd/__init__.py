@@ -300,6 +300,9 @@ class Batch:
     def set_fused(self, level=2):
         _check(self._lib.bmfr_batch_set_fused(self._h, int(level)))
 
+    def set_lanes(self, lanes=2):
+        _check(self._lib.bmfr_batch_set_lanes(self._h, int(lanes)))
+
     def set_fitter_config(self, rows_per_lane, blocks_per_group, grid_cap=0):
         _check(self._lib.bmfr_batch_set_fitter_config(self._h, rows_per_lane, blocks_per_group,
                                                       grid_cap))

@@ -90,6 +90,7 @@ SIGNATURES = [
     ("bmfr_batch_synchronize", c_int32, [ctypes.c_void_p]),
     ("bmfr_batch_set_graph", c_int32, [ctypes.c_void_p, c_int32]),
     ("bmfr_batch_set_fused", c_int32, [ctypes.c_void_p, c_int32]),
+    ("bmfr_batch_set_lanes", c_int32, [ctypes.c_void_p, c_int32]),
     ("bmfr_batch_set_fitter_config", c_int32, [ctypes.c_void_p, c_int32, c_int32, c_int32]),
     ("bmfr_batch_stage_times", c_int32, [ctypes.c_void_p, c_float_p]),
     ("bmfr_batch_autotune", c_int32, [ctypes.c_void_p, c_int32, ctypes.c_char_p, ctypes.c_size_t]),

@@ -715,6 +715,7 @@ namespace {
 struct BatchGroup {
     bool g16 = false, t16 = false;
     int qr = 0;
+    int lane = 0;                        // software-pipeline lane (0 or 1)
     std::vector<int> members;            // indices into bmfr_batch_s::streams
     bmfr::BatchStream *d_streams = nullptr;
     bmfr::BatchStep *d_steps = nullptr;
@@ -744,6 +745,9 @@ struct bmfr_batch_s {
     bmfr::FitterConfig fitter{16, 4};
     int grid_cap = 0;
     int fuse = BMFR_FUSE_345;
+    int lanes = 2;                        // 2: lane 1 runs one stage behind lane 0 on stream2
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_s1 = nullptr, ev_join = nullptr;
     bool use_graph = false;
     hipGraphExec_t graph = nullptr;
     // pinned staging of the per-step parameters, one slot per group and ring entry
@@ -755,40 +759,66 @@ struct bmfr_batch_s {
 
 namespace {
 
-int32_t batch_enqueue_stages(bmfr_batch b, bool timed) {
-    // the five stages, each as one launch per storage group
-    for (int st = 0; st < kStages; ++st) {
-        if (timed) HIP_TRY(hipEventRecord(b->ev_start[st], b->stream));
-        for (auto &g : b->groups) {
-            const bmfr::BatchLaunch L = g.launch();
-            hipError_t e = hipSuccess;
-            switch (st) {
-                case 0:
-                    e = bmfr::launch_batch_accumulate_noisy_data(
-                        L, g.starts[bmfr::kStartsPadded][L.n], g.g16, g.t16, b->stream);
-                    break;
-                case 1:
-                    e = bmfr::launch_batch_fitter(L, g.starts[bmfr::kStartsBlocks][L.n], g.t16, b->fitter,
-                                                  b->grid_cap, g.qr, b->stream);
-                    break;
-                case 2:  // weighted_sum fused into stage 4 (0 ms)
-                    break;
-                case 3:
-                    if (b->fuse == BMFR_FUSE_345)
-                        e = bmfr::launch_batch_filter_taa(L, g.starts[bmfr::kStartsBlocks][L.n], g.g16, b->stream);
-                    else
-                        e = bmfr::launch_batch_weighted_sum_filtered(L, g.starts[bmfr::kStartsPadded][L.n],
-                                                                     g.g16, b->stream);
-                    break;
-                default:  // taa, unless fused into stage 4 (0 ms)
-                    if (b->fuse != BMFR_FUSE_345)
-                        e = bmfr::launch_batch_taa(L, g.starts[bmfr::kStartsImage][L.n], g.g16, b->stream);
-                    break;
-            }
-            if (e != hipSuccess) return fail(BMFR_ERR_HIP, "batched stage %d launch failed: %s", st, hipGetErrorString(e));
+// Stage `st` of the batch's groups in `lane` (-1: every group) on `stream`,
+// one launch per group.
+int32_t batch_enqueue_stage(bmfr_batch b, int st, int lane, hipStream_t stream) {
+    for (auto &g : b->groups) {
+        if (lane >= 0 && g.lane != lane) continue;
+        const bmfr::BatchLaunch L = g.launch();
+        hipError_t e = hipSuccess;
+        switch (st) {
+            case 0:
+                e = bmfr::launch_batch_accumulate_noisy_data(L, g.starts[bmfr::kStartsPadded][L.n], g.g16, g.t16,
+                                                             stream);
+                break;
+            case 1:
+                e = bmfr::launch_batch_fitter(L, g.starts[bmfr::kStartsBlocks][L.n], g.t16, b->fitter, b->grid_cap,
+                                              g.qr, stream);
+                break;
+            case 2:  // weighted_sum fused into stage 4 (0 ms)
+                break;
+            case 3:
+                if (b->fuse == BMFR_FUSE_345)
+                    e = bmfr::launch_batch_filter_taa(L, g.starts[bmfr::kStartsBlocks][L.n], g.g16, stream);
+                else
+                    e = bmfr::launch_batch_weighted_sum_filtered(L, g.starts[bmfr::kStartsPadded][L.n], g.g16,
+                                                                 stream);
+                break;
+            default:  // taa, unless fused into stage 4 (0 ms)
+                if (b->fuse != BMFR_FUSE_345)
+                    e = bmfr::launch_batch_taa(L, g.starts[bmfr::kStartsImage][L.n], g.g16, stream);
+                break;
         }
-        if (timed) HIP_TRY(hipEventRecord(b->ev_end[st], b->stream));
+        if (e != hipSuccess) return fail(BMFR_ERR_HIP, "batched stage %d launch failed: %s", st, hipGetErrorString(e));
     }
+    return BMFR_OK;
+}
+
+// The five stages of a step.  Timed per stage: stage by stage over every
+// group on the batch's stream.  Otherwise, with two lanes, a two-deep
+// software pipeline: lane 0 runs on the batch's stream, lane 1 on stream2
+// starting when lane 0's stage 1 is done, so that each lane's fitter
+// (VALU-bound) overlaps the other lane's HBM-streaming stages; lane 1 joins
+// the batch's stream at the end.  The lanes' streams are disjoint.
+int32_t batch_enqueue_stages(bmfr_batch b, bool timed) {
+    if (timed || b->lanes == 1) {
+        for (int st = 0; st < kStages; ++st) {
+            if (timed) HIP_TRY(hipEventRecord(b->ev_start[st], b->stream));
+            if (int32_t r = batch_enqueue_stage(b, st, -1, b->stream)) return r;
+            if (timed) HIP_TRY(hipEventRecord(b->ev_end[st], b->stream));
+        }
+        return BMFR_OK;
+    }
+    if (int32_t r = batch_enqueue_stage(b, 0, 0, b->stream)) return r;
+    HIP_TRY(hipEventRecord(b->ev_s1, b->stream));
+    HIP_TRY(hipStreamWaitEvent(b->stream2, b->ev_s1, 0));
+    if (int32_t r = batch_enqueue_stage(b, 0, 1, b->stream2)) return r;
+    for (int st = 1; st < kStages; ++st)
+        if (int32_t r = batch_enqueue_stage(b, st, 0, b->stream)) return r;
+    for (int st = 1; st < kStages; ++st)
+        if (int32_t r = batch_enqueue_stage(b, st, 1, b->stream2)) return r;
+    HIP_TRY(hipEventRecord(b->ev_join, b->stream2));
+    HIP_TRY(hipStreamWaitEvent(b->stream, b->ev_join, 0));
     return BMFR_OK;
 }
 
@@ -812,25 +842,41 @@ int32_t bmfr_batch_create(const bmfr_stream *streams, int32_t count, bmfr_batch 
     if (!b) return fail(BMFR_ERR_INVALID_ARGUMENT, "host allocation failed");
     b->streams.assign(streams, streams + count);
     b->device = streams[0]->device;
+    // pipeline lanes: largest streams first, each to the lane with fewer pixels
+    std::vector<int> lane_of(count, 0), order(count);
+    for (int i = 0; i < count; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+        return (int64_t)streams[x]->p.width * streams[x]->p.height > (int64_t)streams[y]->p.width * streams[y]->p.height;
+    });
+    int64_t lane_px[2] = {0, 0};
+    for (int i : order) {
+        const int l = lane_px[1] < lane_px[0] ? 1 : 0;
+        lane_of[i] = l;
+        lane_px[l] += (int64_t)streams[i]->p.width * streams[i]->p.height;
+    }
     for (int i = 0; i < count; ++i) {
         const bool g16 = streams[i]->p.storage == BMFR_STORAGE_F16;
         const bool t16 = streams[i]->p.tmp_storage == BMFR_STORAGE_F16;
         const int qr = streams[i]->p.qr_mode;
         BatchGroup *grp = nullptr;
         for (auto &g : b->groups)
-            if (g.g16 == g16 && g.t16 == t16 && g.qr == qr) grp = &g;
+            if (g.g16 == g16 && g.t16 == t16 && g.qr == qr && g.lane == lane_of[i]) grp = &g;
         if (!grp) {
             b->groups.emplace_back();
             grp = &b->groups.back();
             grp->g16 = g16;
             grp->t16 = t16;
             grp->qr = qr;
+            grp->lane = lane_of[i];
         }
         grp->members.push_back(i);
     }
     hipError_t e = hipSetDevice(b->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream2, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_s1, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming);
     for (int i = 0; i < kStages && e == hipSuccess; ++i) {
         e = hipEventCreate(&b->ev_start[i]);
         if (e == hipSuccess) e = hipEventCreate(&b->ev_end[i]);
@@ -894,6 +940,7 @@ int32_t bmfr_batch_destroy(bmfr_batch b) {
     if (!b) return BMFR_OK;
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->stream2) (void)hipStreamSynchronize(b->stream2);
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     for (auto &g : b->groups) {
         if (g.d_streams) (void)hipFree(g.d_streams);
@@ -907,8 +954,11 @@ int32_t bmfr_batch_destroy(bmfr_batch b) {
     for (auto &ev : b->ring_ev)
         if (ev) (void)hipEventDestroy(ev);
     if (b->done) (void)hipEventDestroy(b->done);
+    if (b->ev_s1) (void)hipEventDestroy(b->ev_s1);
+    if (b->ev_join) (void)hipEventDestroy(b->ev_join);
     if (b->h_steps) (void)hipHostFree(b->h_steps);
     if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->stream2) (void)hipStreamDestroy(b->stream2);
     delete b;
     return BMFR_OK;
 }
@@ -918,6 +968,17 @@ int32_t bmfr_batch_set_fused(bmfr_batch b, int32_t level) {
     if (level != BMFR_FUSE_34 && level != BMFR_FUSE_345)
         return fail(BMFR_ERR_INVALID_ARGUMENT, "batches run fuse level 1 or 2, not %d", level);
     b->fuse = level;
+    if (b->graph) {
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    return BMFR_OK;
+}
+
+int32_t bmfr_batch_set_lanes(bmfr_batch b, int32_t lanes) {
+    if (!b) return fail(BMFR_ERR_INVALID_ARGUMENT, "batch is null");
+    if (lanes != 1 && lanes != 2) return fail(BMFR_ERR_INVALID_ARGUMENT, "lanes %d is not 1 or 2", lanes);
+    b->lanes = lanes;
     if (b->graph) {
         (void)hipGraphExecDestroy(b->graph);
         b->graph = nullptr;
@@ -980,8 +1041,9 @@ int32_t bmfr_batch_run(bmfr_batch b, const int32_t *frames, const float *matrice
                                hipMemcpyHostToDevice, b->stream));
     }
     HIP_TRY(hipEventRecord(b->ring_ev[slot], b->stream));
-    // 3. the five batched stages (captured once into a graph when enabled)
-    if (b->use_graph && !timed) {
+    // 3. the five batched stages (captured once into a graph when enabled; a
+    // timed replay records only the step's first start and last end)
+    if (b->use_graph) {
         if (!b->graph) {
             hipGraph_t graph;
             HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));

@@ -234,7 +234,8 @@ int32_t bmfr_batch_destroy(bmfr_batch batch);
 /* One step: frames[i], matrices[16*i..], offsets[2*i..] are member i's
  * bmfr_accumulate_noisy_data arguments.  Ordered after each member's pending
  * work; each member's later work waits for it; members' double buffers are
- * swapped.  timed != 0 records per-stage events (no graph replay). */
+ * swapped.  timed != 0 records events: around every stage of an eager step
+ * (run stage by stage), or around the whole replay when a graph is used. */
 int32_t bmfr_batch_run(bmfr_batch batch, const int32_t *frames, const float *matrices,
                        const float *offsets, int32_t timed);
 int32_t bmfr_batch_synchronize(bmfr_batch batch);
@@ -242,6 +243,12 @@ int32_t bmfr_batch_synchronize(bmfr_batch batch);
 int32_t bmfr_batch_set_graph(bmfr_batch batch, int32_t enable);
 /* BMFR_FUSE_34 or BMFR_FUSE_345 (default) for the batch's stages 3-5. */
 int32_t bmfr_batch_set_fused(bmfr_batch batch, int32_t level);
+/* 2 (default): the members are split into two lanes of about equal pixel
+ * count and an untimed (or graph-replayed) step runs them as a two-deep
+ * software pipeline on two HIP streams, each lane's fitter overlapping the
+ * other lane's streaming stages; 1: stage by stage over all members.  A step
+ * timed per stage always runs stage by stage. */
+int32_t bmfr_batch_set_lanes(bmfr_batch batch, int32_t lanes);
 /* Fitter shape for the batch; grid_cap > 0 bounds the persistent grid. */
 int32_t bmfr_batch_set_fitter_config(bmfr_batch batch, int32_t rows_per_lane,
                                      int32_t blocks_per_group, int32_t grid_cap);

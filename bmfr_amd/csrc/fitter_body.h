@@ -742,9 +742,15 @@ template <int RPL, int BPG, typename TT, bool HALF_ROUND, bool COMP>
 __global__ __launch_bounds__(64 * (16 / RPL) * BPG) void k_fitter(const FitterParams p) {
     __shared__ float red[2][BPG][16 / RPL][kFastSlots];
     const int sub = threadIdx.x / (64 * (16 / RPL));
-    const int blk_raw = blockIdx.x * BPG + sub;
-    const bool active = blk_raw < p.blocks;
-    const int blk = active ? blk_raw : p.blocks - 1;
+    // Work-group g runs on XCD g % 8 and takes the blocks of that XCD's eighth
+    // of the frame (raster order, as stage 1's XCD-contiguous tiles wrote
+    // them), last-written first, so that its tmp_data can still be in that
+    // XCD's L2.
+    const int x = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
+    const int b0 = (int)((int64_t)x * p.blocks / kXcds), b1 = (int)((int64_t)(x + 1) * p.blocks / kXcds);
+    const int idx = j * BPG + sub;
+    const bool active = idx < b1 - b0;
+    const int blk = active ? b1 - 1 - idx : 0;
     if constexpr (!COMP) {
         fitter_block_fast<RPL, BPG, TT, HALF_ROUND>(
             reinterpret_cast<const TT *>(p.tmp) + (int64_t)blk * (kBufferCount * kBlockPixels),
@@ -798,7 +804,8 @@ __global__ __launch_bounds__(64 * (16 / RPL) * BPG) void kb_fitter(const BatchLa
 template <int RPL, int BPG, bool COMP>
 inline hipError_t launch_rpl(const FitterParams &fp, bool tmp_f16, hipStream_t s) {
     constexpr int threads = 64 * (16 / RPL) * BPG;
-    const dim3 grid((fp.blocks + BPG - 1) / BPG);
+    const int per_xcd = (fp.blocks + kXcds - 1) / kXcds;   // XCD-matched block order (k_fitter)
+    const dim3 grid(kXcds * ((per_xcd + BPG - 1) / BPG));
     if (tmp_f16)
         hipLaunchKernelGGL((k_fitter<RPL, BPG, __half, true, COMP>), grid, dim3(threads), 0, s, fp);
     else

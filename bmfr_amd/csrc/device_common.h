@@ -132,6 +132,19 @@ __device__ __forceinline__ int xcd_tile(int b, int n) {
     return x * q + (x < r ? x : r) + j;
 }
 
+// Member of a batch that owns work unit `unit`: the largest s < n with
+// starts[s] <= unit (starts[0] = 0, non-decreasing), by binary search over
+// the prefix table (wave-uniform when `unit` is).
+__device__ __forceinline__ int batch_stream_of(const int32_t *starts, int n, int unit) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (starts[mid] <= unit) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 __device__ __forceinline__ float round_half(float v) {
     return __half2float(__float2half_rn(v));
 }

@@ -163,8 +163,7 @@ __global__ __launch_bounds__(256) void kb_refactor(const BatchLaunch L) {
     const int32_t *starts = L.starts + kStartsBlocks * (kMaxBatch + 1);
     const int u = blockIdx.x;
     if (u >= starts[L.n]) return;
-    int s = 0;
-    while (s + 1 < L.n && starts[s + 1] <= u) ++s;
+    const int s = batch_stream_of(starts, L.n, u);
     const int blk = u - starts[s];
     const BatchStream &bs = L.streams[s];
     if (!bs.flags || !((const int32_t *)bs.flags)[blk]) return;

@@ -777,8 +777,7 @@ __global__ __launch_bounds__(64 * (16 / RPL) * BPG) void kb_fitter(const BatchLa
         const int raw = g * BPG + sub;
         const bool active = raw < total;
         const int u = active ? raw : total - 1;
-        int s = 0;
-        while (s + 1 < L.n && starts[s + 1] <= u) ++s;
+        const int s = batch_stream_of(starts, L.n, u);
         const int blk = u - starts[s];
         const BatchStream &bs = L.streams[s];
         int32_t *fl = (int32_t *)bs.flags;

@@ -660,14 +660,8 @@ __global__ __launch_bounds__(256) void k_taa(const StageArgs a, const StageBuffe
 // ---------------------------------------------------------------------------
 // Batched kernels: one 1-D grid over the concatenated tiles of all streams
 // of a batch.  Each work-group finds its stream from the prefix table
-// (uniform, scalar loop over at most kMaxBatch entries), then runs the same
-// per-pixel body as the single-stream kernel.
-__device__ __forceinline__ int find_stream(const int32_t *starts, int n, int unit) {
-    int s = 0;
-    while (s + 1 < n && starts[s + 1] <= unit) ++s;
-    return s;
-}
-
+// (uniform binary search, batch_stream_of), then runs the same per-pixel
+// body as the single-stream kernel.
 __device__ __forceinline__ void resolve_batch(const BatchLaunch &L, int s, StageArgs &a, StageBuffers &b) {
     const BatchStream &bs = L.streams[s];
     const BatchStep &st = L.steps[s];
@@ -706,7 +700,7 @@ template <typename GT, typename TT>
 __global__ __launch_bounds__(256) void kb_accumulate_noisy_data(const BatchLaunch L) {
     const int32_t *starts = L.starts + kStartsPadded * (kMaxBatch + 1);
     const int unit = xcd_tile(blockIdx.x, gridDim.x);
-    const int s = find_stream(starts, L.n, unit);
+    const int s = batch_stream_of(starts, L.n, unit);
     StageArgs a;
     StageBuffers b;
     resolve_batch(L, s, a, b);
@@ -725,7 +719,7 @@ __global__ __launch_bounds__(256) void kb_image_stage(const BatchLaunch L) {
     const int32_t *starts =
         L.starts + (STAGE == 345 ? kStartsBlocks : kBlockTiles ? kStartsPadded : kStartsImage) * (kMaxBatch + 1);
     const int unit = xcd_tile(blockIdx.x, gridDim.x);
-    const int s = find_stream(starts, L.n, unit);
+    const int s = batch_stream_of(starts, L.n, unit);
     StageArgs a;
     StageBuffers b;
     resolve_batch(L, s, a, b);

@@ -10,6 +10,7 @@
 // that a wave reads 64 consecutive AoS pixels (768 contiguous bytes).
 #include "device_common.h"
 #include "kernels.h"
+#include "stage1.h"
 #include "tone_curve.h"
 
 namespace bmfr {
@@ -23,161 +24,21 @@ namespace bmfr {
 template <typename GT, typename TT>
 __device__ __forceinline__ void stage1_body(const StageArgs &a, const StageBuffers &bufs, int gx, int gy,
                                             int lin, bool write_noise) {
-    const GT *__restrict__ cur_normals = (const GT *)bufs.normals_cur;
-    const GT *__restrict__ prev_normals = (const GT *)bufs.normals_prev;
-    const GT *__restrict__ cur_positions = (const GT *)bufs.positions_cur;
-    const GT *__restrict__ prev_positions = (const GT *)bufs.positions_prev;
-    const GT *__restrict__ cur_noisy = (const GT *)bufs.noisy_in;
-    const GT *__restrict__ prev_noisy = (const GT *)bufs.noisy_acc_prev;
-    const uint8_t *__restrict__ prev_spp = (const uint8_t *)bufs.spp_prev;
-    GT *__restrict__ acc_noisy = (GT *)bufs.noisy_acc_cur;
-    float2 *__restrict__ out_prev_pixel = (float2 *)bufs.prev_pixel;
-    uint8_t *__restrict__ accept_bools = (uint8_t *)bufs.accept;
-    uint8_t *__restrict__ cur_spp = (uint8_t *)bufs.spp_cur;
     TT *__restrict__ tmp_data = (TT *)bufs.tmp_data;
     if (write_noise && lin < kNoiseTable) {
         // the padded grid has at least 96x96 = 9216 work-items (W, H >= 47)
         ((double *)bufs.noise)[lin] =
             noise_term(lin & (kBlockPixels - 1), 1 + (lin >> 10), a.frame, a.noise_amount);
     }
-    const int W = a.width, H = a.height;
-    const int2 off = block_offset(a.frame);
-    const int pwx = gx - kBlock / 2 + off.x;
-    const int pwy = gy - kBlock / 2 + off.y;
-    const int px = mirror(pwx, W), py = mirror(pwy, H);
-    const int64_t lp = (int64_t)py * W + px;
-
-    const float3 wp = load3(cur_positions, lp);
-    const float3 nrm = load3(cur_normals, lp);
-    const float3 cur = load3(cur_noisy, lp);
-
-    float ppf_x = (float)px, ppf_y = (float)py;
-    uint32_t store_accept = 0;
-    float blend_alpha = 1.f;
-    float3 prev_col = make_float3(0.f, 0.f, 0.f);
-    float sample_spp = 0.f;
-    if (a.frame > 0) {
-        const float *m = a.matrix;
-        // row-vector matrix, dot() left to right, w = 1 (bmfr.cl:343-356)
-        float uvx = ((m[0] * wp.x + m[4] * wp.y) + m[8] * wp.z) + m[12] * 1.f;
-        float uvy = ((m[1] * wp.x + m[5] * wp.y) + m[9] * wp.z) + m[13] * 1.f;
-        const float ww = ((m[3] * wp.x + m[7] * wp.y) + m[11] * wp.z) + m[15] * 1.f;
-        uvx = uvx / ww;
-        uvy = uvy / ww;
-        uvx = uvx + 1.f;
-        uvy = uvy + 1.f;
-        uvx = uvx / 2.f;
-        uvy = uvy / 2.f;
-        ppf_x = uvx * (float)W;
-        ppf_y = uvy * (float)H;
-        ppf_x = ppf_x - a.pixel_offset_x;
-        ppf_y = ppf_y - (1.f - a.pixel_offset_y);
-        const int ipx = cvt_rtn(ppf_x), ipy = cvt_rtn(ppf_y);
-        const float fx = ppf_x - (float)ipx, fy = ppf_y - (float)ipy;
-        const float omx = 1.f - fx, omy = 1.f - fy;
-        const float wts[4] = {omx * omy, fx * omy, omx * fy, fx * fy};
-        // All four taps' history is loaded up front (one memory round trip
-        // instead of three dependent ones: position, then normal, then
-        // colour and spp); a tap outside the image reads the pixel's own
-        // texel and is never used.  The tests and sums below keep the
-        // reference's order (bmfr.cl:374-419).
-        bool tap_in[4];
-        int64_t tap_ls[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int sx = wrap_add(ipx, i & 1);
-            const int sy = wrap_add(ipy, i >> 1);
-            tap_in[i] = sx >= 0 && sy >= 0 && sx < W && sy < H;
-            tap_ls[i] = tap_in[i] ? (int64_t)wrap_mad(sy, W, sx) : lp;
-        }
-        float3 tpp[4], tpn[4], tpc[4];
-        float tsp[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            tpp[i] = load3(prev_positions, tap_ls[i]);
-            tpn[i] = load3(prev_normals, tap_ls[i]);
-            tpc[i] = load3(prev_noisy, tap_ls[i]);
-            tsp[i] = (float)prev_spp[tap_ls[i]];
-        }
-        float total_weight = 0.f;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            if (tap_in[i]) {
-                const float3 pp = tpp[i];
-                const float dx = pp.x - wp.x, dy = pp.y - wp.y, dz = pp.z - wp.z;
-                const float pd = (dx * dx + dy * dy) + dz * dz;
-                if (pd < a.position_limit_squared) {
-                    const float3 pn = tpn[i];
-                    const float nx = pn.x - nrm.x, ny = pn.y - nrm.y, nz = pn.z - nrm.z;
-                    const float nd = (nx * nx + ny * ny) + nz * nz;
-                    if (nd < a.normal_limit_squared) {
-                        store_accept |= 1u << i;
-                        sample_spp = sample_spp + wts[i] * tsp[i];
-                        const float3 pc = tpc[i];
-                        prev_col.x = prev_col.x + wts[i] * pc.x;
-                        prev_col.y = prev_col.y + wts[i] * pc.y;
-                        prev_col.z = prev_col.z + wts[i] * pc.z;
-                        total_weight = total_weight + wts[i];
-                    }
-                }
-            }
-        }
-        if (total_weight > 0.f) {  // bmfr.cl:421-429
-            prev_col.x = prev_col.x / total_weight;
-            prev_col.y = prev_col.y / total_weight;
-            prev_col.z = prev_col.z / total_weight;
-            sample_spp = sample_spp / total_weight;
-            blend_alpha = 1.f / (sample_spp + 1.f);
-            blend_alpha = fmaxf(blend_alpha, a.blend_alpha);
-        }
-    }
-    // bmfr.cl:433-442
-    uint32_t new_spp = 1;
-    if (blend_alpha < 1.f) {
-        if (sample_spp > 254.f) {
-            new_spp = 255;
-        } else {
-            float r = __builtin_rintf(sample_spp);
-            r = fminf(fmaxf(r, 0.f), 255.f);
-            new_spp = (uint32_t)r + 1u;
-        }
-    }
-    const float omb = 1.f - blend_alpha;
-    float3 nc;
-    nc.x = blend_alpha * cur.x + omb * prev_col.x;
-    nc.y = blend_alpha * cur.y + omb * prev_col.y;
-    nc.z = blend_alpha * cur.z + omb * prev_col.z;
-
-    // bmfr.cl:448-476: 13 features, block-major
-    const float feat[kBufferCount] = {1.f,
-                                      nrm.x,
-                                      nrm.y,
-                                      nrm.z,
-                                      wp.x,
-                                      wp.y,
-                                      wp.z,
-                                      wp.x * wp.x,
-                                      wp.y * wp.y,
-                                      wp.z * wp.z,
-                                      nc.x,
-                                      nc.y,
-                                      nc.z};
+    float feat[kBufferCount];
+    stage1_pixel<GT>(a, bufs, gx, gy, feat);
     // The work-group's 32x8 tile is rows yin0..yin0+7 of one block, and
     // those rows of a feature are 256 contiguous elements of tmp_data: the
     // features go through LDS and out as 16-byte stores (1.6 per work-item
     // instead of 13 narrow ones).
     __shared__ __align__(16) TT feat_tile[kBufferCount][256];
 #pragma unroll
-    for (int f = 0; f < kBufferCount; ++f) {
-        float v = feat[f];
-        if (v != v) v = 0.f;
-        if constexpr (sizeof(TT) == 2) {
-            v = fmaxf(fminf(v, 65504.f), -65504.f);
-            feat_tile[f][threadIdx.x] = __float2half_rn(v);
-        } else {
-            feat_tile[f][threadIdx.x] = v;
-        }
-    }
+    for (int f = 0; f < kBufferCount; ++f) feat_tile[f][threadIdx.x] = tmp_element<TT>(feat[f]);
     __syncthreads();
     {
         const int yin0 = (gy - (int)(threadIdx.x >> 5)) & (kBlock - 1);
@@ -190,14 +51,6 @@ __device__ __forceinline__ void stage1_body(const StageArgs &a, const StageBuffe
             *reinterpret_cast<uint4 *>(dst + f * kBlockPixels + e) =
                 *reinterpret_cast<const uint4 *>(&feat_tile[f][e]);
         }
-    }
-    // bmfr.cl:478-484 (and the spp of bmfr.cl:442, which mirrored
-    // work-items would only rewrite with the same value)
-    if (pwx >= 0 && pwx < W && pwy >= 0 && pwy < H) {
-        store3(acc_noisy, lp, nc);
-        out_prev_pixel[lp] = make_float2(ppf_x, ppf_y);
-        accept_bools[lp] = (uint8_t)store_accept;
-        cur_spp[lp] = (uint8_t)new_spp;
     }
 }
 

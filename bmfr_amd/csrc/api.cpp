@@ -160,7 +160,7 @@ struct bmfr_stream_s {
     hipEvent_t markers[kMarkers]{};
     hipEvent_t sync_ev = nullptr;   // orders this stream against batch launches
     bool recorded[kStages]{};
-    bmfr::FitterConfig fitter{16, 4};  // one wave per block, four blocks per work-group
+    bmfr::FitterConfig fitter{16, 2};  // one wave per block, two blocks per work-group
     int fuse = BMFR_FUSE_34;   // how bmfr_run_frame runs stages 3-5 (bmfr_fuse_level)
 
     int cur(int a_slot) const { return swapped ? a_slot : a_slot + 1; }

@@ -85,6 +85,16 @@ def default_params(width, height, **overrides):
     return p
 
 
+def parse_feature_buffers(params, not_scaled, scaled):
+    """Sets params' feature set from the reference's two feature strings,
+    NOT_SCALED_FEATURE_BUFFERS and SCALED_FEATURE_BUFFERS (opencl/bmfr.cpp:65-77),
+    e.g. ("1.f,normal.x,normal.y,normal.z,", "world_position.x*world_position.y").
+    The reference's own strings select its set (feature_count 0)."""
+    lib = _lib.load()
+    _check(lib.bmfr_parse_feature_buffers(not_scaled.encode(), scaled.encode(), ctypes.byref(params)))
+    return params
+
+
 def compute_sizes(params):
     lib = _lib.load()
     s = BmfrSizes()
@@ -180,8 +190,8 @@ class Stream:
         _check(self._lib.bmfr_run_frame(self._h, frame, _fp(m), _fp(o)))
 
     def set_fused(self, level=True):
-        """How run_frame runs stages 3-5: FUSE_NONE (False), FUSE_34, or
-        FUSE_345 (True, the default)."""
+        """How run_frame runs stages 3-5: FUSE_NONE (False), FUSE_34 (a new
+        stream's level) or FUSE_345 (True)."""
         if isinstance(level, bool):
             level = FUSE_345 if level else FUSE_NONE
         _check(self._lib.bmfr_set_fused(self._h, int(level)))

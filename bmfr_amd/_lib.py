@@ -29,6 +29,9 @@ class BmfrParams(ctypes.Structure):
         ("tmp_storage", ctypes.c_int32),
         ("qr_mode", ctypes.c_int32),
         ("refactor_threshold", ctypes.c_float),
+        ("feature_count", ctypes.c_int32),          # 0: the reference's set
+        ("features_not_scaled", ctypes.c_int32),
+        ("feature_factors", (ctypes.c_uint8 * 2) * 16),
     ]
 
 
@@ -54,6 +57,7 @@ SIGNATURES = [
     ("bmfr_last_error", ctypes.c_char_p, []),
     ("bmfr_device_count", c_int32, []),
     ("bmfr_default_params", c_int32, [c_int32, c_int32, ctypes.POINTER(BmfrParams)]),
+    ("bmfr_parse_feature_buffers", c_int32, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(BmfrParams)]),
     ("bmfr_compute_sizes", c_int32, [ctypes.POINTER(BmfrParams), ctypes.POINTER(BmfrSizes)]),
     ("bmfr_stream_open", c_int32, [ctypes.POINTER(BmfrParams), ctypes.POINTER(ctypes.c_void_p)]),
     ("bmfr_stream_close", c_int32, [ctypes.c_void_p]),

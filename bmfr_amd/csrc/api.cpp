@@ -98,8 +98,27 @@ int32_t validate(const bmfr_params *p) {
         !(p->second_blend_alpha >= 0.f && p->second_blend_alpha <= 1.f) ||
         !(p->taa_blend_alpha >= 0.f && p->taa_blend_alpha <= 1.f))
         return fail(BMFR_ERR_INVALID_ARGUMENT, "blend factors must lie in [0, 1]");
+    if (p->feature_count < 0 || p->feature_count > BMFR_MAX_FEATURES)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "feature_count %d outside 0..%d", p->feature_count,
+                    BMFR_MAX_FEATURES);
+    if (p->feature_count > 0) {
+        if (p->features_not_scaled < 0 || p->features_not_scaled > p->feature_count)
+            return fail(BMFR_ERR_INVALID_ARGUMENT, "features_not_scaled %d outside 0..%d", p->features_not_scaled,
+                        p->feature_count);
+        for (int i = 0; i < p->feature_count; ++i)
+            for (int k = 0; k < 2; ++k)
+                if (p->feature_factors[i][k] > BMFR_FACTOR_POSITION_Z)
+                    return fail(BMFR_ERR_INVALID_ARGUMENT, "feature %d: unknown factor %d", i,
+                                (int)p->feature_factors[i][k]);
+        if (p->qr_mode != BMFR_QR_FP32)
+            return fail(BMFR_ERR_INVALID_ARGUMENT, "a configured feature set runs with qr_mode BMFR_QR_FP32");
+    }
     return BMFR_OK;
 }
+
+// Features N and scaled features S of params (the reference's set: 10, 6).
+int features_of(const bmfr_params &p) { return p.feature_count ? p.feature_count : 10; }
+int scaled_of(const bmfr_params &p) { return p.feature_count ? p.feature_count - p.features_not_scaled : 6; }
 
 void sizes_of(const bmfr_params &p, bmfr_sizes &s, size_t *slot_bytes) {
     // opencl/bmfr.cpp:107-118
@@ -124,16 +143,17 @@ void sizes_of(const bmfr_params &p, bmfr_sizes &s, size_t *slot_bytes) {
     b[S_SPP_A] = b[S_SPP_B] = P;
     b[S_PREV_PIXEL] = 8 * P;
     b[S_ACCEPT] = P;
-    b[S_TMP] = 13 * t * (size_t)s.padded_pixels;
-    b[S_WEIGHTS] = 30 * 4 * (size_t)s.blocks;
-    b[S_MINS_MAXS] = 12 * 4 * (size_t)s.blocks;
+    const size_t nf = (size_t)features_of(p), ns = (size_t)scaled_of(p);
+    b[S_TMP] = (nf + 3) * t * (size_t)s.padded_pixels;
+    b[S_WEIGHTS] = 3 * nf * 4 * (size_t)s.blocks;
+    b[S_MINS_MAXS] = (ns ? 2 * ns : 1) * 4 * (size_t)s.blocks;
     b[S_FILTERED] = 12 * P;
     b[S_TONE] = 12 * P;
     b[S_OUT_A] = b[S_OUT_B] = 3 * g * P;
     b[S_RESULT_A] = b[S_RESULT_B] = 3 * g * P;
-    b[S_NOISE] = 9 * 1024 * sizeof(double);
+    b[S_NOISE] = (nf > 1 ? nf - 1 : 1) * 1024 * sizeof(double);
     b[S_FLAGS] = 4 * (size_t)s.blocks;
-    b[S_W64] = 30 * 4 * (size_t)s.blocks;
+    b[S_W64] = 3 * nf * 4 * (size_t)s.blocks;
     size_t total = 0;
     for (int i = 0; i < S_COUNT; ++i) {
         if (slot_bytes) slot_bytes[i] = b[i];
@@ -240,6 +260,11 @@ struct bmfr_stream_s {
         a.tmp_f16 = p.tmp_storage == BMFR_STORAGE_F16;
         a.qr_mode = p.qr_mode;
         a.refactor_threshold = p.refactor_threshold;
+        if (p.feature_count) {
+            a.features.count = p.feature_count;
+            a.features.not_scaled = p.features_not_scaled;
+            memcpy(a.features.factor, p.feature_factors, sizeof(a.features.factor));
+        }
         return a;
     }
 };
@@ -321,7 +346,99 @@ int32_t bmfr_default_params(int32_t width, int32_t height, bmfr_params *out) {
     out->tmp_storage = BMFR_STORAGE_F16;
     out->qr_mode = BMFR_QR_FP32;
     out->refactor_threshold = 1e-6f;
+    out->feature_count = 0;   // the reference's set, bmfr.cpp:65-77
+    out->features_not_scaled = 0;
+    memset(out->feature_factors, 0, sizeof(out->feature_factors));
     return validate(out);
+}
+
+}  // extern "C"
+
+namespace {
+
+// One term of the reference's feature strings: `1.f`, `normal.x`,
+// `world_position.y`, or a product `a*b` of two of those.
+bool parse_factor(const std::string &f, uint8_t &id) {
+    static const char *const names[] = {"normal.x", "normal.y", "normal.z",
+                                        "world_position.x", "world_position.y", "world_position.z"};
+    if (f == "1" || f == "1.f" || f == "1.0" || f == "1.0f") {
+        id = BMFR_FACTOR_ONE;
+        return true;
+    }
+    for (int i = 0; i < 6; ++i)
+        if (f == names[i]) {
+            id = (uint8_t)(BMFR_FACTOR_NORMAL_X + i);
+            return true;
+        }
+    return false;
+}
+
+// Appends the terms of one comma-separated string (spaces dropped, empty
+// terms skipped) to `terms`.
+int32_t parse_terms(const char *text, std::vector<std::pair<uint8_t, uint8_t>> &terms) {
+    std::string cur;
+    auto flush = [&]() -> int32_t {
+        if (cur.empty()) return BMFR_OK;
+        const size_t star = cur.find('*');
+        uint8_t a = 0, b = BMFR_FACTOR_ONE;
+        bool ok;
+        if (star == std::string::npos) {
+            ok = parse_factor(cur, a);
+        } else {
+            ok = cur.find('*', star + 1) == std::string::npos && parse_factor(cur.substr(0, star), a) &&
+                 parse_factor(cur.substr(star + 1), b);
+        }
+        if (!ok)
+            return fail(BMFR_ERR_INVALID_ARGUMENT,
+                        "feature term '%s' is not 1.f, normal.x|y|z, world_position.x|y|z or a product of two",
+                        cur.c_str());
+        terms.emplace_back(a, b);
+        cur.clear();
+        return BMFR_OK;
+    };
+    for (const char *c = text; *c; ++c) {
+        if (*c == ' ' || *c == '\t' || *c == '\n' || *c == '\r') continue;
+        if (*c == ',') {
+            if (int32_t st = flush()) return st;
+        } else {
+            cur.push_back(*c);
+        }
+    }
+    return flush();
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t bmfr_parse_feature_buffers(const char *not_scaled, const char *scaled, bmfr_params *params) {
+    if (!not_scaled || !scaled || !params) return fail(BMFR_ERR_INVALID_ARGUMENT, "null argument");
+    std::vector<std::pair<uint8_t, uint8_t>> terms;
+    if (int32_t st = parse_terms(not_scaled, terms)) return st;
+    const int ns = (int)terms.size();
+    if (int32_t st = parse_terms(scaled, terms)) return st;
+    const int n = (int)terms.size();
+    if (n < 1 || n > BMFR_MAX_FEATURES)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "%d feature terms outside 1..%d", n, BMFR_MAX_FEATURES);
+    // The reference's own strings (bmfr.cpp:65-77) select its set, on the
+    // kernels specialised for it.
+    static const uint8_t ref[10][2] = {{0, 0}, {1, 0}, {2, 0}, {3, 0}, {4, 0},
+                                       {5, 0}, {6, 0}, {4, 4}, {5, 5}, {6, 6}};
+    bool is_ref = n == 10 && ns == 4;
+    for (int i = 0; is_ref && i < n; ++i) is_ref = terms[i].first == ref[i][0] && terms[i].second == ref[i][1];
+    memset(params->feature_factors, 0, sizeof(params->feature_factors));
+    if (is_ref) {
+        params->feature_count = 0;
+        params->features_not_scaled = 0;
+        return BMFR_OK;
+    }
+    params->feature_count = n;
+    params->features_not_scaled = ns;
+    for (int i = 0; i < n; ++i) {
+        params->feature_factors[i][0] = terms[i].first;
+        params->feature_factors[i][1] = terms[i].second;
+    }
+    return BMFR_OK;
 }
 
 int32_t bmfr_compute_sizes(const bmfr_params *params, bmfr_sizes *out) {
@@ -506,6 +623,8 @@ int32_t bmfr_fitter_fp64_weights(bmfr_stream s, float *out) {
     if (int32_t st = check_stream(s)) return st;
     if (!out) return fail(BMFR_ERR_INVALID_ARGUMENT, "out is null");
     if (s->noise_frame < 0) return fail(BMFR_ERR_STATE, "the fitter has not run yet");
+    if (s->p.feature_count)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "FP64 weights are computed for the reference's feature set only");
     const bmfr::StageArgs a = s->args(s->noise_frame);
     const bmfr::StageBuffers b = s->buffers();
     HIP_TRY(hipSetDevice(s->device));
@@ -837,6 +956,8 @@ int32_t bmfr_batch_create(const bmfr_stream *streams, int32_t count, bmfr_batch 
             return fail(BMFR_ERR_INVALID_ARGUMENT, "streams of a batch must share one device");
         for (int j = 0; j < i; ++j)
             if (streams[j] == streams[i]) return fail(BMFR_ERR_INVALID_ARGUMENT, "stream %d listed twice", i);
+        if (streams[i]->p.feature_count)
+            return fail(BMFR_ERR_INVALID_ARGUMENT, "stream %d: batches run the reference's feature set", i);
     }
     bmfr_batch b = new (std::nothrow) bmfr_batch_s;
     if (!b) return fail(BMFR_ERR_INVALID_ARGUMENT, "host allocation failed");
@@ -1280,6 +1401,8 @@ int32_t bmfr_server_open_stream(bmfr_server sv, const bmfr_params *params, bmfr_
     *out = nullptr;
     bmfr_sizes z;
     if (int32_t st = bmfr_compute_sizes(params, &z)) return st;
+    if (params->feature_count)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "servers run the reference's feature set");
     if (sv->used + z.device_bytes > sv->budget)
         return fail(BMFR_ERR_OUT_OF_BUDGET, "stream needs %lld bytes, %lld of %lld in use",
                     (long long)z.device_bytes, (long long)sv->used, (long long)sv->budget);

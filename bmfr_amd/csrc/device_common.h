@@ -12,6 +12,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "kernels.h"
+
 namespace bmfr {
 
 constexpr int kBlock = 32;               // BLOCK_EDGE_LENGTH
@@ -21,6 +23,24 @@ constexpr int kFeatures = 10;
 constexpr int kNotScaled = 4;            // FEATURES_NOT_SCALED: 1, n.xyz
 constexpr int kScaled = 6;               // FEATURES_SCALED: p.xyz, p.xyz^2
 constexpr int kREdge = 11;               // R_EDGE
+// Factor `id` of a pixel: 1, n.x, n.y, n.z, p.x, p.y, p.z (ids are
+// wave-uniform kernel arguments, so the selects are scalar-driven).
+__device__ __forceinline__ float feature_factor(int id, float3 n, float3 p) {
+    float v = 1.f;
+    v = id == 1 ? n.x : v;
+    v = id == 2 ? n.y : v;
+    v = id == 3 ? n.z : v;
+    v = id == 4 ? p.x : v;
+    v = id == 5 ? p.y : v;
+    v = id == 6 ? p.z : v;
+    return v;
+}
+
+// Feature f of the set: the reference evaluates the expression text in
+// float (bmfr.cl:448-453, 727-729); a single term is x * 1 = x exactly.
+__device__ __forceinline__ float feature_value(const FeatureSet &fs, int f, float3 n, float3 p) {
+    return feature_factor(fs.factor[f][0], n, p) * feature_factor(fs.factor[f][1], n, p);
+}
 
 // Per-frame jitter of the block grid, opencl/bmfr.cl:267-285.  A table of
 // constants that has to be identical for the results to match.
@@ -101,14 +121,17 @@ __device__ __forceinline__ uint32_t hash32(uint32_t a) {
 }
 
 // The feature noise of one frame depends only on (row, feature, frame)
-// (bmfr.cl:179-181 has no block term), so one table of 9 x 1024 FP64
-// noise terms serves every block of a frame: table[(f-1)*1024 + row] =
-// NOISE_AMOUNT * 2 * (random(row + 1024 f + 13312 frame) - 0.5f), in double
-// because NOISE_AMOUNT is a double literal (bmfr.cl:179).
+// (bmfr.cl:179-181 has no block term), so one table of (N-1) x 1024 FP64
+// noise terms serves every block of a frame (N features, N + 3 buffers):
+// table[(f-1)*1024 + row] = NOISE_AMOUNT * 2 *
+// (random(row + 1024 f + 1024 (N+3) frame) - 0.5f), in double because
+// NOISE_AMOUNT is a double literal (bmfr.cl:179).
 constexpr int kNoiseTable = (kFeatures - 1) * kBlockPixels;
-__device__ __forceinline__ double noise_term(int row, int feature, int frame, double noise_amount) {
+constexpr int kMaxNoiseTable = (kMaxFeatures - 1) * kBlockPixels;
+__device__ __forceinline__ double noise_term(int row, int feature, int frame, double noise_amount,
+                                             int buffer_count = kBufferCount) {
     const uint32_t arg = (uint32_t)row + (uint32_t)feature * kBlockPixels +
-                         (uint32_t)frame * (uint32_t)(kBufferCount * kBlockPixels);
+                         (uint32_t)frame * (uint32_t)(buffer_count * kBlockPixels);
     const float r = (float)hash32(arg) / 4294967296.0f - 0.5f;
     return noise_amount * 2.0 * (double)r;
 }

@@ -215,13 +215,20 @@ hipError_t launch_fitter(const StageArgs &a, const StageBuffers &b, const Fitter
     fp.threshold = a.refactor_threshold;
     fp.blocks = a.blocks;
     fp.frame = a.frame;
+    fp.not_scaled = a.features.not_scaled;
     const bool h = a.tmp_f16 != 0;
+    if (a.features.count) {   // a configured feature set: plain FP32 QR only
+        if (qr_mode != 0) return hipErrorInvalidValue;
+        return a.features.count <= 8 ? fitter_set_lo(fp, h, a.features.count, s)
+                                     : fitter_set_hi(fp, h, a.features.count, s);
+    }
     hipError_t e = fitter_dispatch_single(fp, h, qr_mode >= 1, cfg, s);
     if (e != hipSuccess || qr_mode != 2) return e;
     return launch_refactor_all(a, b, (const int32_t *)b.flags, (float *)b.weights, s);
 }
 
 hipError_t launch_fitter_fp64_all(const StageArgs &a, const StageBuffers &b, float *out, hipStream_t s) {
+    if (a.features.count) return hipErrorInvalidValue;   // the reference's set only
     return launch_refactor_all(a, b, nullptr, out, s);
 }
 

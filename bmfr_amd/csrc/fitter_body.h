@@ -178,6 +178,7 @@ struct FitterParams {
     float threshold;       // flag when min |R_ii| < threshold * max |R_ii|
     int32_t blocks;
     int32_t frame;
+    int32_t not_scaled;    // configured feature sets: features [not_scaled, N) are scaled
 };
 
 // Per-block outputs of the FP32 fitter besides weights and mins_maxs.
@@ -194,23 +195,25 @@ __device__ __forceinline__ float div_by(float a, float d, float y) {
     return __builtin_fmaf(r, y, q);
 }
 
-// Back substitution, bmfr.cl:658-699, by one wave.  Lane L = 3*c + ch holds
-// column c (0..10) of R for colour channel ch: R[r][c] = elem(c, r) (the
-// factorised column c at row r, r < c, wave-uniform), R[c][c] = norms[c];
-// column 10 is Q^T b = rows 0..9 of colour column 10+ch.  Weights are
-// written by lane 30 + ch.
-template <typename Elem>
-__device__ __forceinline__ void back_substitution(Elem elem, const float (&norms)[kFeatures],
-                                                  float *__restrict__ weights, bool active, int lane) {
+// Back substitution, bmfr.cl:658-699, by one wave, for F features
+// (R_EDGE = F + 1).  Lane L = 3*c + ch holds column c (0..F) of R for
+// colour channel ch: R[r][c] = elem(c, r) (the factorised column c at row r,
+// r < c, wave-uniform), R[c][c] = norms[c]; column F is Q^T b = rows 0..F-1
+// of colour column F+ch.  Weights are written by lane 3F + ch.
+template <int F = kFeatures, typename Elem>
+__device__ __forceinline__ void back_substitution(Elem elem, const float (&norms)[F], float *__restrict__ weights,
+                                                  bool active, int lane) {
+    static_assert(3 * (F + 1) <= 64, "one lane per column and channel");
+    constexpr int RE = F + 1;   // R_EDGE
     const int cl = lane / 3;
     const int ch = lane - 3 * cl;
-    float rc[kFeatures];
+    float rc[F];
 #pragma unroll
-    for (int r = 0; r < kFeatures; ++r) rc[r] = 0.f;
+    for (int r = 0; r < F; ++r) rc[r] = 0.f;
 #pragma unroll
-    for (int c = 0; c < kFeatures; ++c) {
+    for (int c = 0; c < F; ++c) {
 #pragma unroll
-        for (int r = 0; r < kFeatures; ++r) {
+        for (int r = 0; r < F; ++r) {
             float val;
             if (r < c) val = elem(c, r);
             else if (r == c) val = norms[c];
@@ -219,29 +222,29 @@ __device__ __forceinline__ void back_substitution(Elem elem, const float (&norms
         }
     }
 #pragma unroll
-    for (int r = 0; r < kFeatures; ++r) {
-        const float v0 = elem(10, r);
-        const float v1 = elem(11, r);
-        const float v2 = elem(12, r);
-        if (cl == 10) rc[r] = ch == 0 ? v0 : (ch == 1 ? v1 : v2);
+    for (int r = 0; r < F; ++r) {
+        const float v0 = elem(F, r);
+        const float v1 = elem(F + 1, r);
+        const float v2 = elem(F + 2, r);
+        if (cl == F) rc[r] = ch == 0 ? v0 : (ch == 1 ? v1 : v2);
     }
 #pragma unroll
-    for (int i = kREdge - 2; i >= 0; --i) {
+    for (int i = RE - 2; i >= 0; --i) {
         const float divider = __shfl(rc[i], 3 * i + ch);
         if (cl >= i) rc[i] = rc[i] / divider;
-        float acc = __shfl(rc[i], 30 + ch);
+        float acc = __shfl(rc[i], 3 * F + ch);
 #pragma unroll
-        for (int j = i + 1; j < kREdge - 1; ++j) acc = acc - __shfl(rc[i], 3 * j + ch);
-        if (cl == 10) rc[i] = acc;
-        const float wi = __shfl(rc[i], 30 + ch);
+        for (int j = i + 1; j < RE - 1; ++j) acc = acc - __shfl(rc[i], 3 * j + ch);
+        if (cl == F) rc[i] = acc;
+        const float wi = __shfl(rc[i], 3 * F + ch);
         if (cl == i) {
 #pragma unroll
             for (int r = 0; r <= i; ++r) rc[r] = rc[r] * wi;
         }
     }
-    if (active && cl == 10 && ch < 3) {
+    if (active && cl == F && ch < 3) {
 #pragma unroll
-        for (int r = 0; r < kFeatures; ++r) weights[r * 3 + ch] = rc[r];
+        for (int r = 0; r < F; ++r) weights[r * 3 + ch] = rc[r];
     }
 }
 
@@ -534,8 +537,8 @@ using FastRed = float[2][BPG][16 / RPL][kFastSlots];
 // w = 0..W-1) and broadcast wave 0's uniform extras `e` through LDS.
 template <int W, typename Op, int N, int NE>
 __device__ __forceinline__ void fast_exchange(float (&v)[N], float (&e)[NE], float *rd, int wave, int lane) {
-    static_assert(N <= 16 && NE <= 16, "slot layout");
     if constexpr (W > 1) {
+        static_assert(N <= 16 && NE <= 16, "slot layout");
         if (lane == 0) {
 #pragma unroll
             for (int i = 0; i < N; ++i) rd[wave * kFastSlots + i] = v[i];
@@ -557,22 +560,30 @@ __device__ __forceinline__ void fast_exchange(float (&v)[N], float (&e)[NE], flo
     }
 }
 
-template <int RPL, typename TT, bool HALF_ROUND>
+// Feature count of fitter template argument NF: 0 is the reference's set
+// (10 features, 4 not scaled, compile-time), 1..16 a configured set.
+template <int NF>
+struct SetShape {
+    static constexpr int F = NF ? NF : kFeatures;   // features
+    static constexpr int C = F + 3;                  // BUFFER_COUNT
+};
+
+template <int RPL, typename TT, bool HALF_ROUND, int C = kBufferCount>
 struct FastRows {
     using L = FastLayout<RPL, TT>;
-    float a[kBufferCount][RPL];
+    float a[C][RPL];
     __device__ __forceinline__ void put(int f, int k, float v) { a[f][k] = HALF_ROUND ? round_half(v) : v; }
     // Row r (< 64 V) of column f, wave-uniform; valid in wave 0.
     __device__ __forceinline__ float pivot_row(int f, int r) const { return readlane(a[f][r % L::V], r / L::V); }
 };
 
-// Pivot step COL (round COL + 1): norm, u, the 12 - COL trailing dot
-// products, update.
-template <int COL, int RPL, int BPG, typename TT, bool HALF_ROUND>
-__device__ __forceinline__ void fast_pivot(FastRows<RPL, TT, HALF_ROUND> &m, float (&norms)[kFeatures],
+// Pivot step COL (round COL + 1): norm, u, the C - 1 - COL trailing dot
+// products, update (C = F + 3 columns).
+template <int COL, int RPL, int BPG, typename TT, bool HALF_ROUND, int F>
+__device__ __forceinline__ void fast_pivot(FastRows<RPL, TT, HALF_ROUND, F + 3> &m, float (&norms)[F],
                                            FastRed<RPL, BPG> &red, int sub, int t, int wave, int lane) {
     using L = FastLayout<RPL, TT>;
-    constexpr int NV = kBufferCount - 1 - COL;
+    constexpr int NV = F + 3 - 1 - COL;
     constexpr int V = L::V;
     constexpr int KC = COL % V;   // element of lane COL / V holding the pivot row
     float x[RPL];
@@ -633,84 +644,122 @@ __device__ __forceinline__ void fast_pivot(FastRows<RPL, TT, HALF_ROUND> &m, flo
     }
 }
 
-template <int COL, int RPL, int BPG, typename TT, bool HALF_ROUND>
-__device__ __forceinline__ void fast_pivots(FastRows<RPL, TT, HALF_ROUND> &m, float (&norms)[kFeatures],
+template <int COL, int RPL, int BPG, typename TT, bool HALF_ROUND, int F>
+__device__ __forceinline__ void fast_pivots(FastRows<RPL, TT, HALF_ROUND, F + 3> &m, float (&norms)[F],
                                             FastRed<RPL, BPG> &red, int sub, int t, int wave, int lane) {
-    if constexpr (COL < kFeatures) {
-        fast_pivot<COL, RPL, BPG, TT, HALF_ROUND>(m, norms, red, sub, t, wave, lane);
-        fast_pivots<COL + 1, RPL, BPG, TT, HALF_ROUND>(m, norms, red, sub, t, wave, lane);
+    if constexpr (COL < F) {
+        fast_pivot<COL, RPL, BPG, TT, HALF_ROUND, F>(m, norms, red, sub, t, wave, lane);
+        fast_pivots<COL + 1, RPL, BPG, TT, HALF_ROUND, F>(m, norms, red, sub, t, wave, lane);
     }
 }
 
 // One 32x32 block by the W waves of work-group slice `sub`; `active` false
 // recomputes a valid block (keeping barriers uniform) without storing.
-template <int RPL, int BPG, typename TT, bool HALF_ROUND>
+// NF: the feature set (SetShape); a configured set (NF > 0) scales its
+// features [not_scaled, NF) and runs one wave per block (RPL 16).
+template <int RPL, int BPG, typename TT, bool HALF_ROUND, int NF = 0>
 __device__ __forceinline__ void fitter_block_fast(const TT *__restrict__ src, float *__restrict__ mins_maxs,
                                                   float *__restrict__ weights, const double *__restrict__ noise,
-                                                  bool active, FastRed<RPL, BPG> &red) {
+                                                  bool active, FastRed<RPL, BPG> &red, int not_scaled = kNotScaled) {
     using L = FastLayout<RPL, TT>;
+    constexpr int F = SetShape<NF>::F, C = SetShape<NF>::C;
+    static_assert(NF == 0 || RPL == 16, "a configured feature set runs one wave per block");
     constexpr int V = L::V, T = L::T;
     const int t = threadIdx.x & (T - 1);
     const int sub = threadIdx.x / T;
     const int lane = threadIdx.x & 63;
     const int wave = t >> 6;
-    FastRows<RPL, TT, HALF_ROUND> m;
+    FastRows<RPL, TT, HALF_ROUND, C> m;
 #pragma unroll
-    for (int f = 0; f < kBufferCount; ++f)
+    for (int f = 0; f < C; ++f)
 #pragma unroll
         for (int q = 0; q < L::Q; ++q) load_rows<TT, V>(src + f * kBlockPixels + q * (T * V) + t * V, &m.a[f][q * V]);
 
     // ---- scaling, bmfr.cl:511-542 (max taken as -min(-x): exact) ----
-    float bmin[kScaled], bmax[kScaled];
-    {
-        float v[2 * kScaled];
+    if constexpr (NF == 0) {
+        float bmin[kScaled], bmax[kScaled];
+        {
+            float v[2 * kScaled];
+#pragma unroll
+            for (int i = 0; i < kScaled; ++i) {
+                float mn = INFINITY, nmx = INFINITY;
+#pragma unroll
+                for (int k = 0; k < RPL; ++k) {
+                    const float x = m.a[kNotScaled + i][k];
+                    mn = fminf(mn, x);
+                    nmx = fminf(nmx, -x);
+                }
+                v[i] = mn;
+                v[kScaled + i] = nmx;
+            }
+            wave_reduce_n<OpMin, 2 * kScaled>(v);
+            float none[1] = {0.f};
+            fast_exchange<L::W, OpMin>(v, none, &red[0][sub][0][0], wave, lane);
+#pragma unroll
+            for (int i = 0; i < kScaled; ++i) {
+                bmin[i] = v[i];
+                bmax[i] = -v[kScaled + i];
+            }
+        }
+        if (active && t == 0) {
+#pragma unroll
+            for (int i = 0; i < kScaled; ++i) {
+                mins_maxs[2 * i + 0] = bmin[i];
+                mins_maxs[2 * i + 1] = bmax[i];
+            }
+        }
+        // scale (bmfr.cl:200-205).  (v-min)/(max-min) is evaluated as
+        // Markstein's correctly rounded quotient: y = RN(1/d) once per
+        // feature, then q = RN(a*y), r = RN(a - q*d) (exact),
+        // q' = RN(q + r*y) = RN(a/d).  Without the division
+        // (|max - min| <= 1) the same three operations with d = y = 1 return
+        // the numerator exactly.
 #pragma unroll
         for (int i = 0; i < kScaled; ++i) {
+            const float dd = bmax[i] - bmin[i];
+            const bool divide = fabsf(dd) > 1.0f;
+            const float d = divide ? dd : 1.f;
+            const float y = divide ? 1.0f / dd : 1.f;
+#pragma unroll
+            for (int k = 0; k < RPL; ++k) m.put(kNotScaled + i, k, div_by(m.a[kNotScaled + i][k] - bmin[i], d, y));
+        }
+    } else {
+        // every feature's bounds are reduced (one round); those of the scaled
+        // ones [not_scaled, F) are stored and applied, as above
+        float v[2 * F];
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
             float mn = INFINITY, nmx = INFINITY;
 #pragma unroll
             for (int k = 0; k < RPL; ++k) {
-                const float x = m.a[kNotScaled + i][k];
-                mn = fminf(mn, x);
-                nmx = fminf(nmx, -x);
+                mn = fminf(mn, m.a[f][k]);
+                nmx = fminf(nmx, -m.a[f][k]);
             }
-            v[i] = mn;
-            v[kScaled + i] = nmx;
+            v[f] = mn;
+            v[F + f] = nmx;
         }
-        wave_reduce_n<OpMin, 2 * kScaled>(v);
-        float none[1] = {0.f};
-        fast_exchange<L::W, OpMin>(v, none, &red[0][sub][0][0], wave, lane);
+        wave_reduce_n<OpMin, 2 * F>(v);
 #pragma unroll
-        for (int i = 0; i < kScaled; ++i) {
-            bmin[i] = v[i];
-            bmax[i] = -v[kScaled + i];
+        for (int f = 0; f < F; ++f) {
+            if (f < not_scaled) continue;   // uniform: unscaled columns stay as loaded
+            const float bmin = v[f], bmax = -v[F + f];
+            if (active && t == 0) {
+                mins_maxs[2 * (f - not_scaled) + 0] = bmin;
+                mins_maxs[2 * (f - not_scaled) + 1] = bmax;
+            }
+            const float dd = bmax - bmin;
+            const bool divide = fabsf(dd) > 1.0f;
+            const float d = divide ? dd : 1.f;
+            const float y = divide ? 1.0f / dd : 1.f;
+#pragma unroll
+            for (int k = 0; k < RPL; ++k) m.put(f, k, div_by(m.a[f][k] - bmin, d, y));
         }
-    }
-    if (active && t == 0) {
-#pragma unroll
-        for (int i = 0; i < kScaled; ++i) {
-            mins_maxs[2 * i + 0] = bmin[i];
-            mins_maxs[2 * i + 1] = bmax[i];
-        }
-    }
-    // scale (bmfr.cl:200-205).  (v-min)/(max-min) is evaluated as Markstein's
-    // correctly rounded quotient: y = RN(1/d) once per feature, then
-    // q = RN(a*y), r = RN(a - q*d) (exact), q' = RN(q + r*y) = RN(a/d).
-    // Without the division (|max - min| <= 1) the same three operations with
-    // d = y = 1 return the numerator exactly.
-#pragma unroll
-    for (int i = 0; i < kScaled; ++i) {
-        const float dd = bmax[i] - bmin[i];
-        const bool divide = fabsf(dd) > 1.0f;
-        const float d = divide ? dd : 1.f;
-        const float y = divide ? 1.0f / dd : 1.f;
-#pragma unroll
-        for (int k = 0; k < RPL; ++k) m.put(kNotScaled + i, k, div_by(m.a[kNotScaled + i][k] - bmin[i], d, y));
     }
     // ---- feature noise of the first pass (bmfr.cl:623-627): the value plus
     // the FP64 noise term, rounded once to float (NOISE_AMOUNT is a double
-    // literal) ----
+    // literal); features 1..F-1 ----
 #pragma unroll
-    for (int f = 1; f < kFeatures; ++f) {
+    for (int f = 1; f < F; ++f) {
 #pragma unroll
         for (int q = 0; q < L::Q; ++q) {
             const double *nz = noise + (f - 1) * kBlockPixels + q * (T * V) + t * V;
@@ -723,13 +772,13 @@ __device__ __forceinline__ void fitter_block_fast(const TT *__restrict__ src, fl
         }
     }
 
-    // ---- Householder QR over the 10 feature columns, bmfr.cl:549-656 ----
-    float norms[kFeatures];
-    fast_pivots<0, RPL, BPG, TT, HALF_ROUND>(m, norms, red, sub, t, wave, lane);
+    // ---- Householder QR over the F feature columns, bmfr.cl:549-656 ----
+    float norms[F];
+    fast_pivots<0, RPL, BPG, TT, HALF_ROUND, F>(m, norms, red, sub, t, wave, lane);
 
     // ---- back substitution, bmfr.cl:658-699, in wave 0 ----
     if (wave == 0)
-        back_substitution([&](int c, int r) { return m.pivot_row(c, r); }, norms, weights, active, lane);
+        back_substitution<F>([&](int c, int r) { return m.pivot_row(c, r); }, norms, weights, active, lane);
 }
 
 
@@ -762,6 +811,33 @@ __global__ __launch_bounds__(64 * (16 / RPL) * BPG) void k_fitter(const FitterPa
             p.mins_maxs + (int64_t)blk * (kScaled * 2), p.weights + (int64_t)blk * (kFeatures * 3),
             p.noise, BlockFlag{p.flags ? p.flags + blk : nullptr, p.threshold}, active, red);
     }
+}
+
+// A configured feature set of NF features (NF + 3 columns): one wave per
+// block, two blocks per work-group, the XCD-matched order of k_fitter.
+template <typename TT, bool HALF_ROUND, int NF>
+__global__ __launch_bounds__(128) void k_fitter_set(const FitterParams p) {
+    constexpr int BPG = 2, C = NF + 3;
+    __shared__ float red[2][BPG][1][kFastSlots];
+    const int sub = threadIdx.x >> 6;
+    const int x = blockIdx.x % kXcds, j = blockIdx.x / kXcds;
+    const int b0 = (int)((int64_t)x * p.blocks / kXcds), b1 = (int)((int64_t)(x + 1) * p.blocks / kXcds);
+    const int idx = j * BPG + sub;
+    const bool active = idx < b1 - b0;
+    const int blk = active ? b1 - 1 - idx : 0;
+    fitter_block_fast<16, BPG, TT, HALF_ROUND, NF>(
+        reinterpret_cast<const TT *>(p.tmp) + (int64_t)blk * (C * kBlockPixels),
+        p.mins_maxs + (int64_t)blk * ((NF - p.not_scaled) * 2), p.weights + (int64_t)blk * (NF * 3), p.noise,
+        active, red, p.not_scaled);
+}
+
+template <int NF>
+inline hipError_t launch_set(const FitterParams &fp, bool tmp_f16, hipStream_t s) {
+    const int per_xcd = (fp.blocks + kXcds - 1) / kXcds;
+    const dim3 grid(kXcds * ((per_xcd + 1) / 2));
+    if (tmp_f16) hipLaunchKernelGGL((k_fitter_set<__half, true, NF>), grid, dim3(128), 0, s, fp);
+    else hipLaunchKernelGGL((k_fitter_set<float, false, NF>), grid, dim3(128), 0, s, fp);
+    return hipGetLastError();
 }
 
 // Persistent batched fitter: the work-groups stride over the concatenated
@@ -861,5 +937,9 @@ hipError_t fitter_dispatch_single(const FitterParams &fp, bool tmp_f16, bool com
                                   hipStream_t s);
 hipError_t fitter_dispatch_batch(const BatchLaunch &L, int total, bool tmp_f16, bool comp,
                                  const FitterConfig &cfg, int grid_cap, hipStream_t s);
+// Configured feature sets of 1..8 and 9..16 features (fitter_set_lo.hip,
+// fitter_set_hi.hip).
+hipError_t fitter_set_lo(const FitterParams &fp, bool tmp_f16, int features, hipStream_t s);
+hipError_t fitter_set_hi(const FitterParams &fp, bool tmp_f16, int features, hipStream_t s);
 
 }  // namespace bmfr

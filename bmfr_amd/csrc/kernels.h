@@ -6,6 +6,18 @@
 
 namespace bmfr {
 
+constexpr int kMaxFeatures = 16;         // longest configurable feature set
+constexpr int kMaxColumns = kMaxFeatures + 3;
+
+// A configurable feature set (opencl/bmfr.cpp:63-77): feature i of a pixel
+// is factor[i][0] * factor[i][1] over {1, normal.xyz, world_position.xyz}
+// (factor ids 0..6); features [not_scaled, count) are scaled per block.
+struct FeatureSet {
+    int32_t count;
+    int32_t not_scaled;
+    uint8_t factor[kMaxFeatures][2];
+};
+
 // Per-launch scalars of one stream (the reference's -D values plus the
 // per-frame kernel arguments of opencl/bmfr.cpp:429-476).
 struct StageArgs {
@@ -22,6 +34,7 @@ struct StageArgs {
     float pixel_offset_x, pixel_offset_y;
     int32_t qr_mode;       // bmfr_qr_mode
     float refactor_threshold;
+    FeatureSet features;   // read by the kernels of a configured feature set only
 };
 
 // Device pointers of one stream's buffer set for the current frame (the
@@ -37,7 +50,7 @@ struct StageBuffers {
     void *filtered, *tone_mapped;
     void *out_cur, *out_prev;
     void *result_cur, *result_prev;
-    void *noise;          // double[9*1024]: the fitter's feature noise of one frame
+    void *noise;          // double[(N-1)*1024]: the fitter's feature noise of one frame
     void *flags;          // int32[blocks]: FP64 re-factorisation flags
 };
 

@@ -6,14 +6,20 @@
 
 namespace bmfr {
 
+// What stage 1 hands to the block matrix for one work-item: the pixel's
+// world position and normal (the variables of the feature expressions,
+// bmfr.cl:448-453) and its accumulated noisy colour (the last three
+// columns).
+struct Stage1Pixel {
+    float3 wp, nrm, col;
+};
+
 // Stage 1: accumulate_noisy_data, opencl/bmfr.cl:287-485, for one
 // work-item (gx, gy) of the padded grid: reprojection, discard, spp and the
 // blend, the pixel's own outputs stored (by the work-item that owns the
-// pixel), and its 13 features returned in `feat` for the caller's block
-// matrix (bmfr.cl:448-476).
+// pixel), and what its block-matrix row is made of returned.
 template <typename GT>
-__device__ __forceinline__ void stage1_pixel(const StageArgs &a, const StageBuffers &bufs, int gx, int gy,
-                                             float (&feat)[kBufferCount]) {
+__device__ __forceinline__ Stage1Pixel stage1_pixel(const StageArgs &a, const StageBuffers &bufs, int gx, int gy) {
     const GT *__restrict__ cur_normals = (const GT *)bufs.normals_cur;
     const GT *__restrict__ prev_normals = (const GT *)bufs.normals_prev;
     const GT *__restrict__ cur_positions = (const GT *)bufs.positions_cur;
@@ -133,22 +139,6 @@ __device__ __forceinline__ void stage1_pixel(const StageArgs &a, const StageBuff
     nc.y = blend_alpha * cur.y + omb * prev_col.y;
     nc.z = blend_alpha * cur.z + omb * prev_col.z;
 
-    // bmfr.cl:448-476: the 13 features
-    const float fv[kBufferCount] = {1.f,
-                                      nrm.x,
-                                      nrm.y,
-                                      nrm.z,
-                                      wp.x,
-                                      wp.y,
-                                      wp.z,
-                                      wp.x * wp.x,
-                                      wp.y * wp.y,
-                                      wp.z * wp.z,
-                                      nc.x,
-                                      nc.y,
-                                      nc.z};
-#pragma unroll
-    for (int f = 0; f < kBufferCount; ++f) feat[f] = fv[f];
     // bmfr.cl:478-484 (and the spp of bmfr.cl:442, which mirrored
     // work-items would only rewrite with the same value)
     if (pwx >= 0 && pwx < W && pwy >= 0 && pwy < H) {
@@ -157,6 +147,17 @@ __device__ __forceinline__ void stage1_pixel(const StageArgs &a, const StageBuff
         accept_bools[lp] = (uint8_t)store_accept;
         cur_spp[lp] = (uint8_t)new_spp;
     }
+    return Stage1Pixel{wp, nrm, nc};
+}
+
+// The reference's 13 buffers of a pixel (bmfr.cpp:65-77 as bmfr.cl:448-453
+// evaluates them): 1, normal, world position, its squares, colour.
+__device__ __forceinline__ void default_features(const Stage1Pixel &q, float (&feat)[kBufferCount]) {
+    const float3 wp = q.wp, nrm = q.nrm, nc = q.col;
+    const float fv[kBufferCount] = {1.f,  nrm.x, nrm.y,       nrm.z,       wp.x,        wp.y, wp.z,
+                                    wp.x * wp.x, wp.y * wp.y, wp.z * wp.z, nc.x, nc.y,  nc.z};
+#pragma unroll
+    for (int f = 0; f < kBufferCount; ++f) feat[f] = fv[f];
 }
 
 

@@ -18,35 +18,56 @@ namespace bmfr {
 // ---------------------------------------------------------------------------
 // Stage 1: accumulate_noisy_data, opencl/bmfr.cl:287-485.
 // One work-item of the padded grid (gx, gy); `lin` is its linear index,
-// used by the first 9216 work-items to also write the frame's noise table.
-// Called by all 256 work-items of a 32x8 tile (gx = 32 tx + t % 32,
-// gy = 8 ty + t / 32, t = threadIdx.x): the body has a barrier.
-template <typename GT, typename TT>
+// used by the first (N-1)*1024 work-items (N features) to also write the
+// frame's noise table.  Called by all 256 work-items of a 32x8 tile
+// (gx = 32 tx + t % 32, gy = 8 ty + t / 32, t = threadIdx.x): the body has a
+// barrier.  CUSTOM evaluates the feature set in `a.features` (N + 3
+// columns) instead of the reference's 13.
+template <typename GT, typename TT, bool CUSTOM = false>
 __device__ __forceinline__ void stage1_body(const StageArgs &a, const StageBuffers &bufs, int gx, int gy,
                                             int lin, bool write_noise) {
     TT *__restrict__ tmp_data = (TT *)bufs.tmp_data;
-    if (write_noise && lin < kNoiseTable) {
+    const int nf = CUSTOM ? a.features.count : kFeatures;
+    const int ncol = nf + 3;
+    if constexpr (CUSTOM) {
+        // up to 15 x 1024 terms; the padded grid has at least 96x96 work-items
+        if (write_noise) {
+            const int nt = (nf - 1) * kBlockPixels, stride = a.margin_width * a.margin_height;
+            for (int i = lin; i < nt; i += stride)
+                ((double *)bufs.noise)[i] =
+                    noise_term(i & (kBlockPixels - 1), 1 + (i >> 10), a.frame, a.noise_amount, ncol);
+        }
+    } else if (write_noise && lin < kNoiseTable) {
         // the padded grid has at least 96x96 = 9216 work-items (W, H >= 47)
         ((double *)bufs.noise)[lin] =
             noise_term(lin & (kBlockPixels - 1), 1 + (lin >> 10), a.frame, a.noise_amount);
     }
-    float feat[kBufferCount];
-    stage1_pixel<GT>(a, bufs, gx, gy, feat);
+    const Stage1Pixel q = stage1_pixel<GT>(a, bufs, gx, gy);
     // The work-group's 32x8 tile is rows yin0..yin0+7 of one block, and
     // those rows of a feature are 256 contiguous elements of tmp_data: the
     // features go through LDS and out as 16-byte stores (1.6 per work-item
     // instead of 13 narrow ones).
-    __shared__ __align__(16) TT feat_tile[kBufferCount][256];
+    __shared__ __align__(16) TT feat_tile[CUSTOM ? kMaxColumns : kBufferCount][256];
+    if constexpr (CUSTOM) {
+        for (int f = 0; f < nf; ++f)
+            feat_tile[f][threadIdx.x] = tmp_element<TT>(feature_value(a.features, f, q.nrm, q.wp));
+        feat_tile[nf][threadIdx.x] = tmp_element<TT>(q.col.x);
+        feat_tile[nf + 1][threadIdx.x] = tmp_element<TT>(q.col.y);
+        feat_tile[nf + 2][threadIdx.x] = tmp_element<TT>(q.col.z);
+    } else {
+        float feat[kBufferCount];
+        default_features(q, feat);
 #pragma unroll
-    for (int f = 0; f < kBufferCount; ++f) feat_tile[f][threadIdx.x] = tmp_element<TT>(feat[f]);
+        for (int f = 0; f < kBufferCount; ++f) feat_tile[f][threadIdx.x] = tmp_element<TT>(feat[f]);
+    }
     __syncthreads();
     {
         const int yin0 = (gy - (int)(threadIdx.x >> 5)) & (kBlock - 1);
         const int64_t blk = (int64_t)(gy / kBlock) * a.blocks_x + gx / kBlock;
-        TT *dst = tmp_data + blk * (kBufferCount * kBlockPixels) + yin0 * kBlock;
+        TT *dst = tmp_data + blk * (ncol * kBlockPixels) + yin0 * kBlock;
         constexpr int kVec = 16 / (int)sizeof(TT);             // elements per store
         constexpr int kPerFeature = 256 / kVec;
-        for (int c = threadIdx.x; c < kBufferCount * kPerFeature; c += 256) {
+        for (int c = threadIdx.x; c < ncol * kPerFeature; c += 256) {
             const int f = c / kPerFeature, e = (c - f * kPerFeature) * kVec;
             *reinterpret_cast<uint4 *>(dst + f * kBlockPixels + e) =
                 *reinterpret_cast<const uint4 *>(&feat_tile[f][e]);
@@ -54,14 +75,14 @@ __device__ __forceinline__ void stage1_body(const StageArgs &a, const StageBuffe
     }
 }
 
-template <typename GT, typename TT>
+template <typename GT, typename TT, bool CUSTOM>
 __global__ __launch_bounds__(256) void k_accumulate_noisy_data(const StageArgs a, const StageBuffers b,
                                                                int write_noise) {
     const int tiles_x = a.margin_width / 32;
     const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int ty = tile / tiles_x, tx = tile - ty * tiles_x;
-    stage1_body<GT, TT>(a, b, tx * 32 + (threadIdx.x & 31), ty * 8 + (threadIdx.x >> 5),
-                        tile * 256 + threadIdx.x, write_noise != 0);
+    stage1_body<GT, TT, CUSTOM>(a, b, tx * 32 + (threadIdx.x & 31), ty * 8 + (threadIdx.x >> 5),
+                                tile * 256 + threadIdx.x, write_noise != 0);
 }
 
 // 64x4 image tile of a single-stream launch, XCD-contiguous.
@@ -92,51 +113,69 @@ __device__ __forceinline__ void block_tile(const StageArgs &a, int local, int &x
 
 // Filtered colour of in-image pixel lp in block g: sum of block weights
 // times the pixel's (scaled) features, negatives clamped (bmfr.cl:712-750).
-template <typename GT>
+// CUSTOM evaluates the feature set in `a.features`.
+template <typename GT, bool CUSTOM = false>
 __device__ __forceinline__ float3 stage3_color_of(const StageArgs &a, const StageBuffers &bufs, float3 wp, float3 n,
                                                  int64_t g) {
     const float *__restrict__ weights = (const float *)bufs.weights;
     const float *__restrict__ mins_maxs = (const float *)bufs.mins_maxs;
-    const float feat[kFeatures] = {1.f,  n.x,  n.y,         n.z,         wp.x,
-                                   wp.y, wp.z, wp.x * wp.x, wp.y * wp.y, wp.z * wp.z};
-    const float *w = weights + g * (kFeatures * 3);
-    const float *mm = mins_maxs + g * (kScaled * 2);
     float cx = 0.f, cy = 0.f, cz = 0.f;
+    if constexpr (CUSTOM) {
+        const FeatureSet &fs = a.features;
+        const int nf = fs.count, ns = fs.count - fs.not_scaled;
+        const float *w = weights + g * (nf * 3);
+        const float *mm = mins_maxs + g * (ns * 2);
+        for (int f = 0; f < nf; ++f) {
+            float v = feature_value(fs, f, n, wp);
+            if (f >= fs.not_scaled) {
+                const int i = f - fs.not_scaled;
+                v = scale_feature(v, mm[i * 2], mm[i * 2 + 1]);
+            }
+            cx = cx + w[f * 3 + 0] * v;
+            cy = cy + w[f * 3 + 1] * v;
+            cz = cz + w[f * 3 + 2] * v;
+        }
+    } else {
+        const float feat[kFeatures] = {1.f,  n.x,  n.y,         n.z,         wp.x,
+                                       wp.y, wp.z, wp.x * wp.x, wp.y * wp.y, wp.z * wp.z};
+        const float *w = weights + g * (kFeatures * 3);
+        const float *mm = mins_maxs + g * (kScaled * 2);
 #pragma unroll
-    for (int f = 0; f < kFeatures; ++f) {
-        float v = feat[f];
-        if (f >= kNotScaled) v = scale_feature(v, mm[(f - kNotScaled) * 2], mm[(f - kNotScaled) * 2 + 1]);
-        cx = cx + w[f * 3 + 0] * v;
-        cy = cy + w[f * 3 + 1] * v;
-        cz = cz + w[f * 3 + 2] * v;
+        for (int f = 0; f < kFeatures; ++f) {
+            float v = feat[f];
+            if (f >= kNotScaled) v = scale_feature(v, mm[(f - kNotScaled) * 2], mm[(f - kNotScaled) * 2 + 1]);
+            cx = cx + w[f * 3 + 0] * v;
+            cy = cy + w[f * 3 + 1] * v;
+            cz = cz + w[f * 3 + 2] * v;
+        }
     }
     return make_float3(cx < 0.f ? 0.f : cx, cy < 0.f ? 0.f : cy, cz < 0.f ? 0.f : cz);
 }
 
-template <typename GT>
+template <typename GT, bool CUSTOM = false>
 __device__ __forceinline__ float3 stage3_color(const StageArgs &a, const StageBuffers &bufs, int64_t lp,
                                               int64_t g) {
-    return stage3_color_of<GT>(a, bufs, load3((const GT *)bufs.positions_cur, lp),
-                               load3((const GT *)bufs.normals_cur, lp), g);
+    return stage3_color_of<GT, CUSTOM>(a, bufs, load3((const GT *)bufs.positions_cur, lp),
+                                       load3((const GT *)bufs.normals_cur, lp), g);
 }
 
-template <typename GT>
+template <typename GT, bool CUSTOM = false>
 __device__ __forceinline__ void stage3_body(const StageArgs &a, const StageBuffers &bufs, int local) {
     int x, y;
     int64_t g;
     block_tile(a, local, x, y, g);
     if (x < 0 || y < 0 || x >= a.width || y >= a.height) return;
     const int64_t lp = (int64_t)y * a.width + x;
-    const float3 c = stage3_color<GT>(a, bufs, lp, g);
+    const float3 c = stage3_color<GT, CUSTOM>(a, bufs, lp, g);
     float *__restrict__ output = (float *)bufs.filtered;
     output[lp * 3 + 0] = c.x;
     output[lp * 3 + 1] = c.y;
     output[lp * 3 + 2] = c.z;
 }
 
-template <typename GT>
+template <typename GT, bool CUSTOM>
 __global__ __launch_bounds__(256) void k_weighted_sum(const StageArgs a, const StageBuffers b) {
-    stage3_body<GT>(a, b, xcd_tile(blockIdx.x, gridDim.x));
+    stage3_body<GT, CUSTOM>(a, b, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 // ---------------------------------------------------------------------------
@@ -253,7 +292,7 @@ __device__ __forceinline__ void stage4_body(const StageArgs &a, const StageBuffe
 // Stages 3 and 4 fused for the chained paths, on stage 3's launch shape: the
 // filtered colour stays in registers (the `filtered` buffer is not
 // written); same arithmetic.
-template <typename GT>
+template <typename GT, bool CUSTOM = false>
 __device__ __forceinline__ void stage34_body(const StageArgs &a, const StageBuffers &bufs, int local) {
     int x, y;
     int64_t g;
@@ -261,12 +300,12 @@ __device__ __forceinline__ void stage34_body(const StageArgs &a, const StageBuff
     if (x < 0 || y < 0 || x >= a.width || y >= a.height) return;
     const int64_t lp = (int64_t)y * a.width + x;
     const Stage4Gather<GT> gth = stage4_gather<GT>(a, bufs, lp);
-    stage4_finish<GT>(a, bufs, lp, stage3_color<GT>(a, bufs, lp, g), gth);
+    stage4_finish<GT>(a, bufs, lp, stage3_color<GT, CUSTOM>(a, bufs, lp, g), gth);
 }
 
-template <typename GT>
+template <typename GT, bool CUSTOM>
 __global__ __launch_bounds__(256) void k_weighted_sum_filtered(const StageArgs a, const StageBuffers b) {
-    stage34_body<GT>(a, b, xcd_tile(blockIdx.x, gridDim.x));
+    stage34_body<GT, CUSTOM>(a, b, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 template <typename GT>
@@ -436,7 +475,7 @@ __device__ __forceinline__ void stage5_body(const StageArgs &a, const StageBuffe
 constexpr int kRing = kBlock + 2;                 // 34x34 tile with the ring
 constexpr int kRingTexels = 4 * kBlock + 4;       // 132 ring texels
 
-template <typename GT>
+template <typename GT, bool CUSTOM = false>
 __device__ __forceinline__ void stage345_body(const StageArgs &a, const StageBuffers &bufs, int blk) {
     __shared__ float4 tile[kRing][kRing];
     const int W = a.width, H = a.height;
@@ -458,7 +497,7 @@ __device__ __forceinline__ void stage345_body(const StageArgs &a, const StageBuf
         if (x >= 0 && y >= 0 && x < W && y < H) {
             const int64_t lp = (int64_t)y * W + x;
             const Stage4Gather<GT> gth = stage4_gather<GT>(a, bufs, lp);
-            const float3 ac = stage4_blend<GT>(a, stage3_color<GT>(a, bufs, lp, blk), gth);
+            const float3 ac = stage4_blend<GT>(a, stage3_color<GT, CUSTOM>(a, bufs, lp, blk), gth);
             store3((GT *)bufs.out_cur, lp, ac);
             mine[r] = tone_map(gth.albedo, ac);
             pf[r] = gth.pf;
@@ -481,7 +520,7 @@ __device__ __forceinline__ void stage345_body(const StageArgs &a, const StageBuf
             const int64_t g = (int64_t)((y + kBlock / 2 - off.y) / kBlock) * a.blocks_x +
                               (x + kBlock / 2 - off.x) / kBlock;
             const Stage4Gather<GT> gth = stage4_gather<GT>(a, bufs, lp);
-            const float3 ac = stage4_blend<GT>(a, stage3_color<GT>(a, bufs, lp, g), gth);
+            const float3 ac = stage4_blend<GT>(a, stage3_color<GT, CUSTOM>(a, bufs, lp, g), gth);
             t = ycocg_texel(tone_map(gth.albedo, ac));
         }
         tile[ry + 1][rx + 1] = t;
@@ -498,9 +537,9 @@ __device__ __forceinline__ void stage345_body(const StageArgs &a, const StageBuf
     }
 }
 
-template <typename GT>
+template <typename GT, bool CUSTOM>
 __global__ __launch_bounds__(256) void k_filter_taa(const StageArgs a, const StageBuffers b) {
-    stage345_body<GT>(a, b, xcd_tile(blockIdx.x, gridDim.x));
+    stage345_body<GT, CUSTOM>(a, b, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 template <typename GT>
@@ -596,7 +635,10 @@ __global__ __launch_bounds__(256) void kb_image_stage(const BatchLaunch L) {
 template <typename GT, typename TT>
 static hipError_t launch_s1(const StageArgs &a, const StageBuffers &b, bool wn, hipStream_t s) {
     dim3 grid((a.margin_width / 32) * (a.margin_height / 8));
-    hipLaunchKernelGGL((k_accumulate_noisy_data<GT, TT>), grid, dim3(256), 0, s, a, b, wn ? 1 : 0);
+    if (a.features.count)
+        hipLaunchKernelGGL((k_accumulate_noisy_data<GT, TT, true>), grid, dim3(256), 0, s, a, b, wn ? 1 : 0);
+    else
+        hipLaunchKernelGGL((k_accumulate_noisy_data<GT, TT, false>), grid, dim3(256), 0, s, a, b, wn ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -608,16 +650,19 @@ hipError_t launch_accumulate_noisy_data(const StageArgs &a, const StageBuffers &
     return a.tmp_f16 ? launch_s1<float, __half>(a, b, wn, s) : launch_s1<float, float>(a, b, wn, s);
 }
 
-__global__ __launch_bounds__(256) void k_noise_table(int frame, double noise_amount,
+// The noise table of N features: (N-1) x 1024 terms, buffer count N + 3.
+__global__ __launch_bounds__(256) void k_noise_table(int frame, double noise_amount, int features,
                                                      double *__restrict__ noise) {
     const int lin = blockIdx.x * 256 + threadIdx.x;
-    if (lin < kNoiseTable)
-        noise[lin] = noise_term(lin & (kBlockPixels - 1), 1 + (lin >> 10), frame, noise_amount);
+    if (lin < (features - 1) * kBlockPixels)
+        noise[lin] = noise_term(lin & (kBlockPixels - 1), 1 + (lin >> 10), frame, noise_amount, features + 3);
 }
 
 hipError_t launch_noise_table(const StageArgs &a, const StageBuffers &b, hipStream_t s) {
-    hipLaunchKernelGGL(k_noise_table, dim3(kNoiseTable / 256), dim3(256), 0, s, a.frame,
-                       a.noise_amount, (double *)b.noise);
+    const int nf = a.features.count ? a.features.count : kFeatures;
+    if (nf < 2) return hipSuccess;   // one feature: no noised column
+    hipLaunchKernelGGL(k_noise_table, dim3((nf - 1) * kBlockPixels / 256), dim3(256), 0, s, a.frame,
+                       a.noise_amount, nf, (double *)b.noise);
     return hipGetLastError();
 }
 
@@ -628,12 +673,32 @@ static inline dim3 block_grid(const StageArgs &a) {
     return dim3((a.margin_width / 32) * (a.margin_height / 8));
 }
 
-hipError_t launch_weighted_sum(const StageArgs &a, const StageBuffers &b, hipStream_t s) {
-    if (a.storage_f16)
-        hipLaunchKernelGGL((k_weighted_sum<__half>), block_grid(a), dim3(256), 0, s, a, b);
+// Launches kernel K over (G-buffer type, configured feature set or not).
+enum { K_WEIGHTED_SUM, K_WEIGHTED_SUM_FILTERED, K_FILTER_TAA };
+template <int K, typename GT, bool CUSTOM>
+static void launch_one(const StageArgs &a, dim3 grid, const StageBuffers &b, hipStream_t s) {
+    if constexpr (K == K_WEIGHTED_SUM)
+        hipLaunchKernelGGL((k_weighted_sum<GT, CUSTOM>), grid, dim3(256), 0, s, a, b);
+    else if constexpr (K == K_WEIGHTED_SUM_FILTERED)
+        hipLaunchKernelGGL((k_weighted_sum_filtered<GT, CUSTOM>), grid, dim3(256), 0, s, a, b);
     else
-        hipLaunchKernelGGL((k_weighted_sum<float>), block_grid(a), dim3(256), 0, s, a, b);
+        hipLaunchKernelGGL((k_filter_taa<GT, CUSTOM>), grid, dim3(256), 0, s, a, b);
+}
+template <int K>
+static hipError_t launch_gt_set(const StageArgs &a, dim3 grid, const StageBuffers &b, hipStream_t s) {
+    const bool custom = a.features.count != 0;
+    if (a.storage_f16) {
+        if (custom) launch_one<K, __half, true>(a, grid, b, s);
+        else launch_one<K, __half, false>(a, grid, b, s);
+    } else {
+        if (custom) launch_one<K, float, true>(a, grid, b, s);
+        else launch_one<K, float, false>(a, grid, b, s);
+    }
     return hipGetLastError();
+}
+
+hipError_t launch_weighted_sum(const StageArgs &a, const StageBuffers &b, hipStream_t s) {
+    return launch_gt_set<K_WEIGHTED_SUM>(a, block_grid(a), b, s);
 }
 
 hipError_t launch_accumulate_filtered_data(const StageArgs &a, const StageBuffers &b, hipStream_t s) {
@@ -645,19 +710,11 @@ hipError_t launch_accumulate_filtered_data(const StageArgs &a, const StageBuffer
 }
 
 hipError_t launch_weighted_sum_filtered(const StageArgs &a, const StageBuffers &b, hipStream_t s) {
-    if (a.storage_f16)
-        hipLaunchKernelGGL((k_weighted_sum_filtered<__half>), block_grid(a), dim3(256), 0, s, a, b);
-    else
-        hipLaunchKernelGGL((k_weighted_sum_filtered<float>), block_grid(a), dim3(256), 0, s, a, b);
-    return hipGetLastError();
+    return launch_gt_set<K_WEIGHTED_SUM_FILTERED>(a, block_grid(a), b, s);
 }
 
 hipError_t launch_filter_taa(const StageArgs &a, const StageBuffers &b, hipStream_t s) {
-    if (a.storage_f16)
-        hipLaunchKernelGGL((k_filter_taa<__half>), dim3(a.blocks), dim3(256), 0, s, a, b);
-    else
-        hipLaunchKernelGGL((k_filter_taa<float>), dim3(a.blocks), dim3(256), 0, s, a, b);
-    return hipGetLastError();
+    return launch_gt_set<K_FILTER_TAA>(a, dim3(a.blocks), b, s);
 }
 
 hipError_t launch_taa(const StageArgs &a, const StageBuffers &b, hipStream_t s) {

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define BMFR_API_VERSION 1
+#define BMFR_API_VERSION 2
 
 typedef enum bmfr_status {
     BMFR_OK = 0,
@@ -49,6 +49,27 @@ typedef enum bmfr_qr_mode {
                                  whose min |R_ii| < refactor_threshold*max |R_ii| */
 } bmfr_qr_mode;
 
+/* Feature sets (opencl/bmfr.cpp:63-77, 193-214).  The reference configures
+ * its fitted features as OpenCL expression text, NOT_SCALED_FEATURE_BUFFERS
+ * then SCALED_FEATURE_BUFFERS, over the variables `normal` and
+ * `world_position` that accumulate_noisy_data and weighted_sum both hold
+ * (bmfr.cl:448-453, 725-729).  Here a feature is the product of two factors
+ * of that pixel, feature = factor[0] * factor[1] (BMFR_FACTOR_ONE for a
+ * single term: x * 1 is exact), which spans the reference's default set and
+ * every polynomial of degree <= 2 in those six channels.  The first
+ * `features_not_scaled` features enter the fit as they are; the others are
+ * scaled to [0, 1] per block (bmfr.cl:511-542). */
+#define BMFR_MAX_FEATURES 16
+typedef enum bmfr_factor {
+    BMFR_FACTOR_ONE = 0,
+    BMFR_FACTOR_NORMAL_X = 1,
+    BMFR_FACTOR_NORMAL_Y = 2,
+    BMFR_FACTOR_NORMAL_Z = 3,
+    BMFR_FACTOR_POSITION_X = 4,
+    BMFR_FACTOR_POSITION_Y = 5,
+    BMFR_FACTOR_POSITION_Z = 6
+} bmfr_factor;
+
 /* Runtime form of the reference's -D options (opencl/bmfr.cpp:56-118, 205-232). */
 typedef struct bmfr_params {
     int32_t width;                  /* IMAGE_WIDTH */
@@ -65,10 +86,20 @@ typedef struct bmfr_params {
                                        F16 is USE_HALF_PRECISION_IN_TMP_DATA=1 */
     int32_t qr_mode;                /* bmfr_qr_mode */
     float refactor_threshold;       /* used by BMFR_QR_KAHAN_FP64 (1e-6) */
+    /* Fitted features, BUFFER_COUNT - 3 of bmfr.cpp:204 (1..BMFR_MAX_FEATURES);
+     * 0 selects the reference's set: 1, normal.xyz not scaled, then
+     * world_position.xyz and its squares scaled (bmfr.cpp:65-77).  Another set
+     * runs with qr_mode BMFR_QR_FP32 on single streams (not in batches or
+     * servers). */
+    int32_t feature_count;
+    int32_t features_not_scaled;    /* FEATURES_NOT_SCALED (0..feature_count) */
+    uint8_t feature_factors[BMFR_MAX_FEATURES][2];   /* bmfr_factor pairs */
 } bmfr_params;
 
 /* Buffer ids for the debug/parity accessors.  Element types: F = float or
- * half per `storage`, T = float or half per `tmp_storage`, u8 = uint8. */
+ * half per `storage`, T = float or half per `tmp_storage`, u8 = uint8.  With
+ * N features (10 for the reference's set) of which S are scaled (6), a
+ * block holds N + 3 tmp_data columns, 3N weights and 2S mins_maxs. */
 typedef enum bmfr_buffer {
     BMFR_BUF_ALBEDO = 0,          /* float[3P]   albedo_buffer */
     BMFR_BUF_NORMALS_CUR = 1,     /* F[3P]       normals_buffer.current() */
@@ -82,9 +113,9 @@ typedef enum bmfr_buffer {
     BMFR_BUF_SPP_PREV = 9,        /* u8[P] */
     BMFR_BUF_PREV_PIXEL = 10,     /* float2[P]   prev_pixels_buffer */
     BMFR_BUF_ACCEPT = 11,         /* u8[P]       accept_buffer */
-    BMFR_BUF_TMP_DATA = 12,       /* T[13Q]      in_buffer (block-major) */
-    BMFR_BUF_WEIGHTS = 13,        /* float[30 nB] */
-    BMFR_BUF_MINS_MAXS = 14,      /* float[12 nB] */
+    BMFR_BUF_TMP_DATA = 12,       /* T[(N+3)Q]   in_buffer (block-major) */
+    BMFR_BUF_WEIGHTS = 13,        /* float[3N nB] */
+    BMFR_BUF_MINS_MAXS = 14,      /* float[2S nB] */
     BMFR_BUF_FILTERED = 15,       /* float[3P] */
     BMFR_BUF_TONE_MAPPED = 16,    /* float[3P] */
     BMFR_BUF_OUT_CUR = 17,        /* F[3P]       out_buffer.current() */
@@ -115,8 +146,16 @@ const char *bmfr_last_error(void);
 int32_t bmfr_device_count(void);
 
 /* Reference defaults (opencl/bmfr.cpp:58-62, 88) for a W x H stream:
- * F32 storage, F16 tmp_data, plain FP32 QR. */
+ * F32 storage, F16 tmp_data, plain FP32 QR, the reference's feature set. */
 int32_t bmfr_default_params(int32_t width, int32_t height, bmfr_params *out);
+/* Sets params' feature set from the reference's two feature strings
+ * (NOT_SCALED_FEATURE_BUFFERS, SCALED_FEATURE_BUFFERS, opencl/bmfr.cpp:65-77;
+ * counted like bmfr.cpp:193-201).  Comma-separated terms; a term is `1.f`
+ * (also `1`, `1.0`, `1.0f`), `normal.x|y|z`, `world_position.x|y|z`, or the
+ * product `a*b` of two of those; spaces are ignored and an empty term (the
+ * reference's trailing comma) is skipped.  Anything else, or more than
+ * BMFR_MAX_FEATURES terms, is BMFR_ERR_INVALID_ARGUMENT naming the term. */
+int32_t bmfr_parse_feature_buffers(const char *not_scaled, const char *scaled, bmfr_params *params);
 /* Validates params and fills sizes.  W and H must be >= 47: stage 1 mirrors
  * padded-grid coordinates up to WORKSET+29 back into the image
  * (opencl/bmfr.cl:209-216, 312-316), which needs WORKSET+30 <= 2*size. */

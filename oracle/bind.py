@@ -26,6 +26,9 @@ class OrcParams(ctypes.Structure):
         ("taa_blend_alpha", ctypes.c_float),
         ("noise_amount", ctypes.c_double),
         ("half_tmp", ctypes.c_int32),
+        ("feature_count", ctypes.c_int32),          # 0: the reference's set
+        ("features_not_scaled", ctypes.c_int32),
+        ("feature_factors", (ctypes.c_uint8 * 2) * 16),
     ]
 
 
@@ -63,6 +66,12 @@ def load():
         "orc_random": (ctypes.c_float, [ctypes.c_uint32]),
         "orc_add_random": (ctypes.c_float, [ctypes.c_float, ctypes.c_int32, ctypes.c_int32,
                                             ctypes.c_int32, ctypes.c_int32, ctypes.c_double]),
+        "orc_add_random_n": (ctypes.c_float, [ctypes.c_float, ctypes.c_int32, ctypes.c_int32,
+                                              ctypes.c_int32, ctypes.c_int32, ctypes.c_double,
+                                              ctypes.c_int32]),
+        "orc_feature_count": (ctypes.c_int32, [P]),
+        "orc_not_scaled": (ctypes.c_int32, [P]),
+        "orc_feature": (ctypes.c_float, [P, ctypes.c_int32, _P, _P]),
         "orc_f32_to_f16": (ctypes.c_uint16, [ctypes.c_float]),
         "orc_f16_to_f32": (ctypes.c_float, [ctypes.c_uint16]),
         "orc_scale": (ctypes.c_float, [ctypes.c_float, ctypes.c_float, ctypes.c_float]),
@@ -93,13 +102,49 @@ def _ptr(a):
     return a.ctypes.data
 
 
-def params(width, height, half_tmp=True, **kw):
+# factor ids of a feature term: 1, normal.xyz, world_position.xyz
+FACTORS = {"1": 0, "n.x": 1, "n.y": 2, "n.z": 3, "p.x": 4, "p.y": 5, "p.z": 6}
+
+
+def params(width, height, half_tmp=True, features=None, not_scaled=None, **kw):
+    """Oracle parameters; `features` (optional) is a list of terms such as
+    "1", "n.y", "p.x*p.z" (at most 16), the first `not_scaled` unscaled."""
     p = OrcParams()
     load().orc_default_params(width, height, ctypes.byref(p))
     p.half_tmp = 1 if half_tmp else 0
+    if features is not None:
+        set_features(p, features, not_scaled)
     for k, v in kw.items():
         setattr(p, k, v)
     return p
+
+
+def factor_pairs(features):
+    """[(a, b)] factor ids of feature terms ("p.x*p.y" -> (4, 5), "n.z" -> (3, 0))."""
+    pairs = []
+    for t in features:
+        f = t.split("*")
+        assert 1 <= len(f) <= 2, t
+        pairs.append((FACTORS[f[0]], FACTORS[f[1]] if len(f) == 2 else 0))
+    return pairs
+
+
+def set_features(p, features, not_scaled):
+    pairs = factor_pairs(features)
+    assert 1 <= len(pairs) <= 16 and 0 <= not_scaled <= len(pairs)
+    p.feature_count = len(pairs)
+    p.features_not_scaled = not_scaled
+    for i, (a, b) in enumerate(pairs):
+        p.feature_factors[i][0] = a
+        p.feature_factors[i][1] = b
+
+
+def feature_count(p):
+    return p.feature_count or 10
+
+
+def scaled_count(p):
+    return p.feature_count - p.features_not_scaled if p.feature_count else 6
 
 
 def sizes(width, height):
@@ -127,9 +172,10 @@ class OracleStream:
         self.spp = [np.zeros(P, np.uint8), np.zeros(P, np.uint8)]
         self.prev_pixel = np.zeros(2 * P, np.float32)
         self.accept = np.zeros(P, np.uint8)
-        self.tmp = np.zeros(13 * s.padded, np.float32)
-        self.weights = np.zeros(30 * s.blocks, np.float32)
-        self.mins_maxs = np.zeros(12 * s.blocks, np.float32)
+        nf, ns = feature_count(p), scaled_count(p)
+        self.tmp = np.zeros((nf + 3) * s.padded, np.float32)
+        self.weights = np.zeros(3 * nf * s.blocks, np.float32)
+        self.mins_maxs = np.zeros(max(2 * ns, 1) * s.blocks, np.float32)
         self.filtered = z3()
         self.tone = z3()
         self.out = [z3(), z3()]
@@ -201,10 +247,10 @@ class OracleStream:
 
 
 def fitter_inputs(p, tmp, block, frame):
-    """(A[1024,10], b[1024,3], mins_maxs[12]) as the QR sees them."""
-    A = np.zeros((1024, 10), np.float64)
+    """(A[1024,N], b[1024,3], mins_maxs[2S]) as the QR sees them."""
+    A = np.zeros((1024, feature_count(p)), np.float64)
     b = np.zeros((1024, 3), np.float64)
-    mm = np.zeros(12, np.float32)
+    mm = np.zeros(max(2 * scaled_count(p), 1), np.float32)
     load().orc_fitter_inputs(ctypes.byref(p), _ptr(tmp), block, frame, _ptr(A), _ptr(b), _ptr(mm))
     return A, b, mm
 

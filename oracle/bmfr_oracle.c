@@ -53,6 +53,38 @@ void orc_default_params(int32_t w, int32_t h, orc_params *p) {
     p->taa_blend_alpha = 0.2f;
     p->noise_amount = 1e-2;
     p->half_tmp = 1;
+    p->feature_count = 0;
+    p->features_not_scaled = 0;
+    memset(p->feature_factors, 0, sizeof(p->feature_factors));
+}
+
+int32_t orc_feature_count(const orc_params *p) { return p->feature_count ? p->feature_count : ORC_FEATURES; }
+int32_t orc_not_scaled(const orc_params *p) { return p->feature_count ? p->features_not_scaled : ORC_NOT_SCALED; }
+
+static float orc_factor(int32_t id, const float n[3], const float wp[3]) {
+    switch (id) {
+        case 1: return n[0];
+        case 2: return n[1];
+        case 3: return n[2];
+        case 4: return wp[0];
+        case 5: return wp[1];
+        case 6: return wp[2];
+        default: return 1.f;
+    }
+}
+
+float orc_feature(const orc_params *p, int32_t f, const float n[3], const float wp[3]) {
+    if (p->feature_count == 0) {
+        /* bmfr.cpp:65-77: 1.f, normal.xyz, world_position.xyz, its squares */
+        switch (f) {
+            case 0: return 1.f;
+            case 1: case 2: case 3: return n[f - 1];
+            case 4: case 5: case 6: return wp[f - 4];
+            default: return wp[f - 7] * wp[f - 7];
+        }
+    }
+    /* an expression `a*b` is one rounded float product; `a` alone is a*1 */
+    return orc_factor(p->feature_factors[f][0], n, wp) * orc_factor(p->feature_factors[f][1], n, wp);
 }
 
 /* bmfr.cl:267-285: 16-entry jitter table, indexed by frame % 16. */
@@ -94,17 +126,22 @@ float orc_random(uint32_t a) {
     return (float)orc_hash(a) / (float)4294967295u;
 }
 
-float orc_add_random(float value, int32_t id, int32_t sub_vector, int32_t feature,
-                     int32_t frame, double noise_amount) {
+float orc_add_random_n(float value, int32_t id, int32_t sub_vector, int32_t feature,
+                       int32_t frame, double noise_amount, int32_t buffer_count) {
     /* bmfr.cl:173-182.  NOISE_AMOUNT is the double literal 1e-2, so
      * NOISE_AMOUNT * 2.f * (r - 0.5f) is evaluated in double, added to the
      * float value in double, and rounded to float on return. */
     uint32_t arg = (uint32_t)id + (uint32_t)sub_vector * ORC_LOCAL +
                    (uint32_t)feature * ORC_BLOCK_PIXELS +
-                   (uint32_t)frame * ORC_BUFFER_COUNT * ORC_BLOCK_PIXELS;
+                   (uint32_t)frame * (uint32_t)buffer_count * ORC_BLOCK_PIXELS;
     float r = orc_random(arg) - 0.5f;
     double noise = noise_amount * 2.0 * (double)r;
     return (float)((double)value + noise);
+}
+
+float orc_add_random(float value, int32_t id, int32_t sub_vector, int32_t feature,
+                     int32_t frame, double noise_amount) {
+    return orc_add_random_n(value, id, sub_vector, feature, frame, noise_amount, ORC_BUFFER_COUNT);
 }
 
 uint16_t orc_f32_to_f16(float f) {
@@ -256,6 +293,7 @@ int32_t orc_accumulate_noisy_data(
     orc_block_offset(frame_number, off);
     const float pos_lim = p->position_limit_squared;
     const float nrm_lim = p->normal_limit_squared;
+    const int32_t NF = orc_feature_count(p), NB = NF + 3;
 
 #pragma omp parallel for schedule(static)
     for (int32_t gy = 0; gy < s.margin_h; ++gy) {
@@ -351,16 +389,18 @@ int32_t orc_accumulate_noisy_data(
             float nc[3];
             for (int c = 0; c < 3; ++c) nc[c] = blend_alpha * cur[c] + omb * prev_col[c];
 
-            /* bmfr.cl:448-476 */
-            const float feat[ORC_BUFFER_COUNT] = {
-                1.f, nrm[0], nrm[1], nrm[2], wp[0], wp[1], wp[2],
-                wp[0] * wp[0], wp[1] * wp[1], wp[2] * wp[2], nc[0], nc[1], nc[2]};
+            /* bmfr.cl:448-476: FEATURE_BUFFERS, then the colour */
+            float feat[ORC_MAX_FEATURES + 3];
+            for (int f = 0; f < NF; ++f) feat[f] = orc_feature(p, f, nrm, wp);
+            feat[NF] = nc[0];
+            feat[NF + 1] = nc[1];
+            feat[NF + 2] = nc[2];
             const int32_t xin = gx % ORC_BLOCK, yin = gy % ORC_BLOCK;
             const int32_t xb = gx / ORC_BLOCK, yb = gy / ORC_BLOCK;
             const int64_t base = (int64_t)xin + (int64_t)yin * ORC_BLOCK +
-                                 (int64_t)xb * ORC_BLOCK_PIXELS * ORC_BUFFER_COUNT +
-                                 (int64_t)yb * s.blocks_x * ORC_BLOCK_PIXELS * ORC_BUFFER_COUNT;
-            for (int f = 0; f < ORC_BUFFER_COUNT; ++f) {
+                                 (int64_t)xb * ORC_BLOCK_PIXELS * NB +
+                                 (int64_t)yb * s.blocks_x * ORC_BLOCK_PIXELS * NB;
+            for (int f = 0; f < NB; ++f) {
                 float v = feat[f];
                 if (v != v) v = 0.f;
                 if (p->half_tmp) {
@@ -391,14 +431,16 @@ static inline float st_round(const orc_params *p, float v) {
 
 static void fitter_block(const orc_params *p, float *d, float *mins_maxs, float *weights,
                          int32_t blk, int32_t frame) {
+    const int NF = orc_feature_count(p), NNS = orc_not_scaled(p);
+    const int NB = NF + 3, NS = NF - NNS, RE = NF + 1;   /* BUFFER_COUNT, FEATURES_SCALED, R_EDGE */
     float sum_vec[ORC_LOCAL];
     float u_vec[ORC_BLOCK_PIXELS];
-    float R[ORC_R_EDGE][ORC_R_EDGE][3];
+    float R[ORC_MAX_FEATURES + 1][ORC_MAX_FEATURES + 1][3];
     memset(R, 0, sizeof(R));
 #define IN(f, sv, id) d[(f) * ORC_BLOCK_PIXELS + (sv) * ORC_LOCAL + (id)]
 
     /* Scaling, bmfr.cl:511-542 */
-    for (int f = ORC_NOT_SCALED; f < ORC_BUFFER_COUNT - 3; ++f) {
+    for (int f = NNS; f < NB - 3; ++f) {
         float tmax[ORC_LOCAL], tmin[ORC_LOCAL];
         for (int id = 0; id < ORC_LOCAL; ++id) {
             float mx = -INFINITY, mn = INFINITY;
@@ -414,7 +456,7 @@ static void fitter_block(const orc_params *p, float *d, float *mins_maxs, float 
         const float bmax = orc_tree_max(sum_vec);
         memcpy(sum_vec, tmin, sizeof(sum_vec));
         const float bmin = orc_tree_min(sum_vec);
-        const int idx = (blk * ORC_SCALED + f - ORC_NOT_SCALED) * 2;
+        const int idx = (blk * NS + f - NNS) * 2;
         mins_maxs[idx + 0] = bmin;
         mins_maxs[idx + 1] = bmax;
         for (int i = 0; i < ORC_BLOCK_PIXELS; ++i)
@@ -422,8 +464,8 @@ static void fitter_block(const orc_params *p, float *d, float *mins_maxs, float 
     }
 
     /* Householder QR, bmfr.cl:546-656 */
-    for (int col = 0; col < ORC_BUFFER_COUNT; ++col) {
-        const int cl = col < ORC_BUFFER_COUNT - 3 ? col : ORC_BUFFER_COUNT - 3;
+    for (int col = 0; col < NB; ++col) {
+        const int cl = col < NB - 3 ? col : NB - 3;
         for (int id = 0; id < ORC_LOCAL; ++id) {
             float acc = 0.f;
             for (int sv = 0; sv < 4; ++sv) {
@@ -436,23 +478,23 @@ static void fitter_block(const orc_params *p, float *d, float *mins_maxs, float 
         }
         float vec_length = orc_tree_sum(sum_vec);
         /* R entries above the pivot are the column's current values
-         * (bmfr.cl:575-579); row 10 of the colour columns is never read
+         * (bmfr.cl:575-579); row N of the colour columns is never read
          * by the back substitution and is skipped. */
-        if (col < ORC_BUFFER_COUNT - 3) {
+        if (col < NB - 3) {
             for (int r = 0; r < col; ++r)
                 R[r][col][0] = R[r][col][1] = R[r][col][2] = u_vec[r];
         } else {
-            for (int r = 0; r < ORC_R_EDGE - 1; ++r) R[r][ORC_R_EDGE - 1][col - 10] = u_vec[r];
+            for (int r = 0; r < RE - 1; ++r) R[r][RE - 1][col - NF] = u_vec[r];
         }
         /* work-item `col`, bmfr.cl:580-587 */
         float u_length_squared = vec_length;
         vec_length = sqrtf(vec_length + u_vec[cl] * u_vec[cl]);
         u_vec[cl] = u_vec[cl] - vec_length;
         u_length_squared = u_length_squared + u_vec[cl] * u_vec[cl];
-        if (col < ORC_BUFFER_COUNT - 3) R[col][col][0] = R[col][col][1] = R[col][col][2] = vec_length;
+        if (col < NB - 3) R[col][col][0] = R[col][col][1] = R[col][col][2] = vec_length;
 
         /* trailing columns, bmfr.cl:606-655 (CACHE_TMP_DATA=1) */
-        for (int fb = cl + 1; fb < ORC_BUFFER_COUNT; ++fb) {
+        for (int fb = cl + 1; fb < NB; ++fb) {
             float cache[ORC_LOCAL][4];
             for (int id = 0; id < ORC_LOCAL; ++id) {
                 float acc = 0.f;
@@ -460,8 +502,8 @@ static void fitter_block(const orc_params *p, float *d, float *mins_maxs, float 
                     const int index = id + sv * ORC_LOCAL;
                     if (index >= cl) {
                         float t = IN(fb, sv, id);
-                        if (col == 0 && fb < ORC_BUFFER_COUNT - 3)
-                            t = orc_add_random(t, id, sv, fb, frame, p->noise_amount);
+                        if (col == 0 && fb < NB - 3)
+                            t = orc_add_random_n(t, id, sv, fb, frame, p->noise_amount, NB);
                         cache[id][sv] = t;
                         acc = acc + t * u_vec[index];
                     }
@@ -486,16 +528,16 @@ static void fitter_block(const orc_params *p, float *d, float *mins_maxs, float 
 
     /* Back substitution, bmfr.cl:658-692, per colour channel. */
     for (int ch = 0; ch < 3; ++ch) {
-        for (int i = ORC_R_EDGE - 2; i >= 0; --i) {
+        for (int i = RE - 2; i >= 0; --i) {
             const float divider = R[i][i][ch];
-            for (int c = i; c < ORC_R_EDGE; ++c) R[i][c][ch] = R[i][c][ch] / divider;
-            for (int j = i + 1; j < ORC_R_EDGE - 1; ++j)
-                R[i][ORC_R_EDGE - 1][ch] = R[i][ORC_R_EDGE - 1][ch] - R[i][j][ch];
-            const float wi = R[i][ORC_R_EDGE - 1][ch];
+            for (int c = i; c < RE; ++c) R[i][c][ch] = R[i][c][ch] / divider;
+            for (int j = i + 1; j < RE - 1; ++j)
+                R[i][RE - 1][ch] = R[i][RE - 1][ch] - R[i][j][ch];
+            const float wi = R[i][RE - 1][ch];
             for (int r = 0; r <= i; ++r) R[r][i][ch] = R[r][i][ch] * wi;
         }
-        for (int f = 0; f < ORC_FEATURES; ++f)
-            weights[((int64_t)blk * ORC_FEATURES + f) * 3 + ch] = R[f][ORC_R_EDGE - 1][ch];
+        for (int f = 0; f < NF; ++f)
+            weights[((int64_t)blk * NF + f) * 3 + ch] = R[f][RE - 1][ch];
     }
 #undef IN
 }
@@ -504,17 +546,18 @@ void orc_fitter(const orc_params *p, float *tmp_data, float *mins_maxs, float *w
                 int32_t frame_number, int32_t blocks_first, int32_t blocks_count) {
 #pragma omp parallel for schedule(dynamic, 4)
     for (int32_t b = blocks_first; b < blocks_first + blocks_count; ++b)
-        fitter_block(p, tmp_data + (int64_t)b * ORC_BUFFER_COUNT * ORC_BLOCK_PIXELS, mins_maxs,
+        fitter_block(p, tmp_data + (int64_t)b * (orc_feature_count(p) + 3) * ORC_BLOCK_PIXELS, mins_maxs,
                      weights, b, frame_number);
 }
 
 void orc_fitter_inputs(const orc_params *p, const float *tmp_data, int32_t block,
                        int32_t frame, double *A, double *b, float *mm) {
-    const float *d = tmp_data + (int64_t)block * ORC_BUFFER_COUNT * ORC_BLOCK_PIXELS;
+    const int NF = orc_feature_count(p), NNS = orc_not_scaled(p), NB = NF + 3;
+    const float *d = tmp_data + (int64_t)block * NB * ORC_BLOCK_PIXELS;
     float sum_vec[ORC_LOCAL];
-    for (int f = 0; f < ORC_BUFFER_COUNT; ++f) {
+    for (int f = 0; f < NB; ++f) {
         float bmin = 0.f, bmax = 0.f;
-        if (f >= ORC_NOT_SCALED && f < ORC_FEATURES) {
+        if (f >= NNS && f < NF) {
             float tmax[ORC_LOCAL], tmin[ORC_LOCAL];
             for (int id = 0; id < ORC_LOCAL; ++id) {
                 float mx = -INFINITY, mn = INFINITY;
@@ -530,18 +573,18 @@ void orc_fitter_inputs(const orc_params *p, const float *tmp_data, int32_t block
             bmax = orc_tree_max(sum_vec);
             memcpy(sum_vec, tmin, sizeof(sum_vec));
             bmin = orc_tree_min(sum_vec);
-            mm[(f - ORC_NOT_SCALED) * 2 + 0] = bmin;
-            mm[(f - ORC_NOT_SCALED) * 2 + 1] = bmax;
+            mm[(f - NNS) * 2 + 0] = bmin;
+            mm[(f - NNS) * 2 + 1] = bmax;
         }
         for (int i = 0; i < ORC_BLOCK_PIXELS; ++i) {
             float v = d[f * ORC_BLOCK_PIXELS + i];
-            if (f >= ORC_NOT_SCALED && f < ORC_FEATURES) v = st_round(p, orc_scale(v, bmin, bmax));
-            if (f >= 1 && f < ORC_FEATURES)
-                v = orc_add_random(v, i % ORC_LOCAL, i / ORC_LOCAL, f, frame, p->noise_amount);
-            if (f < ORC_FEATURES)
-                A[(int64_t)i * ORC_FEATURES + f] = (double)v;
+            if (f >= NNS && f < NF) v = st_round(p, orc_scale(v, bmin, bmax));
+            if (f >= 1 && f < NF)
+                v = orc_add_random_n(v, i % ORC_LOCAL, i / ORC_LOCAL, f, frame, p->noise_amount, NB);
+            if (f < NF)
+                A[(int64_t)i * NF + f] = (double)v;
             else
-                b[(int64_t)i * 3 + (f - ORC_FEATURES)] = (double)v;
+                b[(int64_t)i * 3 + (f - NF)] = (double)v;
         }
     }
 }
@@ -599,6 +642,7 @@ void orc_weighted_sum(const orc_params *p, const float *weights, const float *mi
     orc_sizes s;
     orc_sizes_of(p->width, p->height, &s);
     const int32_t W = p->width, H = p->height;
+    const int32_t NF = orc_feature_count(p), NNS = orc_not_scaled(p);
     int32_t off[2];
     orc_block_offset(frame_number, off);
 #pragma omp parallel for schedule(static)
@@ -613,17 +657,14 @@ void orc_weighted_sum(const orc_params *p, const float *weights, const float *mi
                                  current_positions[lp * 3 + 2]};
             const float n[3] = {current_normals[lp * 3 + 0], current_normals[lp * 3 + 1],
                                 current_normals[lp * 3 + 2]};
-            const float feat[ORC_FEATURES] = {1.f,   n[0],  n[1],          n[2],
-                                              wp[0], wp[1], wp[2],         wp[0] * wp[0],
-                                              wp[1] * wp[1], wp[2] * wp[2]};
             float col[3] = {0.f, 0.f, 0.f};
-            for (int f = 0; f < ORC_FEATURES; ++f) { /* bmfr.cl:733-747 */
-                float v = feat[f];
-                if (f >= ORC_NOT_SCALED) {
-                    const int64_t mi = (g * ORC_SCALED + f - ORC_NOT_SCALED) * 2;
+            for (int f = 0; f < NF; ++f) { /* bmfr.cl:727-747 */
+                float v = orc_feature(p, f, n, wp);
+                if (f >= NNS) {
+                    const int64_t mi = (g * (NF - NNS) + f - NNS) * 2;
                     v = orc_scale(v, mins_maxs[mi + 0], mins_maxs[mi + 1]);
                 }
-                const float *wt = weights + (g * ORC_FEATURES + f) * 3;
+                const float *wt = weights + (g * NF + f) * 3;
                 col[0] = col[0] + wt[0] * v;
                 col[1] = col[1] + wt[1] * v;
                 col[2] = col[2] + wt[2] * v;

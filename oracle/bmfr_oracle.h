@@ -36,6 +36,7 @@ extern "C" {
 #define ORC_NOT_SCALED 4         /* 1, n.x, n.y, n.z */
 #define ORC_SCALED 6             /* p.xyz, p.xyz^2 */
 #define ORC_R_EDGE 11            /* bmfr.cpp:221 */
+#define ORC_MAX_FEATURES 16      /* longest configurable feature set */
 #define ORC_LOCAL 256            /* bmfr.cpp:116 */
 
 typedef struct {
@@ -48,7 +49,22 @@ typedef struct {
     float taa_blend_alpha;      /* TAA_BLEND_ALPHA 0.2f */
     double noise_amount;        /* NOISE_AMOUNT 1e-2 (a double literal) */
     int32_t half_tmp;           /* USE_HALF_PRECISION_IN_TMP_DATA */
+    /* The feature set (bmfr.cpp:63-77, 193-214).  feature_count 0 is the
+     * reference's 10 (ORC_FEATURES/ORC_NOT_SCALED above).  Otherwise feature
+     * i is factor[i][0] * factor[i][1] over {1, normal.xyz,
+     * world_position.xyz} (ids 0..6), the first features_not_scaled unscaled:
+     * BUFFER_COUNT = feature_count + 3, R_EDGE = feature_count + 1. */
+    int32_t feature_count;
+    int32_t features_not_scaled;
+    uint8_t feature_factors[ORC_MAX_FEATURES][2];
 } orc_params;
+
+/* N, FEATURES_NOT_SCALED and BUFFER_COUNT of a params set. */
+int32_t orc_feature_count(const orc_params *p);
+int32_t orc_not_scaled(const orc_params *p);
+/* Feature f of a pixel with normal n and world position wp, as the
+ * reference's FEATURE_BUFFERS expression evaluates it (bmfr.cl:448-453). */
+float orc_feature(const orc_params *p, int32_t f, const float n[3], const float wp[3]);
 
 /* Derived sizes, bmfr.cpp:104-118. */
 typedef struct {
@@ -73,6 +89,9 @@ float orc_random(uint32_t a);
 /* bmfr.cl:173-182 */
 float orc_add_random(float value, int32_t id, int32_t sub_vector, int32_t feature,
                      int32_t frame, double noise_amount);
+/* The same for BUFFER_COUNT buffer_count (orc_add_random: 13). */
+float orc_add_random_n(float value, int32_t id, int32_t sub_vector, int32_t feature,
+                       int32_t frame, double noise_amount, int32_t buffer_count);
 /* IEEE binary16 conversion, round to nearest even (vstore_half). */
 uint16_t orc_f32_to_f16(float f);
 float orc_f16_to_f32(uint16_t h);
@@ -93,7 +112,7 @@ float orc_tree_max(float *sum_vec);
 
 /* Stage 1, bmfr.cl:287-485.  Returns 0, or -1 (and does nothing) when W or
  * H is below ORC_MIN_SIZE.  Buffers are AoS float3 with row stride W over
- * the first P texels; tmp_data has 13*Q floats in block-major order
+ * the first P texels; tmp_data has (N+3)*Q floats in block-major order
  * [by][bx][feature][y_in][x_in] (bmfr.cl:455-464).  In half mode the values
  * written to tmp_data are binary16-representable floats. */
 int32_t orc_accumulate_noisy_data(
@@ -107,24 +126,25 @@ int32_t orc_accumulate_noisy_data(
     float *out_prev_frame_pixel,         /* out: float2[P] */
     uint8_t *accept_bools,               /* out: uchar[P] */
     uint8_t *current_spp,                /* out: uchar[P] */
-    float *tmp_data,                     /* out: 13*Q */
+    float *tmp_data,                     /* out: (N+3)*Q */
     const float prev_frame_camera_matrix[16],
     const float pixel_offset[2],
     int32_t frame_number);
 
 /* Stage 2, bmfr.cl:487-700.  tmp_data is updated in place like the
  * reference (scaled features, then Householder updates).  mins_maxs gets
- * 12 floats per block, weights 30 floats per block ([block][feature][rgb]).
+ * 2S floats per block (S scaled features, 6 in the reference's set),
+ * weights 3N per block ([block][feature][rgb]).
  * blocks_first/blocks_count select a sub-range of blocks (for sampling). */
 void orc_fitter(const orc_params *p, float *tmp_data, float *mins_maxs,
                 float *weights, int32_t frame_number,
                 int32_t blocks_first, int32_t blocks_count);
 
 /* The fitter's regression input for one block, exactly as the QR of
- * bmfr.cl:549-656 sees it at column 0: features 4..9 scaled
- * (bmfr.cl:538-541, half-rounded in half mode), features 1..9 plus noise
- * (bmfr.cl:623-627), colour unchanged.  A is 1024x10 row-major, b 1024x3.
- * Also returns the block's 12 mins/maxs.  tmp_data is not modified. */
+ * bmfr.cl:549-656 sees it at column 0: the scaled features scaled
+ * (bmfr.cl:538-541, half-rounded in half mode), features 1..N-1 plus noise
+ * (bmfr.cl:623-627), colour unchanged.  A is 1024xN row-major, b 1024x3.
+ * Also returns the block's 2S mins/maxs.  tmp_data is not modified. */
 void orc_fitter_inputs(const orc_params *p, const float *tmp_data, int32_t block,
                        int32_t frame_number, double *A, double *b, float *mins_maxs);
 

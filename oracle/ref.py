@@ -38,7 +38,7 @@ def load():
         lib.ref_error.restype = ctypes.c_char_p
         lib.ref_device_count.restype = ctypes.c_int
         lib.ref_open.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                 ctypes.POINTER(V)]
+                                 ctypes.c_int, ctypes.c_int, ctypes.POINTER(V)]
         lib.ref_upload.argtypes = [V, V, V, V, V]
         lib.ref_stage.argtypes = [V, ctypes.c_int, ctypes.c_int, V, V]
         lib.ref_end_frame.argtypes = [V]
@@ -51,18 +51,34 @@ def load():
     return _lib
 
 
-def code_object(w, h, half):
-    return os.path.join(REF_DIR, f"bmfr_{w}x{h}_h{1 if half else 0}.co")
+# Feature sets that oracle/Makefile also compiles the reference for (the
+# FEATURE_BUFFERS strings of its -D options): name -> (factor pairs,
+# not_scaled) in the oracle's ids (1, normal.xyz, world_position.xyz).
+FEATURE_SETS = {
+    "cross5": ([(0, 0), (2, 0), (4, 5), (6, 0), (4, 6)], 2),
+    "quad16": ([(0, 0), (1, 0), (2, 0), (3, 0), (4, 0), (5, 0), (6, 0), (4, 4), (5, 5), (6, 6),
+                (4, 5), (5, 6), (4, 6), (1, 4), (2, 5), (3, 6)], 4),
+}
+
+
+def code_object(w, h, half, feature_set=None):
+    fs = f"_{feature_set}" if feature_set else ""
+    return os.path.join(REF_DIR, f"bmfr_{w}x{h}_h{1 if half else 0}{fs}.co")
 
 
 class RefRun:
     """The reference's frame chain on the GPU, stage by stage."""
 
-    def __init__(self, w, h, half=True):
+    def __init__(self, w, h, half=True, feature_set=None):
         lib = load()
         self.lib, self.w, self.h, self.half = lib, w, h, half
+        n, ns = 10, 4
+        if feature_set:
+            pairs, ns = FEATURE_SETS[feature_set]
+            n = len(pairs)
         hd = ctypes.c_void_p()
-        rc = lib.ref_open(code_object(w, h, half).encode(), w, h, 1 if half else 0, ctypes.byref(hd))
+        rc = lib.ref_open(code_object(w, h, half, feature_set).encode(), w, h, 1 if half else 0, n, n - ns,
+                          ctypes.byref(hd))
         if rc != 0:
             raise RuntimeError(f"ref_open: {lib.ref_error().decode()}")
         self.h_ = hd

@@ -120,7 +120,10 @@ int ref_device_count(void) {
     return (int)total;
 }
 
-int ref_open(const char *code_object, int W, int H, int half, void **out) {
+// features / scaled: the code object's BUFFER_COUNT - 3 and FEATURES_SCALED
+// (10 and 6 for the reference's set), which size tmp_data, weights, mins_maxs
+// and the local R matrix (opencl/bmfr.cpp:321-343, 354-358).
+int ref_open(const char *code_object, int W, int H, int half, int features, int scaled, void **out) {
     *out = nullptr;
     FILE *f = fopen(code_object, "rb");
     if (!f) {
@@ -198,13 +201,13 @@ int ref_open(const char *code_object, int W, int H, int half, void **out) {
         {&r->normals.m[0], OUT * 12, -1}, {&r->normals.m[1], OUT * 12, -1},
         {&r->positions.m[0], OUT * 12, -1}, {&r->positions.m[1], OUT * 12, -1},
         {&r->noisy.m[0], OUT * 12, -1}, {&r->noisy.m[1], OUT * 12, -1},
-        {&r->in_buffer, MG * 13 * (half ? 2 : 4), R_TMP}, {&r->filtered, OUT * 12, R_FILTERED},
+        {&r->in_buffer, MG * (features + 3) * (half ? 2 : 4), R_TMP}, {&r->filtered, OUT * 12, R_FILTERED},
         {&r->out.m[0], MG * 12, -1}, {&r->out.m[1], MG * 12, -1},
         {&r->result.m[0], OUT * 12, -1}, {&r->result.m[1], OUT * 12, -1},
         {&r->prev_pixels, OUT * 8, R_PREV_PIXELS}, {&r->accept, OUT, R_ACCEPT},
         {&r->albedo, P * 12, R_ALBEDO}, {&r->tone, P * 12, R_TONE},
-        {&r->weights, (size_t)r->blocks * 30 * 4, R_WEIGHTS},
-        {&r->mins_maxs, (size_t)r->blocks * 12 * 4, R_MINS_MAXS},
+        {&r->weights, (size_t)r->blocks * 3 * features * 4, R_WEIGHTS},
+        {&r->mins_maxs, (size_t)r->blocks * 2 * (scaled ? scaled : 1) * 4, R_MINS_MAXS},
         {&r->spp.m[0], OUT, -1}, {&r->spp.m[1], OUT, -1},
     };
     for (auto &b : bufs)
@@ -221,7 +224,7 @@ int ref_open(const char *code_object, int W, int H, int half, void **out) {
     // Static arguments, opencl/bmfr.cpp:349-384
     CLT(clSetKernelArg(r->k_accum, 0, sizeof(cl_mem), &r->prev_pixels));
     CLT(clSetKernelArg(r->k_accum, 1, sizeof(cl_mem), &r->accept));
-    const size_t r_size = (11 * 12 / 2) * sizeof(cl_float3);  // COMPRESSED_R, bmfr.cpp:356-358
+    const size_t r_size = ((features + 1) * (features + 2) / 2) * sizeof(cl_float3);  // COMPRESSED_R, bmfr.cpp:354-358
     CLT(clSetKernelArg(r->k_fit, 0, 256 * sizeof(float), nullptr));
     CLT(clSetKernelArg(r->k_fit, 1, 1024 * sizeof(float), nullptr));
     CLT(clSetKernelArg(r->k_fit, 2, r_size, nullptr));

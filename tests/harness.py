@@ -23,17 +23,51 @@ def gpu_params(w, h, tmp_f16=True, storage_f16=False, **kw):
         storage=bmfr_amd.STORAGE_F16 if storage_f16 else bmfr_amd.STORAGE_F32, **kw)
 
 
-class Lockstep:
-    """A GPU stream and an oracle stream fed the same frames."""
+# The reference's feature set as explicit factor pairs (bmfr.cpp:65-77):
+# 1, normal.xyz unscaled; world_position.xyz and its squares scaled.
+REFERENCE_SET = ([(0, 0), (1, 0), (2, 0), (3, 0), (4, 0), (5, 0), (6, 0), (4, 4), (5, 5), (6, 6)], 4)
 
-    def __init__(self, w, h, tmp_f16=True, storage_f16=False, sc=None, fitter_cfg=None):
+
+def set_factor_pairs(p, pairs, not_scaled):
+    """Writes a feature set into library or oracle params (same fields)."""
+    p.feature_count = len(pairs)
+    p.features_not_scaled = not_scaled
+    for i in range(16):
+        a, b = pairs[i] if i < len(pairs) else (0, 0)
+        p.feature_factors[i][0] = a
+        p.feature_factors[i][1] = b
+
+
+def copy_features(dst, src):
+    dst.feature_count = src.feature_count
+    dst.features_not_scaled = src.features_not_scaled
+    for i in range(16):
+        dst.feature_factors[i][0] = src.feature_factors[i][0]
+        dst.feature_factors[i][1] = src.feature_factors[i][1]
+
+
+class Lockstep:
+    """A GPU stream and an oracle stream fed the same frames.  `features` is
+    the reference's pair of feature strings (NOT_SCALED, SCALED), parsed by
+    the library; `pairs` an explicit (factor pairs, not_scaled)."""
+
+    def __init__(self, w, h, tmp_f16=True, storage_f16=False, sc=None, fitter_cfg=None, features=None,
+                 pairs=None):
         self.w, self.h = w, h
         self.storage_f16 = storage_f16
         self.sc = sc or scene(w, h)
-        self.g = bmfr_amd.Stream(gpu_params(w, h, tmp_f16, storage_f16))
+        gp = gpu_params(w, h, tmp_f16, storage_f16)
+        if features is not None:
+            bmfr_amd.parse_feature_buffers(gp, *features)
+        if pairs is not None:
+            set_factor_pairs(gp, *pairs)
+        self.g = bmfr_amd.Stream(gp)
         if fitter_cfg:
             self.g.set_fitter_config(*fitter_cfg)
-        self.o = bind.OracleStream(bind.params(w, h, half_tmp=tmp_f16))
+        op = bind.params(w, h, half_tmp=tmp_f16)
+        copy_features(op, gp)
+        self.o = bind.OracleStream(op)
+        self.features = bind.feature_count(op)
 
     def upload(self, frame):
         f = self.sc.render_host(frame)

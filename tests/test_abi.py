@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_api_version_and_status_strings():
     lib = _lib.load()
-    assert lib.bmfr_api_version() == 1
+    assert lib.bmfr_api_version() == 2
     for st in range(6):
         assert lib.bmfr_status_string(st)
     assert lib.bmfr_status_string(99) == b"unknown status"

@@ -28,27 +28,36 @@ import pytest
 
 import bmfr_amd
 from oracle import bind, ref
-from tests.harness import psnr, rel_err, scene
+from tests.harness import psnr, rel_err, scene, set_factor_pairs
 
 pytestmark = pytest.mark.gpu
 
 
-def _ref_ready(w, h, half):
-    if not ref.available() or not os.path.exists(ref.code_object(w, h, half)):
+def _ref_ready(w, h, half, feature_set=None):
+    if not ref.available() or not os.path.exists(ref.code_object(w, h, half, feature_set)):
         pytest.skip("reference code objects not built (oracle/Makefile ref needs the reference tree)")
     if ref.load().ref_device_count() < 1:
         pytest.skip("no OpenCL GPU device")
 
 
-@pytest.mark.parametrize("w,h,half", [(96, 64, True), (256, 256, True), (256, 256, False)])
-def test_stage_by_stage_against_reference(gpu, w, h, half):
-    _ref_ready(w, h, half)
+# The reference's set at three sizes, and two configured sets (its
+# FEATURE_BUFFERS strings in oracle/Makefile; 5 and 16 features).
+@pytest.mark.parametrize("w,h,half,fs", [(96, 64, True, None), (256, 256, True, None), (256, 256, False, None),
+                                         (96, 64, True, "cross5"), (96, 64, False, "cross5"),
+                                         (96, 64, True, "quad16"), (96, 64, False, "quad16")])
+def test_stage_by_stage_against_reference(gpu, w, h, half, fs):
+    _ref_ready(w, h, half, fs)
     P = w * h
-    r = ref.RefRun(w, h, half)
-    g = bmfr_amd.Stream(bmfr_amd.default_params(
-        w, h, tmp_storage=bmfr_amd.STORAGE_F16 if half else bmfr_amd.STORAGE_F32))
-    sc = scene(w, h)
+    r = ref.RefRun(w, h, half, fs)
+    gparams = bmfr_amd.default_params(w, h, tmp_storage=bmfr_amd.STORAGE_F16 if half else bmfr_amd.STORAGE_F32)
     op = bind.params(w, h, half_tmp=half)
+    n = 10
+    if fs:
+        set_factor_pairs(gparams, *ref.FEATURE_SETS[fs])
+        set_factor_pairs(op, *ref.FEATURE_SETS[fs])
+        n = len(ref.FEATURE_SETS[fs][0])
+    g = bmfr_amd.Stream(gparams)
+    sc = scene(w, h)
     int_mismatch = 0
     worst = {}
     try:
@@ -81,12 +90,14 @@ def test_stage_by_stage_against_reference(gpu, w, h, half):
             g.fitter(fr)
             rm = r.read("mins_maxs")
             assert np.array_equal(rm, g.read_buffer("mins_maxs")), fr
-            rw = r.read("weights").reshape(-1, 10, 3).astype(np.float64)
-            gw = g.read_buffer("weights").reshape(-1, 10, 3).astype(np.float64)
+            rw = r.read("weights").reshape(-1, n, 3).astype(np.float64)
+            gw = g.read_buffer("weights").reshape(-1, n, 3).astype(np.float64)
             tmp32 = rt.astype(np.float32)
             fe = 0.0
             for blk in range(rw.shape[0]):
-                A, _, _ = bind.fitter_inputs(op, tmp32, blk, fr)
+                A, b, _ = bind.fitter_inputs(op, tmp32, blk, fr)
+                if fs == "quad16" and bind.lstsq_fp64(A, b)[1] < 1e-6:
+                    continue   # degree-2 columns nearly collinear: no well-defined fit
                 fr_, fg = A @ rw[blk], A @ gw[blk]
                 fe = max(fe, float(np.abs(fg - fr_).max() / (np.abs(fr_).max() + 1e-3)))
             worst["fit"] = max(worst.get("fit", 0), fe)
@@ -119,7 +130,8 @@ def test_stage_by_stage_against_reference(gpu, w, h, half):
             g.write_buffer("spp_cur", rs)
             r.end_frame()
             g.end_frame()
-        print(f"{w}x{h} half={half}: integer mismatches {int_mismatch}/{12 * P}, worst {worst}")
+        print(f"{w}x{h} half={half} features={fs or 'reference'}: integer mismatches "
+              f"{int_mismatch}/{12 * P}, worst {worst}")
         assert int_mismatch <= 1e-4 * 12 * P
         assert worst["prev_pixel"] < 1e-6          # |dx| < 1e-6 * max(W, H)
         assert worst["acc"] < 5e-3

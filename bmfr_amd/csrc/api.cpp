@@ -151,7 +151,7 @@ void sizes_of(const bmfr_params &p, bmfr_sizes &s, size_t *slot_bytes) {
     b[S_TONE] = 12 * P;
     b[S_OUT_A] = b[S_OUT_B] = 3 * g * P;
     b[S_RESULT_A] = b[S_RESULT_B] = 3 * g * P;
-    b[S_NOISE] = (nf > 1 ? nf - 1 : 1) * 1024 * sizeof(double);
+    b[S_NOISE] = (nf > 1 ? nf - 1 : 1) * 1024 * sizeof(float);
     b[S_FLAGS] = 4 * (size_t)s.blocks;
     b[S_W64] = 3 * nf * 4 * (size_t)s.blocks;
     size_t total = 0;

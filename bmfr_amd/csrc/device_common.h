@@ -121,20 +121,28 @@ __device__ __forceinline__ uint32_t hash32(uint32_t a) {
 }
 
 // The feature noise of one frame depends only on (row, feature, frame)
-// (bmfr.cl:179-181 has no block term), so one table of (N-1) x 1024 FP64
-// noise terms serves every block of a frame (N features, N + 3 buffers):
-// table[(f-1)*1024 + row] = NOISE_AMOUNT * 2 *
-// (random(row + 1024 f + 1024 (N+3) frame) - 0.5f), in double because
-// NOISE_AMOUNT is a double literal (bmfr.cl:179).
+// (bmfr.cl:179-181 has no block term), so one table of (N-1) x 1024 terms
+// serves every block of a frame (N features, N + 3 buffers).  The table
+// holds the float r = random(row + 1024 f + 1024 (N+3) frame) - 0.5f that
+// the reference computes; the term added is NOISE_AMOUNT * 2 * r in double
+// (NOISE_AMOUNT is a double literal, bmfr.cl:179), formed where it is used
+// (NoiseTable::term), so the table is 4 bytes per term.
 constexpr int kNoiseTable = (kFeatures - 1) * kBlockPixels;
 constexpr int kMaxNoiseTable = (kMaxFeatures - 1) * kBlockPixels;
-__device__ __forceinline__ double noise_term(int row, int feature, int frame, double noise_amount,
-                                             int buffer_count = kBufferCount) {
+__device__ __forceinline__ float noise_random(int row, int feature, int frame, int buffer_count = kBufferCount) {
     const uint32_t arg = (uint32_t)row + (uint32_t)feature * kBlockPixels +
                          (uint32_t)frame * (uint32_t)(buffer_count * kBlockPixels);
-    const float r = (float)hash32(arg) / 4294967296.0f - 0.5f;
-    return noise_amount * 2.0 * (double)r;
+    return (float)hash32(arg) / 4294967296.0f - 0.5f;
 }
+
+// A frame's noise table and 2 * NOISE_AMOUNT: term(i) = RN64(scale * r[i]),
+// the reference's NOISE_AMOUNT * 2.f * (random(...) - 0.5f) (the doubling
+// is exact).
+struct NoiseTable {
+    const float *r;
+    double scale;
+    __device__ __forceinline__ double term(double rv) const { return scale * rv; }
+};
 
 // opencl/bmfr.cl:200-205
 __device__ __forceinline__ float scale_feature(float v, float mn, float mx) {

@@ -50,7 +50,7 @@ __device__ __forceinline__ void block_sum_f64(double (&v)[NV], double (*red)[4])
 
 template <typename TT, bool HALF_ROUND>
 __device__ __forceinline__ void refactor_block(const TT *__restrict__ src, const float *__restrict__ mm,
-                                               const double *__restrict__ noise, float *__restrict__ w_out) {
+                                               NoiseTable noise, float *__restrict__ w_out) {
     __shared__ double red[16][4];
     __shared__ double rl[kFeatures][kBufferCount];
     const int t = threadIdx.x;
@@ -75,7 +75,8 @@ __device__ __forceinline__ void refactor_block(const TT *__restrict__ src, const
                 }
                 v = HALF_ROUND ? round_half(sc) : sc;
             }
-            if (f >= 1 && f < kFeatures) v = (float)((double)v + noise[(f - 1) * kBlockPixels + row]);
+            if (f >= 1 && f < kFeatures)
+                v = (float)((double)v + noise.term((double)noise.r[(f - 1) * kBlockPixels + row]));
             a[f][k] = (double)v;
         }
     }
@@ -143,7 +144,7 @@ __device__ __forceinline__ void refactor_block(const TT *__restrict__ src, const
 struct RefactorParams {
     const void *tmp;
     const float *mins_maxs;
-    const double *noise;
+    NoiseTable noise;
     const int32_t *flags;   // null: every block
     float *out;             // weights (in place) or a separate FP64-solution buffer
     int32_t blocks;
@@ -169,7 +170,8 @@ __global__ __launch_bounds__(256) void kb_refactor(const BatchLaunch L) {
     if (!bs.flags || !((const int32_t *)bs.flags)[blk]) return;
     refactor_block<TT, HALF_ROUND>(reinterpret_cast<const TT *>(bs.tmp_data) + (int64_t)blk * (kBufferCount * kBlockPixels),
                                    (const float *)bs.mins_maxs + (int64_t)blk * (kScaled * 2),
-                                   (const double *)bs.noise, (float *)bs.weights + (int64_t)blk * (kFeatures * 3));
+                                   NoiseTable{(const float *)bs.noise, 2.0 * bs.args.noise_amount},
+                                   (float *)bs.weights + (int64_t)blk * (kFeatures * 3));
 }
 
 static hipError_t launch_refactor_all(const StageArgs &a, const StageBuffers &b, const int32_t *flags,
@@ -177,7 +179,7 @@ static hipError_t launch_refactor_all(const StageArgs &a, const StageBuffers &b,
     RefactorParams rp;
     rp.tmp = b.tmp_data;
     rp.mins_maxs = (const float *)b.mins_maxs;
-    rp.noise = (const double *)b.noise;
+    rp.noise = NoiseTable{(const float *)b.noise, 2.0 * a.noise_amount};
     rp.flags = flags;
     rp.out = out;
     rp.blocks = a.blocks;
@@ -210,7 +212,8 @@ hipError_t launch_fitter(const StageArgs &a, const StageBuffers &b, const Fitter
     fp.tmp = b.tmp_data;
     fp.mins_maxs = (float *)b.mins_maxs;
     fp.weights = (float *)b.weights;
-    fp.noise = (const double *)b.noise;
+    fp.noise = (const float *)b.noise;
+    fp.noise_scale = 2.0 * a.noise_amount;
     fp.flags = qr_mode == 2 ? (int32_t *)b.flags : nullptr;
     fp.threshold = a.refactor_threshold;
     fp.blocks = a.blocks;

@@ -173,7 +173,8 @@ struct FitterParams {
     const void *tmp;
     float *mins_maxs;
     float *weights;
-    const double *noise;   // the frame's noise table (kNoiseTable terms)
+    const float *noise;    // the frame's noise table (random() - 0.5f per term)
+    double noise_scale;    // 2 * NOISE_AMOUNT
     int32_t *flags;        // per block: 1 = re-factor in FP64 (null: not wanted)
     float threshold;       // flag when min |R_ii| < threshold * max |R_ii|
     int32_t blocks;
@@ -259,7 +260,7 @@ constexpr int red_slots(bool comp) { return comp ? 32 : 16; }
 
 template <int RPL, int BPG, typename TT, bool HALF_ROUND, bool COMP>
 __device__ __forceinline__ void fitter_block(const TT *__restrict__ src, float *__restrict__ mins_maxs,
-                                             float *__restrict__ weights, const double *__restrict__ noise,
+                                             float *__restrict__ weights, NoiseTable noise,
                                              BlockFlag flag, bool active,
                                              float (&red)[2][BPG][16 / RPL][red_slots(COMP)]) {
     constexpr int RS = red_slots(COMP);
@@ -273,9 +274,6 @@ __device__ __forceinline__ void fitter_block(const TT *__restrict__ src, float *
     const int t = tid - sub * T;
     const int lane = tid & 63;
     const int wave = t >> 6;
-    struct {
-        const double *noise;
-    } p = {noise};
 
     // Row held in element k of this lane: rows t*V..t*V+V-1 of each T*V strip.
     auto row_of = [&](int k) { return (k / V) * (T * V) + t * V + (k % V); };
@@ -342,19 +340,21 @@ __device__ __forceinline__ void fitter_block(const TT *__restrict__ src, float *
     // FP64, rounded once to float (NOISE_AMOUNT is a double literal) ----
 #pragma unroll
     for (int f = 1; f < kFeatures; ++f) {
-        const double *nz = p.noise + (f - 1) * kBlockPixels;
+        const float *nz = noise.r + (f - 1) * kBlockPixels;
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-            double n[V];
+            float n[V];
 #pragma unroll
-            for (int e = 0; e < V; e += 2) {
-                const double2 x = *reinterpret_cast<const double2 *>(nz + q * (T * V) + t * V + e);
+            for (int e = 0; e < V; e += 4) {
+                const float4 x = *reinterpret_cast<const float4 *>(nz + q * (T * V) + t * V + e);
                 n[e] = x.x;
                 n[e + 1] = x.y;
+                n[e + 2] = x.z;
+                n[e + 3] = x.w;
             }
 #pragma unroll
             for (int e = 0; e < V; ++e)
-                a[f][q * V + e] = (float)((double)a[f][q * V + e] + n[e]);
+                a[f][q * V + e] = (float)((double)a[f][q * V + e] + noise.term((double)n[e]));
         }
     }
 
@@ -659,8 +659,8 @@ __device__ __forceinline__ void fast_pivots(FastRows<RPL, TT, HALF_ROUND, F + 3>
 // features [not_scaled, NF) and runs one wave per block (RPL 16).
 template <int RPL, int BPG, typename TT, bool HALF_ROUND, int NF = 0>
 __device__ __forceinline__ void fitter_block_fast(const TT *__restrict__ src, float *__restrict__ mins_maxs,
-                                                  float *__restrict__ weights, const double *__restrict__ noise,
-                                                  bool active, FastRed<RPL, BPG> &red, int not_scaled = kNotScaled) {
+                                                  float *__restrict__ weights, NoiseTable noise, bool active,
+                                                  FastRed<RPL, BPG> &red, int not_scaled = kNotScaled) {
     using L = FastLayout<RPL, TT>;
     constexpr int F = SetShape<NF>::F, C = SetShape<NF>::C;
     static_assert(NF == 0 || RPL == 16, "a configured feature set runs one wave per block");
@@ -762,12 +762,14 @@ __device__ __forceinline__ void fitter_block_fast(const TT *__restrict__ src, fl
     for (int f = 1; f < F; ++f) {
 #pragma unroll
         for (int q = 0; q < L::Q; ++q) {
-            const double *nz = noise + (f - 1) * kBlockPixels + q * (T * V) + t * V;
+            const float *nz = noise.r + (f - 1) * kBlockPixels + q * (T * V) + t * V;
 #pragma unroll
-            for (int e = 0; e < V; e += 2) {
-                const double2 x = *reinterpret_cast<const double2 *>(nz + e);
-                m.a[f][q * V + e] = (float)((double)m.a[f][q * V + e] + x.x);
-                m.a[f][q * V + e + 1] = (float)((double)m.a[f][q * V + e + 1] + x.y);
+            for (int e = 0; e < V; e += 4) {
+                const float4 x = *reinterpret_cast<const float4 *>(nz + e);
+                const float r4[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                    m.a[f][q * V + e + i] = (float)((double)m.a[f][q * V + e + i] + noise.term((double)r4[i]));
             }
         }
     }
@@ -803,13 +805,14 @@ __global__ __launch_bounds__(64 * (16 / RPL) * BPG) void k_fitter(const FitterPa
     if constexpr (!COMP) {
         fitter_block_fast<RPL, BPG, TT, HALF_ROUND>(
             reinterpret_cast<const TT *>(p.tmp) + (int64_t)blk * (kBufferCount * kBlockPixels),
-            p.mins_maxs + (int64_t)blk * (kScaled * 2), p.weights + (int64_t)blk * (kFeatures * 3), p.noise,
-            active, red);
+            p.mins_maxs + (int64_t)blk * (kScaled * 2), p.weights + (int64_t)blk * (kFeatures * 3),
+            NoiseTable{p.noise, p.noise_scale}, active, red);
     } else {
         fitter_block<RPL, BPG, TT, HALF_ROUND, COMP>(
             reinterpret_cast<const TT *>(p.tmp) + (int64_t)blk * (kBufferCount * kBlockPixels),
             p.mins_maxs + (int64_t)blk * (kScaled * 2), p.weights + (int64_t)blk * (kFeatures * 3),
-            p.noise, BlockFlag{p.flags ? p.flags + blk : nullptr, p.threshold}, active, red);
+            NoiseTable{p.noise, p.noise_scale}, BlockFlag{p.flags ? p.flags + blk : nullptr, p.threshold}, active,
+            red);
     }
 }
 
@@ -827,8 +830,8 @@ __global__ __launch_bounds__(128) void k_fitter_set(const FitterParams p) {
     const int blk = active ? b1 - 1 - idx : 0;
     fitter_block_fast<16, BPG, TT, HALF_ROUND, NF>(
         reinterpret_cast<const TT *>(p.tmp) + (int64_t)blk * (C * kBlockPixels),
-        p.mins_maxs + (int64_t)blk * ((NF - p.not_scaled) * 2), p.weights + (int64_t)blk * (NF * 3), p.noise,
-        active, red, p.not_scaled);
+        p.mins_maxs + (int64_t)blk * ((NF - p.not_scaled) * 2), p.weights + (int64_t)blk * (NF * 3),
+        NoiseTable{p.noise, p.noise_scale}, active, red, p.not_scaled);
 }
 
 template <int NF>
@@ -861,12 +864,14 @@ __global__ __launch_bounds__(64 * (16 / RPL) * BPG) void kb_fitter(const BatchLa
             fitter_block_fast<RPL, BPG, TT, HALF_ROUND>(
                 reinterpret_cast<const TT *>(bs.tmp_data) + (int64_t)blk * (kBufferCount * kBlockPixels),
                 (float *)bs.mins_maxs + (int64_t)blk * (kScaled * 2),
-                (float *)bs.weights + (int64_t)blk * (kFeatures * 3), (const double *)bs.noise, active, red);
+                (float *)bs.weights + (int64_t)blk * (kFeatures * 3),
+                NoiseTable{(const float *)bs.noise, 2.0 * bs.args.noise_amount}, active, red);
         } else {
             fitter_block<RPL, BPG, TT, HALF_ROUND, COMP>(
                 reinterpret_cast<const TT *>(bs.tmp_data) + (int64_t)blk * (kBufferCount * kBlockPixels),
                 (float *)bs.mins_maxs + (int64_t)blk * (kScaled * 2),
-                (float *)bs.weights + (int64_t)blk * (kFeatures * 3), (const double *)bs.noise,
+                (float *)bs.weights + (int64_t)blk * (kFeatures * 3),
+                NoiseTable{(const float *)bs.noise, 2.0 * bs.args.noise_amount},
                 BlockFlag{fl ? fl + blk : nullptr, bs.args.refactor_threshold}, active, red);
         }
         // A block ends with its last round in LDS buffer 0 (round 20 of

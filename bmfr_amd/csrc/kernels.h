@@ -50,7 +50,7 @@ struct StageBuffers {
     void *filtered, *tone_mapped;
     void *out_cur, *out_prev;
     void *result_cur, *result_prev;
-    void *noise;          // double[(N-1)*1024]: the fitter's feature noise of one frame
+    void *noise;          // float[(N-1)*1024]: the fitter's feature noise of one frame
     void *flags;          // int32[blocks]: FP64 re-factorisation flags
 };
 

@@ -34,13 +34,11 @@ __device__ __forceinline__ void stage1_body(const StageArgs &a, const StageBuffe
         if (write_noise) {
             const int nt = (nf - 1) * kBlockPixels, stride = a.margin_width * a.margin_height;
             for (int i = lin; i < nt; i += stride)
-                ((double *)bufs.noise)[i] =
-                    noise_term(i & (kBlockPixels - 1), 1 + (i >> 10), a.frame, a.noise_amount, ncol);
+                ((float *)bufs.noise)[i] = noise_random(i & (kBlockPixels - 1), 1 + (i >> 10), a.frame, ncol);
         }
     } else if (write_noise && lin < kNoiseTable) {
         // the padded grid has at least 96x96 = 9216 work-items (W, H >= 47)
-        ((double *)bufs.noise)[lin] =
-            noise_term(lin & (kBlockPixels - 1), 1 + (lin >> 10), a.frame, a.noise_amount);
+        ((float *)bufs.noise)[lin] = noise_random(lin & (kBlockPixels - 1), 1 + (lin >> 10), a.frame);
     }
     const Stage1Pixel q = stage1_pixel<GT>(a, bufs, gx, gy);
     // The work-group's 32x8 tile is rows yin0..yin0+7 of one block, and
@@ -651,18 +649,17 @@ hipError_t launch_accumulate_noisy_data(const StageArgs &a, const StageBuffers &
 }
 
 // The noise table of N features: (N-1) x 1024 terms, buffer count N + 3.
-__global__ __launch_bounds__(256) void k_noise_table(int frame, double noise_amount, int features,
-                                                     double *__restrict__ noise) {
+__global__ __launch_bounds__(256) void k_noise_table(int frame, int features, float *__restrict__ noise) {
     const int lin = blockIdx.x * 256 + threadIdx.x;
     if (lin < (features - 1) * kBlockPixels)
-        noise[lin] = noise_term(lin & (kBlockPixels - 1), 1 + (lin >> 10), frame, noise_amount, features + 3);
+        noise[lin] = noise_random(lin & (kBlockPixels - 1), 1 + (lin >> 10), frame, features + 3);
 }
 
 hipError_t launch_noise_table(const StageArgs &a, const StageBuffers &b, hipStream_t s) {
     const int nf = a.features.count ? a.features.count : kFeatures;
     if (nf < 2) return hipSuccess;   // one feature: no noised column
-    hipLaunchKernelGGL(k_noise_table, dim3((nf - 1) * kBlockPixels / 256), dim3(256), 0, s, a.frame,
-                       a.noise_amount, nf, (double *)b.noise);
+    hipLaunchKernelGGL(k_noise_table, dim3((nf - 1) * kBlockPixels / 256), dim3(256), 0, s, a.frame, nf,
+                       (float *)b.noise);
     return hipGetLastError();
 }
 

@@ -19,7 +19,10 @@ against the CPU oracle).  So:
   end to end                final frame of 8 frames, per-pixel relative
                             error and PSNR against the noise-free render
 The racy margin texels of the reference's tmp_data (bmfr.cl:322 vs 481)
-are compared only through the fitter's tolerance.
+are compared only through the fitter's tolerance; end to end, the pixels
+that blocks holding such texels can reach vary between runs of the
+reference itself, so the 99.9th-percentile bound applies to the interior
+and the whole image gets the 99th.
 """
 import os
 
@@ -170,13 +173,24 @@ def test_end_to_end_against_reference(gpu, half):
         fin = np.isfinite(res_r) & np.isfinite(res_g)
         assert np.array_equal(np.isfinite(res_r), np.isfinite(res_g))
         err = rel_err(res_g[fin], res_r[fin])
+        # The reference's blocks that hold mirrored margin texels race on
+        # current_noisy (bmfr.cl:322 vs 481), so its own output there varies
+        # from run to run.  Such blocks reach at most 48 px into the image
+        # (32 px blocks, jitter up to 16 px); reprojection and TAA carry their
+        # colour a few px per frame.  Pixels 80 px from every edge are free of
+        # it over these 8 frames and get the tight bound.
+        y, x = np.mgrid[0:h, 0:w]
+        inner = np.repeat(((x >= 80) & (x < w - 80) & (y >= 80) & (y < h - 80)).ravel(), 3) & fin
+        err_in = rel_err(res_g[inner], res_r[inner])
         from tests.test_gpu_parity import clean_tone_mapped
         clean = clean_tone_mapped(sc, frames - 1)
         d = abs(psnr(np.clip(res_g, 0, 1), clean) - psnr(np.clip(res_r, 0, 1), clean))
-        print(f"half={half}: median {np.median(err):.2e} p99.9 {np.percentile(err, 99.9):.2e} "
-              f"max {err.max():.2e} psnr delta {d:.2e} dB")
+        print(f"half={half}: median {np.median(err):.2e} p99 {np.percentile(err, 99):.2e} "
+              f"p99.9 {np.percentile(err, 99.9):.2e} max {err.max():.2e}; interior p99.9 "
+              f"{np.percentile(err_in, 99.9):.2e} max {err_in.max():.2e}; psnr delta {d:.2e} dB")
         assert np.median(err) < 1e-4
-        assert np.percentile(err, 99.9) < (2e-2 if half else 5e-3)
+        assert np.percentile(err, 99) < (2e-2 if half else 5e-3)
+        assert np.percentile(err_in, 99.9) < (2e-2 if half else 5e-3)
         assert d < 0.01
     finally:
         r.close()
@@ -224,9 +238,14 @@ def test_reference_kernels_timed_on_the_same_gpu_720p(gpu):
         fin = np.isfinite(res_r) & np.isfinite(res_g)
         assert np.array_equal(np.isfinite(res_r), np.isfinite(res_g))
         err = rel_err(res_g[fin], res_r[fin])
-        print(f"final frame: median {np.median(err):.2e} p99.9 {np.percentile(err, 99.9):.2e}")
+        y, x = np.mgrid[0:h, 0:w]   # the race-free interior (see the end-to-end test)
+        inner = np.repeat(((x >= 80) & (x < w - 80) & (y >= 80) & (y < h - 80)).ravel(), 3) & fin
+        err_in = rel_err(res_g[inner], res_r[inner])
+        print(f"final frame: median {np.median(err):.2e} p99.9 {np.percentile(err, 99.9):.2e}, "
+              f"interior p99.9 {np.percentile(err_in, 99.9):.2e}")
         assert np.median(err) < 1e-4
-        assert np.percentile(err, 99.9) < 2e-2
+        assert np.percentile(err, 99) < 2e-2
+        assert np.percentile(err_in, 99.9) < 2e-2
         # Under a profiler the OpenCL runtime's event timestamps collapse to
         # a few hundred ns; compare only timings that are plausible.
         if ref_ms.min() > 1e-3:

@@ -64,26 +64,61 @@ __device__ __forceinline__ float wave_reduce(float v) {
     return readlane(v, 63);
 }
 
-// N independent 64-lane reductions, step-major (one DPP step for all values
-// before the next, so that independent chains fill each other's hazard
-// slots); results are wave-uniform and each equals wave_reduce<Op> of its
-// input.
+// gfx950 cross-row swaps.  permlane32_swap(a, b): a <- [a lanes 0-31,
+// b lanes 0-31], b <- [a lanes 32-63, b lanes 32-63]; permlane16_swap does
+// the same with 16-lane rows: a <- [a.r0, b.r0, a.r2, b.r2],
+// b <- [a.r1, b.r1, a.r3, b.r3].
+__device__ __forceinline__ void swap32(float &a, float &b) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    a = __uint_as_float(r[0]);
+    b = __uint_as_float(r[1]);
+}
+__device__ __forceinline__ void swap16(float &a, float &b) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    a = __uint_as_float(r[0]);
+    b = __uint_as_float(r[1]);
+}
+
+// N independent 64-lane reductions; results are wave-uniform.  Values go
+// in groups of four through one register: a permlane32 swap and an Op fold
+// each pair (lanes l and l+32; the pair's first value stays in lanes 0-31,
+// the second in 32-63), a permlane16 swap and an Op fold the two pairs
+// (rows r and r+1), leaving value 4g in row 0, 4g+2 in row 1, 4g+1 in row 2
+// and 4g+3 in row 3; four in-row DPP steps then finish all four at once.
+// Each step runs over every group before the next (independent chains fill
+// each other's hazard slots).  About 3.5 instructions per value, against 8
+// for six DPP steps with the two row broadcasts.
 template <typename Op, int N>
 __device__ __forceinline__ void wave_reduce_n(float (&v)[N]) {
+    constexpr int G = (N + 3) / 4;
+    float w[4 * G];
 #pragma unroll
-    for (int i = 0; i < N; ++i) v[i] = Op::f(v[i], dpp_all<0xB1>(v[i]));
+    for (int i = 0; i < 4 * G; ++i) w[i] = i < N ? v[i] : Op::identity;
+    float p[2 * G];
 #pragma unroll
-    for (int i = 0; i < N; ++i) v[i] = Op::f(v[i], dpp_all<0x4E>(v[i]));
+    for (int i = 0; i < 2 * G; ++i) {
+        float a = w[2 * i], b = w[2 * i + 1];
+        swap32(a, b);
+        p[i] = Op::f(a, b);
+    }
+    float s[G];
 #pragma unroll
-    for (int i = 0; i < N; ++i) v[i] = Op::f(v[i], dpp_all<0x141>(v[i]));
+    for (int g = 0; g < G; ++g) {
+        float a = p[2 * g], b = p[2 * g + 1];
+        swap16(a, b);
+        s[g] = Op::f(a, b);
+    }
 #pragma unroll
-    for (int i = 0; i < N; ++i) v[i] = Op::f(v[i], dpp_all<0x140>(v[i]));
+    for (int g = 0; g < G; ++g) s[g] = Op::f(s[g], dpp_all<0x140>(s[g]));   // row_mirror
 #pragma unroll
-    for (int i = 0; i < N; ++i) v[i] = Op::f(v[i], (dpp<0x142, 0xA>(Op::identity, v[i])));
+    for (int g = 0; g < G; ++g) s[g] = Op::f(s[g], dpp_all<0x141>(s[g]));   // row_half_mirror
 #pragma unroll
-    for (int i = 0; i < N; ++i) v[i] = Op::f(v[i], (dpp<0x143, 0xC>(Op::identity, v[i])));
+    for (int g = 0; g < G; ++g) s[g] = Op::f(s[g], dpp_all<0x4E>(s[g]));    // quad_perm [2,3,0,1]
 #pragma unroll
-    for (int i = 0; i < N; ++i) v[i] = readlane(v[i], 63);
+    for (int g = 0; g < G; ++g) s[g] = Op::f(s[g], dpp_all<0xB1>(s[g]));    // quad_perm [1,0,3,2]
+    constexpr int kLane[4] = {0, 32, 16, 48};
+#pragma unroll
+    for (int i = 0; i < N; ++i) v[i] = readlane(s[i / 4], kLane[i % 4]);
 }
 
 // Reduce NV values over the W waves of one block.  `red` points at this
@@ -577,6 +612,18 @@ struct FastRows {
     __device__ __forceinline__ float pivot_row(int f, int r) const { return readlane(a[f][r % L::V], r / L::V); }
 };
 
+// A lane's share of a column dot product, sum over k of x[k] * y(k): four
+// FMA chains joined pairwise (shorter chains round less than one running
+// sum).
+template <int RPL, typename Y>
+__device__ __forceinline__ float lane_dot(const float (&x)[RPL], Y y) {
+    static_assert(RPL % 4 == 0, "four chains");
+    float s[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k = 0; k < RPL; ++k) s[k % 4] = __builtin_fmaf(x[k], y(k), s[k % 4]);
+    return (s[0] + s[1]) + (s[2] + s[3]);
+}
+
 // Pivot step COL (round COL + 1): norm, u, the C - 1 - COL trailing dot
 // products, update (C = F + 3 columns).
 template <int COL, int RPL, int BPG, typename TT, bool HALF_ROUND, int F>
@@ -594,26 +641,9 @@ __device__ __forceinline__ void fast_pivot(FastRows<RPL, TT, HALF_ROUND, F + 3> 
         else x[k] = m.a[COL][k];
     }
     float v[NV + 1];
-    {
-        float s0 = 0.f, s1 = 0.f;
+    v[0] = lane_dot<RPL>(x, [&](int k) { return x[k]; });
 #pragma unroll
-        for (int k = 0; k < RPL; k += 2) {
-            s0 = __builtin_fmaf(x[k], x[k], s0);
-            s1 = __builtin_fmaf(x[k + 1], x[k + 1], s1);
-        }
-        v[0] = s0 + s1;
-    }
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-        const int j = COL + 1 + i;
-        float s0 = 0.f, s1 = 0.f;
-#pragma unroll
-        for (int k = 0; k < RPL; k += 2) {
-            s0 = __builtin_fmaf(x[k], m.a[j][k], s0);
-            s1 = __builtin_fmaf(x[k + 1], m.a[j][k + 1], s1);
-        }
-        v[1 + i] = s0 + s1;
-    }
+    for (int i = 0; i < NV; ++i) v[1 + i] = lane_dot<RPL>(x, [&](int k) { return m.a[COL + 1 + i][k]; });
     wave_reduce_n<OpAdd, NV + 1>(v);
     float e[NV + 1];   // x_c and the pivot row of the trailing columns
     e[0] = m.pivot_row(COL, COL);

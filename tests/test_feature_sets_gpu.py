@@ -178,3 +178,44 @@ def test_batches_and_servers_refuse_configured_sets(gpu):
             s.fitter_fp64_weights()
     finally:
         s.close()
+
+
+@pytest.mark.parametrize("tmp_f16", [False, True])
+def test_configured_set_full_size_1080p(gpu, tmp_f16):
+    """16 features at 1920x1080: stage 1 bit-exact over two frames, the
+    fitter's mins_maxs bit-exact, and on a sample of well-conditioned blocks
+    the fitted colour no further from FP64 least squares than twice the
+    oracle's own distance."""
+    w, h = 1920, 1080
+    ls = Lockstep(w, h, tmp_f16=tmp_f16, sc=scene(w, h, spheres=48), pairs=SETS["quad16"])
+    n = ls.features
+    try:
+        _stage1_frames(ls, 2)
+        fr = 1
+        tmp0 = ls.o.tmp.copy()
+        ls.g.fitter(fr)
+        s = ls.o.s
+        rng = np.random.default_rng(9)
+        sample = np.unique(np.concatenate([[0, s.blocks - 1], rng.choice(s.blocks, 62, replace=False)]))
+        for b in sample:
+            ls.o.fitter(fr, int(b), 1)
+        ns = n - ls.o.p.features_not_scaled
+        mm_g = ls.read("mins_maxs").reshape(s.blocks, 2 * ns)
+        mm_o = ls.o.mins_maxs.reshape(s.blocks, 2 * ns)
+        assert np.array_equal(mm_g[sample], mm_o[sample])
+        Wg = ls.read("weights").reshape(s.blocks, n, 3).astype(np.float64)
+        Wo = ls.o.weights.reshape(s.blocks, n, 3).astype(np.float64)
+        used = 0
+        for b in sample:
+            A, bb, _ = bind.fitter_inputs(ls.o.p, tmp0, int(b), fr)
+            w64, rc = bind.lstsq_fp64(A, bb)
+            if rc < 1e-6:
+                continue
+            used += 1
+            sc_ = np.abs(A @ w64).max() + 1e-3
+            eg = np.abs(A @ Wg[b] - A @ w64).max() / sc_
+            eo = np.abs(A @ Wo[b] - A @ w64).max() / sc_
+            assert eg <= 2.0 * eo + (1e-6 if not tmp_f16 else 1e-4), (b, eg, eo)
+        assert used >= len(sample) // 2
+    finally:
+        ls.close()

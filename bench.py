@@ -26,6 +26,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# 256 CUs x 4 SIMDs at 2.4 GHz, one wave64 VALU instruction per SIMD every 4 cycles
+VALU_ISSUE_PEAK = 1024 * 2.4e9 / 4 / 1e9
+# SQ_INSTS_VALU / SQ_WAVES of the fitter (k_fitter<16, 2, __half, true, false>),
+# profiles/fitter_720p_counters.md
+FITTER_VALU_PER_WAVE = 6134
 W, H = 1280, 720
 SPHERES = 24
 SEED = 0xB3F40000 + 1
@@ -534,6 +539,15 @@ def main():
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(stage_gbs[dom] / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(dom), "algorithmic_bytes": int(sb[dom]),
             "stage_frac": {k: round(v / HBM_PEAK_GBS, 4) for k, v in stage_gbs.items()}}
+    # The fitter's own limit is instruction issue: its code is straight-line,
+    # FITTER_VALU_PER_WAVE instructions per wave (one wave per block, the
+    # launch padded to whole work-groups per XCD), against one wave64 VALU
+    # instruction per 4 cycles per SIMD.
+    waves = 8 * (((nB + 7) // 8 + 1) // 2) * 2
+    issue = waves * FITTER_VALU_PER_WAVE / (stage_ms["fitter"] * 1e-3) / 1e9
+    roof["issue"] = {"bound": "valu", "achieved": round(issue, 1), "peak": VALU_ISSUE_PEAK,
+                     "unit": "G wave-instructions/s", "frac": round(issue / VALU_ISSUE_PEAK, 4),
+                     "valu_per_wave": FITTER_VALU_PER_WAVE, "waves": waves}
     batched = None
     if not args.no_batched:
         batched = run_batched(args.batched_steps, 5, args.batch_lanes)

@@ -520,8 +520,9 @@ def main():
     ap.add_argument("--no-feature-sets", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the path shards by whole render streams, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported (the path shards by "
+              "whole render streams: each GPU's process would run its own streams, with nothing to "
+              "exchange)", file=sys.stderr)
         sys.exit(2)
 
     totals, t, wall, s = run_plain(args)

@@ -69,17 +69,21 @@ def load_frame(data_dir, frame, width, height):
                             ("noisy", NOISY))}
 
 
-def run(data_dir, frames, out_dir=None, width=None, height=None, **params):
+def run(data_dir, frames, out_dir=None, width=None, height=None, features=None, **params):
     """The reference's tasks() loop on a dataset directory: every frame's
     inputs uploaded, the five stages run, the result written to
     out_dir/outputNN.png.  The previous frame's matrix is used for
     reprojection (matrix_index = max(frame - 1, 0), opencl/bmfr.cpp:440-442).
-    Returns the list of (H, W, 3) results."""
-    from . import Stream, default_params
+    `features` is an optional (NOT_SCALED_FEATURE_BUFFERS,
+    SCALED_FEATURE_BUFFERS) pair of the reference's feature strings
+    (opencl/bmfr.cpp:65-77).  Returns the list of (H, W, 3) results."""
+    from . import Stream, default_params, parse_feature_buffers
     if width is None or height is None:
         width, height, _ = exr_info(os.path.join(data_dir, f"{NOISY}0.exr"))
     mats, offs, pl, nl = load_camera(os.path.join(data_dir, "camera_matrices.h"), frames)
     p = default_params(width, height, position_limit_squared=pl, normal_limit_squared=nl, **params)
+    if features is not None:
+        parse_feature_buffers(p, *features)
     results = []
     with Stream(p) as st:
         for f in range(frames):

@@ -230,13 +230,18 @@ def _write_dataset(d, sc, w, h, frames):
 
 
 @pytest.mark.gpu
-def test_dataset_run_equals_direct_frames(gpu, tmp_path):
+@pytest.mark.parametrize("features", [None, ("1.f,normal.y,", "world_position.x*world_position.y,world_position.z")])
+def test_dataset_run_equals_direct_frames(gpu, tmp_path, features):
     w, h, frames = 96, 64, 4
     sc = bmfr_amd.Scene(0xB3F40000, 8, 1e-3, w, h)
     _write_dataset(str(tmp_path), sc, w, h, frames)
     out = tmp_path / "outputs"
-    res = dataset.run(str(tmp_path), frames, str(out))
-    st = bmfr_amd.Stream(bmfr_amd.default_params(w, h))
+    res = dataset.run(str(tmp_path), frames, str(out), features=features)
+    p = bmfr_amd.default_params(w, h)
+    if features is not None:
+        bmfr_amd.parse_feature_buffers(p, *features)
+        assert p.feature_count == 4
+    st = bmfr_amd.Stream(p)
     try:
         for f in range(frames):
             g = sc.render_host(f)

@@ -31,6 +31,22 @@ VALU_ISSUE_PEAK = 1024 * 2.4e9 / 4 / 1e9
 # SQ_INSTS_VALU / SQ_WAVES of the fitter (k_fitter<16, 2, __half, true, false>),
 # profiles/fitter_720p_counters.md
 FITTER_VALU_PER_WAVE = 6134
+
+
+def reference_fitter_lds_bytes():
+    """LDS bytes one block of the reference's fitter moves (bmfr.cl:510-656,
+    13 buffers, 1024 rows): the 4 KB u_vec store of each of the 13 columns,
+    a 4-byte u_vec read in the dot product and another in the update for
+    every row at or below the pivot of every trailing column (the colour
+    columns pivot at row 10 and each has two trailing columns,
+    bmfr.cl:552, 606), and the min and max reductions' 2 x 4 KB of each of
+    the 6 scaled features."""
+    total = 13 * 4096 + 6 * 2 * 4096
+    for col in range(13):
+        cl = min(col, 10)
+        m = 1024 - cl
+        total += 8 * m * (13 - (cl + 1))
+    return total
 W, H = 1280, 720
 SPHERES = 24
 SEED = 0xB3F40000 + 1
@@ -549,10 +565,25 @@ def main():
     roof["issue"] = {"bound": "valu", "achieved": round(issue, 1), "peak": VALU_ISSUE_PEAK,
                      "unit": "G wave-instructions/s", "frac": round(issue / VALU_ISSUE_PEAK, 4),
                      "valu_per_wave": FITTER_VALU_PER_WAVE, "waves": waves}
+    # The reference's fitter streams its block through LDS; this one keeps it
+    # in registers.  The reference's LDS bytes per block over the fitter's
+    # time, against the measured LDS read peak, is the fitter's share of the
+    # LDS roofline its method implies.
+    import bmfr_amd
+    lds_peak = bmfr_amd.lds_read_bandwidth() / 1e9
+    lds_b = reference_fitter_lds_bytes()
+    lds_rate = nB * lds_b / (stage_ms["fitter"] * 1e-3) / 1e9
+    roof["lds"] = {"bound": "lds", "algorithmic_bytes_per_block": lds_b, "achieved": round(lds_rate, 1),
+                   "peak": round(lds_peak, 1), "peak_nominal": round(256 * 128 * 2.4, 1), "unit": "GB/s",
+                   "frac": round(lds_rate / lds_peak, 4)}
+    progress(f"LDS read peak {lds_peak:.0f} GB/s; fitter at {lds_rate:.0f} GB/s of reference LDS traffic")
     batched = None
     if not args.no_batched:
         batched = run_batched(args.batched_steps, 5, args.batch_lanes)
         batched["speedup_vs_plain_720p"] = round(ms_frame / batched["ms_per_720p_frame_equiv"], 2)
+        fl = batched["blocks_per_step"] * lds_b / (batched["stage_ms"]["fitter"] * 1e-3) / 1e9
+        batched["fitter_lds"] = {"achieved": round(fl, 1), "peak": round(lds_peak, 1), "unit": "GB/s",
+                                 "frac": round(fl / lds_peak, 4)}
         progress(f"batched: {batched['step_ms']} ms/step, x{batched['speedup_vs_plain_720p']} vs plain")
     precision = None
     if not args.no_precision:

@@ -71,6 +71,13 @@ def device_count():
     return _lib.load().bmfr_device_count()
 
 
+def lds_read_bandwidth(steps=4096):
+    """Measured LDS read bandwidth of the current GPU, bytes/s."""
+    v = ctypes.c_double()
+    _check(_lib.load().bmfr_measure_lds_read_bandwidth(steps, ctypes.byref(v)))
+    return v.value
+
+
 def default_params(width, height, **overrides):
     """Reference defaults (opencl/bmfr.cpp:58-62, 88) with optional overrides."""
     lib = _lib.load()

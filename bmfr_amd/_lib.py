@@ -56,6 +56,7 @@ SIGNATURES = [
     ("bmfr_status_string", ctypes.c_char_p, [c_int32]),
     ("bmfr_last_error", ctypes.c_char_p, []),
     ("bmfr_device_count", c_int32, []),
+    ("bmfr_measure_lds_read_bandwidth", c_int32, [c_int32, ctypes.POINTER(ctypes.c_double)]),
     ("bmfr_default_params", c_int32, [c_int32, c_int32, ctypes.POINTER(BmfrParams)]),
     ("bmfr_parse_feature_buffers", c_int32, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(BmfrParams)]),
     ("bmfr_compute_sizes", c_int32, [ctypes.POINTER(BmfrParams), ctypes.POINTER(BmfrSizes)]),

@@ -330,6 +330,40 @@ int32_t bmfr_device_count(void) {
     return n;
 }
 
+int32_t bmfr_measure_lds_read_bandwidth(int32_t steps, double *bytes_per_second) {
+    if (!bytes_per_second || steps <= 0)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "need steps > 0 and an output pointer");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(BMFR_ERR_NO_DEVICE, "no HIP device visible");
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    const int groups = prop.multiProcessorCount * 8;   // eight 256-wide work-groups per CU
+    constexpr int kReps = 5;
+    uint4 *out = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&out, (size_t)groups * 256 * sizeof(uint4));
+    if (e == hipSuccess) e = hipStreamCreate(&s);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = bmfr::launch_lds_read(out, groups, steps, s);   // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, s);
+    for (int r = 0; r < kReps && e == hipSuccess; ++r) e = bmfr::launch_lds_read(out, groups, steps, s);
+    if (e == hipSuccess) e = hipEventRecord(e1, s);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e1) hipEventDestroy(e1);
+    if (e0) hipEventDestroy(e0);
+    if (s) hipStreamDestroy(s);
+    if (out) hipFree(out);
+    if (e != hipSuccess) return fail(BMFR_ERR_HIP, "LDS bandwidth probe failed: %s", hipGetErrorString(e));
+    *bytes_per_second = (double)groups * 256 * 64 * (double)steps * kReps / (ms * 1e-3);
+    return BMFR_OK;
+}
+
 int32_t bmfr_default_params(int32_t width, int32_t height, bmfr_params *out) {
     if (!out) return fail(BMFR_ERR_INVALID_ARGUMENT, "out is null");
     // opencl/bmfr.cpp:58-62, 85-88; the position/normal limits come from the

@@ -111,6 +111,9 @@ struct CopyJob {
     int64_t bytes[4];
 };
 hipError_t launch_copy4(const CopyJob &job, int count, hipStream_t s);
+// LDS read bandwidth probe: `groups` work-groups of 256, `steps` x 64 bytes
+// read per work-item; out holds groups * 256 uint4.
+hipError_t launch_lds_read(uint4 *out, int groups, int steps, hipStream_t s);
 
 // write_noise: stage 1 also fills the frame's noise table (spare threads).
 hipError_t launch_accumulate_noisy_data(const StageArgs &a, const StageBuffers &b, bool write_noise,

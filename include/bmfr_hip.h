@@ -144,6 +144,12 @@ const char *bmfr_status_string(int32_t status);
 const char *bmfr_last_error(void);
 /* Number of visible HIP devices (0 without a GPU); never fails. */
 int32_t bmfr_device_count(void);
+/* Measured LDS read bandwidth of the current device in bytes/s: 16-byte
+ * conflict-free reads from a 32 KB tile by eight 256-wide work-groups per
+ * CU, `steps` x 64 bytes per work-item, five timed launches.  No counterpart
+ * in the reference; it is the ceiling against which the fitter's work is
+ * stated (the reference's fitter streams its block matrix through LDS). */
+int32_t bmfr_measure_lds_read_bandwidth(int32_t steps, double *bytes_per_second);
 
 /* Reference defaults (opencl/bmfr.cpp:58-62, 88) for a W x H stream:
  * F32 storage, F16 tmp_data, plain FP32 QR, the reference's feature set. */

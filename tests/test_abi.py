@@ -85,6 +85,9 @@ def test_null_arguments_return_status():
     assert lib.bmfr_fitter(None, 0) == bmfr_amd.ERR_INVALID_ARGUMENT
     assert lib.bmfr_stream_close(None) == bmfr_amd.OK
     assert b"null" in lib.bmfr_last_error()
+    d = ctypes.c_double()
+    assert lib.bmfr_measure_lds_read_bandwidth(0, ctypes.byref(d)) == bmfr_amd.ERR_INVALID_ARGUMENT
+    assert lib.bmfr_measure_lds_read_bandwidth(16, None) == bmfr_amd.ERR_INVALID_ARGUMENT
 
 
 def test_stream_open_without_gpu_reports_no_device():

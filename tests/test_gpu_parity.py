@@ -209,6 +209,15 @@ def clean_tone_mapped(sc, frame, firefly=1e-3):
     return np.clip(np.power(np.maximum(0, alb * mean), 0.454545), 0, 1).ravel()
 
 
+def test_lds_read_bandwidth_probe(gpu):
+    """The LDS ceiling the bench states the fitter against: a measured figure
+    of the right order (256 CUs, 64 to 256 bytes per clock each at about
+    2.4 GHz) and repeatable."""
+    a, b = bmfr_amd.lds_read_bandwidth(), bmfr_amd.lds_read_bandwidth()
+    assert 2e13 < a < 2e14, a
+    assert abs(a - b) / a < 0.2, (a, b)
+
+
 def test_timing_and_download(gpu):
     ls = Lockstep(96, 64)
     try:

@@ -72,19 +72,24 @@ struct ExrImage {
     int compression = kNone;
     std::vector<ExrChannel> channels;
     size_t header_end = 0;
-    int width() const { return xmax - xmin + 1; }
-    int height() const { return ymax - ymin + 1; }
+    // In 64 bits: a corrupted window must not overflow before it is checked.
+    int64_t width() const { return (int64_t)xmax - xmin + 1; }
+    int64_t height() const { return (int64_t)ymax - ymin + 1; }
 };
+
+constexpr int64_t kMaxExrSide = 65536;   // larger windows are refused as malformed
 
 struct Reader {
     const std::vector<uint8_t> &b;
     size_t p = 0;
     bool ok = true;
     explicit Reader(const std::vector<uint8_t> &buf) : b(buf) {}
+    // n more bytes from p are inside the buffer (no wrap for any p)
+    bool fits(size_t n) const { return p <= b.size() && b.size() - p >= n; }
     template <typename T>
     T get() {
         T v{};
-        if (p + sizeof(T) > b.size()) {
+        if (!fits(sizeof(T))) {
             ok = false;
             return v;
         }
@@ -115,16 +120,16 @@ int32_t parse_exr_header(const std::vector<uint8_t> &buf, ExrImage &im, const ch
         if (name.empty()) break;
         const std::string type = r.str();
         const int32_t size = r.get<int32_t>();
-        if (!r.ok || size < 0 || r.p + (size_t)size > buf.size()) {
+        if (!r.ok || size < 0 || !r.fits((size_t)size)) {
             r.ok = false;
             break;
         }
         const size_t end = r.p + (size_t)size;
         if (name == "channels" && type == "chlist") {
-            while (r.p < end) {
+            while (r.ok && r.p < end) {
                 ExrChannel c;
                 c.name = r.str();
-                if (c.name.empty()) break;
+                if (!r.ok || c.name.empty()) break;
                 c.type = r.get<int32_t>();
                 (void)r.get<uint8_t>();   // pLinear
                 r.p += 3;                 // reserved
@@ -144,7 +149,8 @@ int32_t parse_exr_header(const std::vector<uint8_t> &buf, ExrImage &im, const ch
         }
         r.p = end;
     }
-    if (!r.ok || !have_channels || !have_window || im.width() <= 0 || im.height() <= 0)
+    if (!r.ok || !have_channels || !have_window || im.width() <= 0 || im.height() <= 0 ||
+        im.width() > kMaxExrSide || im.height() > kMaxExrSide)
         return fail(BMFR_ERR_INVALID_ARGUMENT, "%s: malformed OpenEXR header", path);
     for (const ExrChannel &c : im.channels) {
         if (c.xs != 1 || c.ys != 1 || c.type < kUint || c.type > kFloat)
@@ -371,8 +377,8 @@ int32_t bmfr_exr_info(const char *path, int32_t *width, int32_t *height, int32_t
     std::vector<uint8_t> buf;
     ExrImage im;
     if (int32_t st = load_exr(path, buf, im)) return st;
-    if (width) *width = im.width();
-    if (height) *height = im.height();
+    if (width) *width = (int32_t)im.width();
+    if (height) *height = (int32_t)im.height();
     if (channels) *channels = (int32_t)im.channels.size();
     return BMFR_OK;
 }
@@ -382,7 +388,7 @@ int32_t bmfr_exr_read_rgb(const char *path, int32_t width, int32_t height, float
     std::vector<uint8_t> buf;
     ExrImage im;
     if (int32_t st = load_exr(path, buf, im)) return st;
-    const int W = im.width(), H = im.height();
+    const int W = (int)im.width(), H = (int)im.height();
     if (W != width || H != height)
         return fail(BMFR_ERR_INVALID_ARGUMENT, "%s is %dx%d, expected %dx%d", path, W, H, width, height);
     // R, G, B by name; a single luminance channel Y fills all three
@@ -406,16 +412,26 @@ int32_t bmfr_exr_read_rgb(const char *path, int32_t width, int32_t height, float
     for (int c = 0; c < chunks; ++c) offsets[c] = r.get<uint64_t>();
     if (!r.ok) return fail(BMFR_ERR_INVALID_ARGUMENT, "%s: truncated chunk table", path);
     std::vector<uint8_t> raw, tmp;
+    std::vector<uint8_t> filled((size_t)H, 0);   // every line exactly once
     for (int c = 0; c < chunks; ++c) {
+        if (offsets[c] > (uint64_t)buf.size())
+            return fail(BMFR_ERR_INVALID_ARGUMENT, "%s: chunk %d offset beyond the file", path, c);
         Reader cr(buf);
         cr.p = (size_t)offsets[c];
         const int32_t y0 = cr.get<int32_t>();
         const int32_t size = cr.get<int32_t>();
-        if (!cr.ok || size < 0 || cr.p + (size_t)size > buf.size())
+        if (!cr.ok || size < 0 || !cr.fits((size_t)size))
             return fail(BMFR_ERR_INVALID_ARGUMENT, "%s: bad chunk %d", path, c);
-        const int line0 = y0 - im.ymin;
-        if (line0 < 0 || line0 >= H) return fail(BMFR_ERR_INVALID_ARGUMENT, "%s: chunk %d starts outside", path, c);
+        const int64_t line0_64 = (int64_t)y0 - im.ymin;
+        if (line0_64 < 0 || line0_64 >= H)
+            return fail(BMFR_ERR_INVALID_ARGUMENT, "%s: chunk %d starts outside", path, c);
+        const int line0 = (int)line0_64;
         const int lines = std::min(lpc, H - line0);
+        for (int l = 0; l < lines; ++l) {
+            if (filled[(size_t)(line0 + l)])
+                return fail(BMFR_ERR_INVALID_ARGUMENT, "%s: line %d appears in two chunks", path, line0 + l);
+            filled[(size_t)(line0 + l)] = 1;
+        }
         const size_t expect = row_bytes * (size_t)lines;
         const uint8_t *data = buf.data() + cr.p;
         if ((size_t)size == expect || im.compression == kNone) {
@@ -426,6 +442,10 @@ int32_t bmfr_exr_read_rgb(const char *path, int32_t width, int32_t height, float
                 return fail(BMFR_ERR_INVALID_ARGUMENT, "%s: bad RLE chunk %d", path, c);
             exr_unpredict(raw);
         } else {
+            // deflate expands at most about 1032:1; a larger claim is malformed
+            // (and would otherwise size the buffer from the header alone)
+            if (expect / 1032 > (size_t)size + 64)
+                return fail(BMFR_ERR_INVALID_ARGUMENT, "%s: ZIP chunk %d claims %zu bytes from %d", path, c, expect, size);
             raw.resize(expect);
             uLongf got = (uLongf)expect;
             if (uncompress(raw.data(), &got, data, (uLong)size) != Z_OK || got != expect)
@@ -456,6 +476,8 @@ int32_t bmfr_exr_read_rgb(const char *path, int32_t width, int32_t height, float
             }
         }
     }
+    for (int y = 0; y < H; ++y)
+        if (!filled[(size_t)y]) return fail(BMFR_ERR_INVALID_ARGUMENT, "%s: line %d is in no chunk", path, y);
     return BMFR_OK;
 }
 

@@ -166,6 +166,64 @@ def test_exr_refusals(tmp_path):
         dataset.read_exr_rgb(ok, 5, 4)
 
 
+@pytest.mark.parametrize("comp", [0, 1, 3])
+def test_exr_reader_survives_corrupted_files(tmp_path, comp):
+    """Seeded byte mutations, truncations and oversized fields of a valid
+    file: the reader either reads an image of the header's size or returns
+    a status; it never reads or writes out of bounds (run under the host
+    AddressSanitizer build as well) and never crashes."""
+    h, w = 20, 11
+    img = _image(h, w, 3 + comp)
+    chans = {c: img[:, :, i].astype(np.float16) for i, c in enumerate("RGB")}
+    good = encode_exr(chans, comp)
+    rng = np.random.default_rng(100 + comp)
+    p = tmp_path / "m.exr"
+    outcomes = {"read": 0, "refused": 0}
+    for trial in range(150):
+        d = bytearray(good)
+        kind = trial % 3
+        if kind == 0:   # a few random bytes anywhere
+            for _ in range(int(rng.integers(1, 6))):
+                d[int(rng.integers(0, len(d)))] = int(rng.integers(0, 256))
+        elif kind == 1:   # truncated
+            d = d[:int(rng.integers(8, len(d)))]
+        else:   # a 32-bit field (size, offset, line number, window) set to an extreme
+            at = int(rng.integers(8, len(d) - 4))
+            d[at:at + 4] = struct.pack("<i", int(rng.choice([-1, 2**31 - 1, -2**31, 1 << 24, 0])))
+        p.write_bytes(bytes(d))
+        try:
+            ww, hh, _ = dataset.exr_info(p)
+            out = dataset.read_exr_rgb(p, ww, hh)
+            assert out.shape == (hh, ww, 3)
+            outcomes["read"] += 1
+        except bmfr_amd.BmfrError:
+            outcomes["refused"] += 1
+    assert outcomes["refused"] > 0 and sum(outcomes.values()) == 150
+
+
+@pytest.mark.parametrize("comp,lines_per_chunk", [(0, 1), (3, 16)])
+@pytest.mark.parametrize("offset", [2**64 - 1, 2**64 - 4, 2**63, None])
+def test_exr_chunk_offset_out_of_file_is_refused(tmp_path, comp, lines_per_chunk, offset):
+    """A chunk offset at or past the end of the file, including values whose
+    sum with a field size wraps a 64-bit index, is refused with a status."""
+    h, w = 20, 11
+    img = _image(h, w, 7)
+    good = bytearray(encode_exr({c: img[:, :, i].astype(np.float16) for i, c in enumerate("RGB")}, comp))
+    chunks = -(-h // lines_per_chunk)
+    # the offset table follows the header; its first entry points just past it
+    table = next(pos for pos in range(8, len(good) - 8)
+                 if struct.unpack_from("<Q", good, pos)[0] == pos + 8 * chunks)
+    bad = bytearray(good)
+    struct.pack_into("<Q", bad, table + 8 * (chunks - 1), len(good) if offset is None else offset)
+    p = tmp_path / "m.exr"
+    p.write_bytes(bytes(bad))
+    with pytest.raises(bmfr_amd.BmfrError) as e:
+        dataset.read_exr_rgb(p, w, h)
+    assert e.value.status == bmfr_amd.ERR_INVALID_ARGUMENT
+    p.write_bytes(bytes(good))
+    assert dataset.read_exr_rgb(p, w, h).shape == (h, w, 3)
+
+
 def test_png_writer_decodes_with_pillow(tmp_path):
     from PIL import Image
     img = _image(13, 17, 9) * 0.3 + 0.5

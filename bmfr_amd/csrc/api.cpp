@@ -355,10 +355,10 @@ int32_t bmfr_measure_lds_read_bandwidth(int32_t steps, double *bytes_per_second)
     if (e == hipSuccess) e = hipEventSynchronize(e1);
     float ms = 0.f;
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-    if (e1) hipEventDestroy(e1);
-    if (e0) hipEventDestroy(e0);
-    if (s) hipStreamDestroy(s);
-    if (out) hipFree(out);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (s) (void)hipStreamDestroy(s);
+    if (out) (void)hipFree(out);
     if (e != hipSuccess) return fail(BMFR_ERR_HIP, "LDS bandwidth probe failed: %s", hipGetErrorString(e));
     *bytes_per_second = (double)groups * 256 * 64 * (double)steps * kReps / (ms * 1e-3);
     return BMFR_OK;

@@ -261,8 +261,10 @@ __device__ __forceinline__ float3 stage4_blend(const StageArgs &a, float3 fc, co
 }
 
 // clamp(pow(max(0, albedo * colour), 0.454545f), 0, 1), bmfr.cl:851-856
-__device__ __forceinline__ float3 tone_map(float3 albedo, float3 ac) {
-    return make_float3(tone_curve(albedo.x * ac.x), tone_curve(albedo.y * ac.y), tone_curve(albedo.z * ac.z));
+template <typename Tables = ToneTables>
+__device__ __forceinline__ float3 tone_map(float3 albedo, float3 ac, const Tables &tab = Tables{}) {
+    return make_float3(tone_curve(albedo.x * ac.x, tab), tone_curve(albedo.y * ac.y, tab),
+                       tone_curve(albedo.z * ac.z, tab));
 }
 
 template <typename GT>
@@ -473,9 +475,29 @@ __device__ __forceinline__ void stage5_body(const StageArgs &a, const StageBuffe
 constexpr int kRing = kBlock + 2;                 // 34x34 tile with the ring
 constexpr int kRingTexels = 4 * kBlock + 4;       // 132 ring texels
 
+// The tone curve's tables read from a copy in LDS (the same values as the
+// constant tables, so the same bits).  This kernel runs at 4 waves per SIMD
+// and its pixels are processed one after another, so an L1/L2 round trip per
+// pixel for the table loads is not hidden; an LDS read is short.
+struct ToneTablesLds {
+    const double2 *lg;   // {c_i, c_i^-y}
+    const double *p2;    // 2^(-y k)
+    __device__ __forceinline__ double c(int i) const { return lg[i].x; }
+    __device__ __forceinline__ double cy(int i) const { return lg[i].y; }
+    __device__ __forceinline__ double pow2(int k) const { return p2[k]; }
+};
+
 template <typename GT, bool CUSTOM = false>
 __device__ __forceinline__ void stage345_body(const StageArgs &a, const StageBuffers &bufs, int blk) {
     __shared__ float4 tile[kRing][kRing];
+    __shared__ double2 tone_log[256];
+    __shared__ double tone_pow2[149];
+    static_assert(sizeof(kToneLog) == sizeof(tone_log) && sizeof(kTonePow2) == sizeof(tone_pow2), "tables");
+    // 256 threads: one log entry each, the first 149 one power of two each
+    tone_log[threadIdx.x] = make_double2(kToneLog[threadIdx.x][0], kToneLog[threadIdx.x][1]);
+    if (threadIdx.x < 149) tone_pow2[threadIdx.x] = kTonePow2[threadIdx.x];
+    __syncthreads();
+    const ToneTablesLds tt{tone_log, tone_pow2};
     const int W = a.width, H = a.height;
     const int2 off = block_offset(a.frame);
     const int by = blk / a.blocks_x, bx = blk - by * a.blocks_x;
@@ -497,7 +519,7 @@ __device__ __forceinline__ void stage345_body(const StageArgs &a, const StageBuf
             const Stage4Gather<GT> gth = stage4_gather<GT>(a, bufs, lp);
             const float3 ac = stage4_blend<GT>(a, stage3_color<GT, CUSTOM>(a, bufs, lp, blk), gth);
             store3((GT *)bufs.out_cur, lp, ac);
-            mine[r] = tone_map(gth.albedo, ac);
+            mine[r] = tone_map(gth.albedo, ac, tt);
             pf[r] = gth.pf;
             t = ycocg_texel(mine[r]);
         }
@@ -519,7 +541,7 @@ __device__ __forceinline__ void stage345_body(const StageArgs &a, const StageBuf
                               (x + kBlock / 2 - off.x) / kBlock;
             const Stage4Gather<GT> gth = stage4_gather<GT>(a, bufs, lp);
             const float3 ac = stage4_blend<GT>(a, stage3_color<GT, CUSTOM>(a, bufs, lp, g), gth);
-            t = ycocg_texel(tone_map(gth.albedo, ac));
+            t = ycocg_texel(tone_map(gth.albedo, ac, tt));
         }
         tile[ry + 1][rx + 1] = t;
     }

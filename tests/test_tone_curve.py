@@ -9,7 +9,7 @@ tests/native/tone_curve_check.cpp runs it on every float in [0, 1] (all
 inputs above 1 clamp to 1) and on the special values, against
   - pow in double precision rounded once to float (correctly rounded up to
     that double rounding): at most 1 ulp, and 1 ulp only where the exact
-    power sits within a few 1e-16 of a rounding boundary (4 of the 1.07e9
+    power sits within a few 1e-16 of a rounding boundary (1 of the 1.07e9
     inputs);
   - the C library's powf (the oracle's tone map): at most 1 ulp.
 A dropped polynomial term, a wrong coefficient, a wrong table entry or a

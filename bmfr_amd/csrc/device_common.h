@@ -144,6 +144,31 @@ struct NoiseTable {
     __device__ __forceinline__ double term(double rv) const { return scale * rv; }
 };
 
+// Correctly rounded float quotients a / d for several numerators sharing one
+// divisor d: one FP64 reciprocal y (v_rcp_f64, then two Newton steps, each
+// with one rounding: relative error below 2^-52), then (float)(a * y).
+// Why that is the IEEE quotient: for floats a and d, a / d is either a float
+// or at least 2^-49 (relative) away from every rounding boundary of float
+// (a - d m, for a boundary m, is a nonzero multiple of 2^-48 |a| / |m|-ish
+// units: |a/d - m| >= 2^(e-48) with 2^e <= |a/d|; for subnormal quotients
+// the distance is at least 2^-174 against an error below 2^-178), and the
+// double product a * y is within 2^-51 of a / d, so both round to the same
+// float.  Zero, infinite and NaN divisors take v_rcp_f64's own result (inf,
+// 0, NaN), which gives a * y the IEEE quotient's value and sign.  Checked on
+// the CPU against IEEE division on 8e8 pairs, including quotients next to
+// rounding boundaries, with the reciprocal's first approximation perturbed
+// by up to 2^-14.  Three quotients cost 15 instructions against about 30.
+__device__ __forceinline__ double quotient_reciprocal(float d) {
+    const double dd = (double)d;
+    const double y0 = __builtin_amdgcn_rcp(dd);
+    double e = __builtin_fma(-dd, y0, 1.0);
+    double y = __builtin_fma(y0, e, y0);
+    e = __builtin_fma(-dd, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    return (__builtin_isfinite(dd) && dd != 0.0) ? y : y0;
+}
+__device__ __forceinline__ float quotient(float a, double y) { return (float)((double)a * y); }
+
 // opencl/bmfr.cl:200-205
 __device__ __forceinline__ float scale_feature(float v, float mn, float mx) {
     if (fabsf(mx - mn) > 1.0f) return (v - mn) / (mx - mn);

@@ -245,11 +245,12 @@ __device__ __forceinline__ float3 stage4_blend(const StageArgs &a, float3 fc, co
             }
         }
         if (total_weight > 0.f) {
-            blend_alpha = 1.f / (float)g.spp;
+            blend_alpha = quotient(1.f, quotient_reciprocal((float)g.spp));   // 1.f / spp
             blend_alpha = fmaxf(blend_alpha, a.second_blend_alpha);
-            pc.x = pc.x / total_weight;
-            pc.y = pc.y / total_weight;
-            pc.z = pc.z / total_weight;
+            const double rw = quotient_reciprocal(total_weight);   // pc / total_weight, IEEE bits
+            pc.x = quotient(pc.x, rw);
+            pc.y = quotient(pc.y, rw);
+            pc.z = quotient(pc.z, rw);
         }
     }
     const float omb = 1.f - blend_alpha;
@@ -409,9 +410,10 @@ __device__ __forceinline__ float3 taa_resolve(const StageArgs &a, float2 pf, flo
             tw = tw + wts[i];
         }
     }
-    pc.x = pc.x / tw;  // unguarded like bmfr.cl:962
-    pc.y = pc.y / tw;
-    pc.z = pc.z / tw;
+    const double rtw = quotient_reciprocal(tw);   // pc / tw, IEEE bits; unguarded like bmfr.cl:962
+    pc.x = quotient(pc.x, rtw);
+    pc.y = quotient(pc.y, rtw);
+    pc.z = quotient(pc.z, rtw);
     const float3 pyc = rgb_to_ycocg(pc);
     float3 cl;
     cl.x = fminf(fmaxf(pyc.x, (mnb.x + mnc.x) / 2.f), (mxb.x + mxc.x) / 2.f);

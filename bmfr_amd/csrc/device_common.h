@@ -155,9 +155,10 @@ struct NoiseTable {
 // double product a * y is within 2^-51 of a / d, so both round to the same
 // float.  Zero, infinite and NaN divisors take v_rcp_f64's own result (inf,
 // 0, NaN), which gives a * y the IEEE quotient's value and sign.  Checked on
-// the CPU against IEEE division on 8e8 pairs, including quotients next to
-// rounding boundaries, with the reciprocal's first approximation perturbed
-// by up to 2^-14.  Three quotients cost 15 instructions against about 30.
+// the host against IEEE division (tests/native/quotient_check.c: 2e8 pairs,
+// including quotients next to rounding boundaries, with the reciprocal's
+// first approximation perturbed by up to 2^-14; one Newton step fails it).
+// Three quotients cost 15 instructions against about 30.
 __device__ __forceinline__ double quotient_reciprocal(float d) {
     const double dd = (double)d;
     const double y0 = __builtin_amdgcn_rcp(dd);

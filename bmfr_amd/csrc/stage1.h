@@ -53,8 +53,9 @@ __device__ __forceinline__ Stage1Pixel stage1_pixel(const StageArgs &a, const St
         float uvx = ((m[0] * wp.x + m[4] * wp.y) + m[8] * wp.z) + m[12] * 1.f;
         float uvy = ((m[1] * wp.x + m[5] * wp.y) + m[9] * wp.z) + m[13] * 1.f;
         const float ww = ((m[3] * wp.x + m[7] * wp.y) + m[11] * wp.z) + m[15] * 1.f;
-        uvx = uvx / ww;
-        uvy = uvy / ww;
+        const double rww = quotient_reciprocal(ww);   // uv / ww with IEEE bits
+        uvx = quotient(uvx, rww);
+        uvy = quotient(uvy, rww);
         uvx = uvx + 1.f;
         uvy = uvy + 1.f;
         uvx = uvx / 2.f;
@@ -114,11 +115,12 @@ __device__ __forceinline__ Stage1Pixel stage1_pixel(const StageArgs &a, const St
             }
         }
         if (total_weight > 0.f) {  // bmfr.cl:421-429
-            prev_col.x = prev_col.x / total_weight;
-            prev_col.y = prev_col.y / total_weight;
-            prev_col.z = prev_col.z / total_weight;
-            sample_spp = sample_spp / total_weight;
-            blend_alpha = 1.f / (sample_spp + 1.f);
+            const double rw = quotient_reciprocal(total_weight);   // IEEE bits of x / total_weight
+            prev_col.x = quotient(prev_col.x, rw);
+            prev_col.y = quotient(prev_col.y, rw);
+            prev_col.z = quotient(prev_col.z, rw);
+            sample_spp = quotient(sample_spp, rw);
+            blend_alpha = quotient(1.f, quotient_reciprocal(sample_spp + 1.f));
             blend_alpha = fmaxf(blend_alpha, a.blend_alpha);
         }
     }

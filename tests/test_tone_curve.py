@@ -1,6 +1,6 @@
 """Stage 4's tone curve, clamp(pow(max(0, v), 0.454545f), 0, 1)
 (opencl/bmfr.cl:851-856), as the library computes it
-(bmfr_amd/csrc/tone_curve.h, a table-driven FP64 log/exp evaluation of our
+(bmfr_amd/csrc/tone_curve.h, a table-driven FP64 evaluation of our
 own).
 
 The header is compiled for the host with g++ (contraction off); it uses only

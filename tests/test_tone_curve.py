@@ -50,3 +50,13 @@ def test_every_float_in_unit_interval(check_output):
 
 def test_special_values(check_output):
     assert check_output["special_bad"] == 0
+
+
+def test_tables_are_the_generator_output():
+    """tone_tables.h is exactly what gen_tone_tables.py writes (every entry
+    the correctly rounded double of its definition, computed at 60 digits)."""
+    import sys
+    csrc = os.path.join(ROOT, "bmfr_amd", "csrc")
+    out = subprocess.run([sys.executable, os.path.join(csrc, "gen_tone_tables.py")], check=True,
+                         capture_output=True, text=True, timeout=120).stdout
+    assert out == open(os.path.join(csrc, "tone_tables.h")).read()

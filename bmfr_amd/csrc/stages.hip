@@ -109,12 +109,57 @@ __device__ __forceinline__ void block_tile(const StageArgs &a, int local, int &x
     g = (int64_t)(ty >> 2) * a.blocks_x + tx;
 }
 
+// The block's feature-scaling reciprocals for the reference's feature set,
+// once per wave: lane l < 6 forms the FP64 reciprocal of max - min of scaled
+// feature l (quotient_reciprocal), and the six are read from those lanes
+// (readlane, SGPR pairs) right there, so that a scaled feature costs three
+// instructions instead of an IEEE division, with the same bits (quotient).
+// Built where every lane of the wave is active and the block is the same for
+// the whole wave.  The reads happen here, not at the use: a readlane inside
+// the per-pixel branch would let the compiler sink the reciprocal's
+// arithmetic into that branch, where lanes outside the image do not compute
+// it.  Configured feature sets (a run-time feature count) keep the division.
+struct ScaleRecips {
+    double s[kScaled];   // wave-uniform
+};
+
+__device__ __forceinline__ double lane_value(double y, int l) {
+    unsigned long long b;
+    __builtin_memcpy(&b, &y, sizeof b);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    const unsigned long long r = ((unsigned long long)hi << 32) | lo;
+    double d;
+    __builtin_memcpy(&d, &r, sizeof d);
+    return d;
+}
+
+__device__ __forceinline__ ScaleRecips scale_recips(const StageBuffers &bufs, int64_t g) {
+    const float *mm = (const float *)bufs.mins_maxs + g * (kScaled * 2);
+    const int l = (int)(threadIdx.x & 63);
+    float d = 1.f;
+    if (l < kScaled) d = mm[l * 2 + 1] - mm[l * 2];
+    const double y = quotient_reciprocal(d);
+    ScaleRecips r;
+#pragma unroll
+    for (int i = 0; i < kScaled; ++i) r.s[i] = lane_value(y, i);
+    return r;
+}
+
+// opencl/bmfr.cl:200-205 with the divisor's reciprocal: the bits of scale_feature
+__device__ __forceinline__ float scale_feature_r(float v, float mn, float mx, double y) {
+    if (fabsf(mx - mn) > 1.0f) return quotient(v - mn, y);
+    return v - mn;
+}
+
 // Filtered colour of in-image pixel lp in block g: sum of block weights
 // times the pixel's (scaled) features, negatives clamped (bmfr.cl:712-750).
-// CUSTOM evaluates the feature set in `a.features`.
+// CUSTOM evaluates the feature set in `a.features`.  With `rec` (the wave's
+// reciprocals for block g, reference set only) the scaling divisions become
+// products.
 template <typename GT, bool CUSTOM = false>
 __device__ __forceinline__ float3 stage3_color_of(const StageArgs &a, const StageBuffers &bufs, float3 wp, float3 n,
-                                                 int64_t g) {
+                                                 int64_t g, const ScaleRecips *rec = nullptr) {
     const float *__restrict__ weights = (const float *)bufs.weights;
     const float *__restrict__ mins_maxs = (const float *)bufs.mins_maxs;
     float cx = 0.f, cy = 0.f, cz = 0.f;
@@ -141,7 +186,11 @@ __device__ __forceinline__ float3 stage3_color_of(const StageArgs &a, const Stag
 #pragma unroll
         for (int f = 0; f < kFeatures; ++f) {
             float v = feat[f];
-            if (f >= kNotScaled) v = scale_feature(v, mm[(f - kNotScaled) * 2], mm[(f - kNotScaled) * 2 + 1]);
+            if (f >= kNotScaled) {
+                const int i = f - kNotScaled;
+                v = rec ? scale_feature_r(v, mm[i * 2], mm[i * 2 + 1], rec->s[i])
+                        : scale_feature(v, mm[i * 2], mm[i * 2 + 1]);
+            }
             cx = cx + w[f * 3 + 0] * v;
             cy = cy + w[f * 3 + 1] * v;
             cz = cz + w[f * 3 + 2] * v;
@@ -152,9 +201,9 @@ __device__ __forceinline__ float3 stage3_color_of(const StageArgs &a, const Stag
 
 template <typename GT, bool CUSTOM = false>
 __device__ __forceinline__ float3 stage3_color(const StageArgs &a, const StageBuffers &bufs, int64_t lp,
-                                              int64_t g) {
+                                              int64_t g, const ScaleRecips *rec = nullptr) {
     return stage3_color_of<GT, CUSTOM>(a, bufs, load3((const GT *)bufs.positions_cur, lp),
-                                       load3((const GT *)bufs.normals_cur, lp), g);
+                                       load3((const GT *)bufs.normals_cur, lp), g, rec);
 }
 
 template <typename GT, bool CUSTOM = false>
@@ -162,9 +211,11 @@ __device__ __forceinline__ void stage3_body(const StageArgs &a, const StageBuffe
     int x, y;
     int64_t g;
     block_tile(a, local, x, y, g);
+    ScaleRecips rec;
+    if constexpr (!CUSTOM) rec = scale_recips(bufs, g);   // all lanes active; g is the work-group's
     if (x < 0 || y < 0 || x >= a.width || y >= a.height) return;
     const int64_t lp = (int64_t)y * a.width + x;
-    const float3 c = stage3_color<GT, CUSTOM>(a, bufs, lp, g);
+    const float3 c = stage3_color<GT, CUSTOM>(a, bufs, lp, g, CUSTOM ? nullptr : &rec);
     float *__restrict__ output = (float *)bufs.filtered;
     output[lp * 3 + 0] = c.x;
     output[lp * 3 + 1] = c.y;
@@ -298,10 +349,12 @@ __device__ __forceinline__ void stage34_body(const StageArgs &a, const StageBuff
     int x, y;
     int64_t g;
     block_tile(a, local, x, y, g);
+    ScaleRecips rec;
+    if constexpr (!CUSTOM) rec = scale_recips(bufs, g);   // all lanes active; g is the work-group's
     if (x < 0 || y < 0 || x >= a.width || y >= a.height) return;
     const int64_t lp = (int64_t)y * a.width + x;
     const Stage4Gather<GT> gth = stage4_gather<GT>(a, bufs, lp);
-    stage4_finish<GT>(a, bufs, lp, stage3_color<GT, CUSTOM>(a, bufs, lp, g), gth);
+    stage4_finish<GT>(a, bufs, lp, stage3_color<GT, CUSTOM>(a, bufs, lp, g, CUSTOM ? nullptr : &rec), gth);
 }
 
 template <typename GT, bool CUSTOM>
@@ -500,6 +553,8 @@ __device__ __forceinline__ void stage345_body(const StageArgs &a, const StageBuf
     if (threadIdx.x < 149) tone_pow2[threadIdx.x] = kTonePow2[threadIdx.x];
     __syncthreads();
     const ToneTablesLds tt{tone_log, tone_pow2};
+    ScaleRecips rec;
+    if constexpr (!CUSTOM) rec = scale_recips(bufs, blk);   // all lanes active; blk is the work-group's
     const int W = a.width, H = a.height;
     const int2 off = block_offset(a.frame);
     const int by = blk / a.blocks_x, bx = blk - by * a.blocks_x;
@@ -519,7 +574,8 @@ __device__ __forceinline__ void stage345_body(const StageArgs &a, const StageBuf
         if (x >= 0 && y >= 0 && x < W && y < H) {
             const int64_t lp = (int64_t)y * W + x;
             const Stage4Gather<GT> gth = stage4_gather<GT>(a, bufs, lp);
-            const float3 ac = stage4_blend<GT>(a, stage3_color<GT, CUSTOM>(a, bufs, lp, blk), gth);
+            const float3 ac =
+                stage4_blend<GT>(a, stage3_color<GT, CUSTOM>(a, bufs, lp, blk, CUSTOM ? nullptr : &rec), gth);
             store3((GT *)bufs.out_cur, lp, ac);
             mine[r] = tone_map(gth.albedo, ac, tt);
             pf[r] = gth.pf;

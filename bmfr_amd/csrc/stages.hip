@@ -423,6 +423,23 @@ __device__ __forceinline__ TaaHistory<GT> taa_history(const StageArgs &a, const 
     return h;
 }
 
+// Minimum and maximum of three floats as v_min3_f32 / v_max3_f32 (IEEE mode:
+// a quiet NaN operand is skipped, -0 orders below +0, so the grouping does
+// not change the result).  Written out because the compiler first
+// canonicalises every operand of fminf/fmaxf it cannot prove canonical (one
+// v_max_f32 x, x, x per texel read from LDS); the tile holds arithmetic
+// results and the quiet NaN of out-of-image texels, which are canonical.
+__device__ __forceinline__ float vmin3(float a, float b, float c) {
+    float r;
+    asm("v_min3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+__device__ __forceinline__ float vmax3(float a, float b, float c) {
+    float r;
+    asm("v_max3_f32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
 // TAA of one pixel (bmfr.cl:884-973): `mine` is its tone-mapped colour, `nb`
 // the top-left texel of its 3x3 YCoCg neighbourhood in an LDS tile of row
 // stride STRIDE (NaN outside the image).
@@ -437,18 +454,18 @@ __device__ __forceinline__ float3 taa_resolve(const StageArgs &a, float2 pf, flo
         for (int dx = 0; dx < 3; ++dx) nb[dy][dx] = nbp[dy * STRIDE + dx];
     // cross: centre and its 4-neighbours; box: all 9
     float3 mnc, mxc, mnb, mxb;
-    mnc.x = fminf(fminf(fminf(nb[0][1].x, nb[1][0].x), fminf(nb[1][1].x, nb[1][2].x)), nb[2][1].x);
-    mnc.y = fminf(fminf(fminf(nb[0][1].y, nb[1][0].y), fminf(nb[1][1].y, nb[1][2].y)), nb[2][1].y);
-    mnc.z = fminf(fminf(fminf(nb[0][1].z, nb[1][0].z), fminf(nb[1][1].z, nb[1][2].z)), nb[2][1].z);
-    mxc.x = fmaxf(fmaxf(fmaxf(nb[0][1].x, nb[1][0].x), fmaxf(nb[1][1].x, nb[1][2].x)), nb[2][1].x);
-    mxc.y = fmaxf(fmaxf(fmaxf(nb[0][1].y, nb[1][0].y), fmaxf(nb[1][1].y, nb[1][2].y)), nb[2][1].y);
-    mxc.z = fmaxf(fmaxf(fmaxf(nb[0][1].z, nb[1][0].z), fmaxf(nb[1][1].z, nb[1][2].z)), nb[2][1].z);
-    mnb.x = fminf(fminf(mnc.x, fminf(nb[0][0].x, nb[0][2].x)), fminf(nb[2][0].x, nb[2][2].x));
-    mnb.y = fminf(fminf(mnc.y, fminf(nb[0][0].y, nb[0][2].y)), fminf(nb[2][0].y, nb[2][2].y));
-    mnb.z = fminf(fminf(mnc.z, fminf(nb[0][0].z, nb[0][2].z)), fminf(nb[2][0].z, nb[2][2].z));
-    mxb.x = fmaxf(fmaxf(mxc.x, fmaxf(nb[0][0].x, nb[0][2].x)), fmaxf(nb[2][0].x, nb[2][2].x));
-    mxb.y = fmaxf(fmaxf(mxc.y, fmaxf(nb[0][0].y, nb[0][2].y)), fmaxf(nb[2][0].y, nb[2][2].y));
-    mxb.z = fmaxf(fmaxf(mxc.z, fmaxf(nb[0][0].z, nb[0][2].z)), fmaxf(nb[2][0].z, nb[2][2].z));
+    mnc.x = vmin3(vmin3(nb[0][1].x, nb[1][0].x, nb[1][1].x), nb[1][2].x, nb[2][1].x);
+    mnc.y = vmin3(vmin3(nb[0][1].y, nb[1][0].y, nb[1][1].y), nb[1][2].y, nb[2][1].y);
+    mnc.z = vmin3(vmin3(nb[0][1].z, nb[1][0].z, nb[1][1].z), nb[1][2].z, nb[2][1].z);
+    mxc.x = vmax3(vmax3(nb[0][1].x, nb[1][0].x, nb[1][1].x), nb[1][2].x, nb[2][1].x);
+    mxc.y = vmax3(vmax3(nb[0][1].y, nb[1][0].y, nb[1][1].y), nb[1][2].y, nb[2][1].y);
+    mxc.z = vmax3(vmax3(nb[0][1].z, nb[1][0].z, nb[1][1].z), nb[1][2].z, nb[2][1].z);
+    mnb.x = vmin3(vmin3(mnc.x, nb[0][0].x, nb[0][2].x), nb[2][0].x, nb[2][2].x);
+    mnb.y = vmin3(vmin3(mnc.y, nb[0][0].y, nb[0][2].y), nb[2][0].y, nb[2][2].y);
+    mnb.z = vmin3(vmin3(mnc.z, nb[0][0].z, nb[0][2].z), nb[2][0].z, nb[2][2].z);
+    mxb.x = vmax3(vmax3(mxc.x, nb[0][0].x, nb[0][2].x), nb[2][0].x, nb[2][2].x);
+    mxb.y = vmax3(vmax3(mxc.y, nb[0][0].y, nb[0][2].y), nb[2][0].y, nb[2][2].y);
+    mxb.z = vmax3(vmax3(mxc.z, nb[0][0].z, nb[0][2].z), nb[2][0].z, nb[2][2].z);
     float3 pc = make_float3(0.f, 0.f, 0.f);
     float tw = 0.f;
     const float fx = pf.x - (float)h.ix, fy = pf.y - (float)h.iy;

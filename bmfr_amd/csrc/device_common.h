@@ -155,7 +155,7 @@ struct NoiseTable {
 // double product a * y is within 2^-51 of a / d, so both round to the same
 // float.  Zero, infinite and NaN divisors take v_rcp_f64's own result (inf,
 // 0, NaN), which gives a * y the IEEE quotient's value and sign.  Checked on
-// the host against IEEE division (tests/native/quotient_check.c: 2e8 pairs,
+// the host against IEEE division (tests/native/quotient_check.c: 4e8 pairs,
 // including quotients next to rounding boundaries, with the reciprocal's
 // first approximation perturbed by up to 2^-14; one Newton step fails it).
 // Three quotients cost 15 instructions against about 30.

@@ -9,11 +9,12 @@ hand-written HIP kernel.  Failures raise ``BmfrError`` carrying the C
 status code.
 """
 import ctypes
+import weakref
 
 import numpy as np
 
 from . import _lib
-from ._lib import BmfrParams, BmfrSizes
+from ._lib import BmfrFrameRequest, BmfrParams, BmfrServerEvent, BmfrSizes
 
 # enum bmfr_status
 OK = 0
@@ -29,6 +30,12 @@ QR_FP32 = 0
 QR_KAHAN = 1
 QR_KAHAN_FP64 = 2
 FUSE_NONE, FUSE_34, FUSE_345 = 0, 1, 2   # bmfr_fuse_level
+# bmfr_surface_format: (id, bytes per pixel, numpy dtype, channels)
+SURFACE_RGB_F32, SURFACE_RGBA_F32, SURFACE_RGBA_F16, SURFACE_RGBA_U8 = 0, 1, 2, 3
+SURFACE_LAYOUT = {SURFACE_RGB_F32: (12, np.float32, 3), SURFACE_RGBA_F32: (16, np.float32, 4),
+                  SURFACE_RGBA_F16: (8, np.float16, 4), SURFACE_RGBA_U8: (4, np.uint8, 4)}
+# bmfr_event_kind
+EVENT_FRAME_DONE, EVENT_FRAME_FAILED, EVENT_STREAM_ADMITTED = 1, 2, 3
 
 # enum bmfr_buffer: id -> (name, element dtype key, elements per pixel kind)
 BUFFERS = {
@@ -69,6 +76,11 @@ def _f32(a, n=None):
 
 def device_count():
     return _lib.load().bmfr_device_count()
+
+
+def clock_seconds():
+    """Seconds on the request loop's clock (bmfr_clock_seconds)."""
+    return _lib.load().bmfr_clock_seconds()
 
 
 def lds_read_bandwidth(steps=4096):
@@ -158,6 +170,26 @@ class Stream:
         bmfr_copy_frame_inputs_device."""
         _check(self._lib.bmfr_copy_frame_inputs_device(
             self._h, *[ctypes.c_void_p(p) for p in (albedo, normals, positions, noisy)]))
+
+    def bind_device_inputs(self, albedo, normals, positions, noisy):
+        """Zero-copy inputs of the next frame: caller-owned device buffers
+        (ints; all None unbinds); see bmfr_bind_device_inputs."""
+        _check(self._lib.bmfr_bind_device_inputs(
+            self._h, *[ctypes.c_void_p(p) for p in (albedo, normals, positions, noisy)]))
+
+    def bind_output_surface(self, surface, fmt=SURFACE_RGB_F32, row_pitch=None):
+        """Every later frame's TAA result also goes to this device surface
+        (an int pointer; None unbinds); row_pitch defaults to a packed row."""
+        if row_pitch is None:
+            row_pitch = self.params.width * SURFACE_LAYOUT[fmt][0]
+        _check(self._lib.bmfr_bind_output_surface(self._h, ctypes.c_void_p(surface), int(fmt), int(row_pitch)))
+
+    @property
+    def hip_stream(self):
+        """The hipStream_t (int) the stream's work runs on."""
+        p = ctypes.c_void_p()
+        _check(self._lib.bmfr_stream_hip_stream(self._h, ctypes.byref(p)))
+        return p.value
 
     # -- the five stages (opencl/bmfr.cpp:429-476) --------------------------------
     def accumulate_noisy_data(self, frame, prev_frame_camera_matrix, pixel_offset):
@@ -344,19 +376,25 @@ class Batch:
 
 
 class Server:
-    """Streams admitted by an HBM budget and stepped in dynamic batches
-    (bmfr_server_* in include/bmfr_hip.h)."""
+    """Streams admitted by an HBM budget and stepped in dynamic batches,
+    explicitly (step) or by the library's request loop (submit / poll); see
+    bmfr_server_* in include/bmfr_hip.h.  Closing the server closes every
+    stream it issued: their Python handles are invalidated."""
 
     def __init__(self, budget_bytes):
         self._lib = _lib.load()
         h = ctypes.c_void_p()
         _check(self._lib.bmfr_server_create(int(budget_bytes), ctypes.byref(h)))
         self._h = h
+        self._issued = weakref.WeakValueDictionary()   # stream handle -> Stream
 
     def close(self):
         if self._h:
             _check(self._lib.bmfr_server_destroy(self._h))
             self._h = None
+            for st in list(self._issued.values()):
+                st._h = None
+            self._issued.clear()
 
     def __del__(self):
         try:
@@ -369,21 +407,44 @@ class Server:
         _check(self._lib.bmfr_server_usage(self._h, ctypes.byref(u), ctypes.byref(b), ctypes.byref(n)))
         return u.value, b.value, n.value
 
+    def _wrap(self, h, params):
+        st = Stream.__new__(Stream)
+        st._lib, st.params, st._h = self._lib, params, ctypes.c_void_p(h)
+        sz = BmfrSizes()
+        _check(self._lib.bmfr_stream_sizes(st._h, ctypes.byref(sz)))
+        st.sizes = sz
+        st._owner = self
+        self._issued[h] = st
+        return st
+
     def open_stream(self, params):
         """A server-owned Stream; raises BmfrError(ERR_OUT_OF_BUDGET) when it
         does not fit the budget."""
         h = ctypes.c_void_p()
         _check(self._lib.bmfr_server_open_stream(self._h, ctypes.byref(params), ctypes.byref(h)))
-        st = Stream.__new__(Stream)
-        st._lib, st.params, st._h = self._lib, params, h
-        sz = BmfrSizes()
-        _check(self._lib.bmfr_stream_sizes(h, ctypes.byref(sz)))
-        st.sizes = sz
-        st._owner = self
-        return st
+        return self._wrap(h.value, params)
+
+    def request_stream(self, params, tag):
+        """Admission FIFO: a Stream when admitted now, else None (the stream
+        arrives later as an EVENT_STREAM_ADMITTED event with `tag`, which
+        poll() turns into a Stream)."""
+        h = ctypes.c_void_p()
+        _check(self._lib.bmfr_server_request_stream(self._h, ctypes.byref(params), int(tag), ctypes.byref(h)))
+        self._pending_params = getattr(self, "_pending_params", {})
+        if h.value:
+            return self._wrap(h.value, params)
+        self._pending_params[int(tag)] = params
+        return None
+
+    def cancel_request(self, tag):
+        f = ctypes.c_int32()
+        _check(self._lib.bmfr_server_cancel_request(self._h, int(tag), ctypes.byref(f)))
+        getattr(self, "_pending_params", {}).pop(int(tag), None)
+        return bool(f.value)
 
     def close_stream(self, st):
         _check(self._lib.bmfr_server_close_stream(self._h, st._h))
+        self._issued.pop(st._h.value if isinstance(st._h, ctypes.c_void_p) else st._h, None)
         st._h = None
 
     def step(self, streams, frames, matrices, offsets):
@@ -392,7 +453,7 @@ class Server:
         fr = np.ascontiguousarray(frames, np.int32)
         m = _f32(matrices, 16 * n)
         o = _f32(offsets, 2 * n)
-        sid = ctypes.c_int32()
+        sid = ctypes.c_int64()
         _check(self._lib.bmfr_server_step(self._h, n, arr, fr.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                                           _fp(m), _fp(o), ctypes.byref(sid)))
         return sid.value
@@ -400,7 +461,7 @@ class Server:
     def query(self, step_id):
         """(done, gpu_ms) of a step id returned by step()."""
         done, ms = ctypes.c_int32(), ctypes.c_float()
-        _check(self._lib.bmfr_server_step_query(self._h, step_id, ctypes.byref(done), ctypes.byref(ms)))
+        _check(self._lib.bmfr_server_step_query(self._h, int(step_id), ctypes.byref(done), ctypes.byref(ms)))
         return bool(done.value), ms.value
 
     def synchronize(self):
@@ -408,6 +469,49 @@ class Server:
 
     def set_fused(self, level=2):
         _check(self._lib.bmfr_server_set_fused(self._h, int(level)))
+
+    def set_policy(self, max_in_flight=2, max_streams_per_step=64):
+        _check(self._lib.bmfr_server_set_policy(self._h, int(max_in_flight), int(max_streams_per_step)))
+
+    def submit(self, stream, frame, prev_frame_camera_matrix, pixel_offset, inputs=None, surface=None,
+               surface_format=SURFACE_RGB_F32, surface_row_pitch=None, producer_stream=None, t_arrival=None,
+               tag=0):
+        """Queues one frame of `stream` for the request loop.  `inputs`: four
+        device pointers (albedo, normals, positions, noisy) or None for the
+        stream's own input buffers."""
+        r = BmfrFrameRequest()
+        r.stream = stream.handle.value if isinstance(stream.handle, ctypes.c_void_p) else stream.handle
+        r.frame_number = int(frame)
+        r.prev_frame_camera_matrix[:] = [float(v) for v in _f32(prev_frame_camera_matrix, 16)]
+        r.pixel_offset[:] = [float(v) for v in _f32(pixel_offset, 2)]
+        if inputs is not None:
+            r.albedo, r.normals, r.positions, r.noisy = inputs
+        if surface is not None:
+            r.surface = surface
+            r.surface_format = int(surface_format)
+            r.surface_row_pitch = int(surface_row_pitch if surface_row_pitch is not None
+                                      else stream.params.width * SURFACE_LAYOUT[surface_format][0])
+        r.producer_stream = producer_stream
+        r.t_arrival = clock_seconds() if t_arrival is None else float(t_arrival)
+        r.tag = int(tag)
+        _check(self._lib.bmfr_server_submit(self._h, ctypes.byref(r)))
+
+    def poll(self, max_events=256, timeout_us=0):
+        """Events since the last poll, as dicts; an admitted stream's event
+        carries its Stream under "stream"."""
+        buf = (BmfrServerEvent * max_events)()
+        n = ctypes.c_int32()
+        _check(self._lib.bmfr_server_poll(self._h, buf, max_events, int(timeout_us), ctypes.byref(n)))
+        out = []
+        for e in buf[:n.value]:
+            d = {k: getattr(e, k) for k, _ in BmfrServerEvent._fields_}
+            if e.kind == EVENT_STREAM_ADMITTED:
+                params = getattr(self, "_pending_params", {}).pop(int(e.tag), None)
+                d["stream"] = self._wrap(e.stream, params) if e.status == OK else None
+            else:
+                d["stream"] = self._issued.get(e.stream)
+            out.append(d)
+        return out
 
 
 class Scene:
@@ -446,6 +550,11 @@ class Scene:
 
     def render_to_stream(self, frame, stream):
         _check(self._lib.bmfr_scene_render_to_stream(self._h, frame, stream.handle))
+
+    def render_device(self, frame, storage, albedo, normals, positions, noisy, hip_stream=None):
+        """Renders a frame into caller-owned device buffers (int pointers)."""
+        _check(self._lib.bmfr_scene_render_device(self._h, frame, int(storage), *[
+            ctypes.c_void_p(p) for p in (albedo, normals, positions, noisy, hip_stream)]))
 
     def frame_args(self, frame):
         """(prev_frame_camera_matrix, pixel_offset) of `frame` as the reference

@@ -50,6 +50,44 @@ class BmfrSizes(ctypes.Structure):
     ]
 
 
+class BmfrFrameRequest(ctypes.Structure):
+    """struct bmfr_frame_request: one frame of a server stream for the
+    request loop (bmfr_server_submit)."""
+    _fields_ = [
+        ("stream", ctypes.c_void_p),
+        ("frame_number", ctypes.c_int32),
+        ("prev_frame_camera_matrix", ctypes.c_float * 16),
+        ("pixel_offset", ctypes.c_float * 2),
+        ("albedo", ctypes.c_void_p),
+        ("normals", ctypes.c_void_p),
+        ("positions", ctypes.c_void_p),
+        ("noisy", ctypes.c_void_p),
+        ("surface", ctypes.c_void_p),
+        ("surface_format", ctypes.c_int32),
+        ("surface_row_pitch", ctypes.c_int64),
+        ("producer_stream", ctypes.c_void_p),
+        ("t_arrival", ctypes.c_double),
+        ("tag", ctypes.c_uint64),
+    ]
+
+
+class BmfrServerEvent(ctypes.Structure):
+    """struct bmfr_server_event (bmfr_server_poll)."""
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("status", ctypes.c_int32),
+        ("stream", ctypes.c_void_p),
+        ("tag", ctypes.c_uint64),
+        ("frame_number", ctypes.c_int32),
+        ("step_streams", ctypes.c_int32),
+        ("step_id", ctypes.c_int64),
+        ("t_arrival", ctypes.c_double),
+        ("t_launch", ctypes.c_double),
+        ("t_done", ctypes.c_double),
+        ("step_gpu_ms", ctypes.c_float),
+    ]
+
+
 # (name, restype, argtypes) of every exported entry point.
 SIGNATURES = [
     ("bmfr_api_version", c_int32, []),
@@ -67,6 +105,9 @@ SIGNATURES = [
     ("bmfr_set_fitter_config", c_int32, [ctypes.c_void_p, c_int32, c_int32]),
     ("bmfr_upload_frame", c_int32, [ctypes.c_void_p, c_float_p, c_float_p, c_float_p, c_float_p]),
     ("bmfr_copy_frame_inputs_device", c_int32, [ctypes.c_void_p] * 5),
+    ("bmfr_bind_device_inputs", c_int32, [ctypes.c_void_p] * 5),
+    ("bmfr_bind_output_surface", c_int32, [ctypes.c_void_p, ctypes.c_void_p, c_int32, ctypes.c_int64]),
+    ("bmfr_stream_hip_stream", c_int32, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
     ("bmfr_accumulate_noisy_data", c_int32, [ctypes.c_void_p, c_int32, c_float_p, c_float_p]),
     ("bmfr_fitter", c_int32, [ctypes.c_void_p, c_int32]),
     ("bmfr_weighted_sum", c_int32, [ctypes.c_void_p, c_int32]),
@@ -108,13 +149,21 @@ SIGNATURES = [
     ("bmfr_server_open_stream", c_int32, [ctypes.c_void_p, ctypes.POINTER(BmfrParams),
                                           ctypes.POINTER(ctypes.c_void_p)]),
     ("bmfr_server_close_stream", c_int32, [ctypes.c_void_p, ctypes.c_void_p]),
+    ("bmfr_server_request_stream", c_int32, [ctypes.c_void_p, ctypes.POINTER(BmfrParams), ctypes.c_uint64,
+                                             ctypes.POINTER(ctypes.c_void_p)]),
+    ("bmfr_server_cancel_request", c_int32, [ctypes.c_void_p, ctypes.c_uint64, ctypes.POINTER(c_int32)]),
     ("bmfr_server_step", c_int32, [ctypes.c_void_p, c_int32, ctypes.POINTER(ctypes.c_void_p),
                                    ctypes.POINTER(c_int32), c_float_p, c_float_p,
-                                   ctypes.POINTER(c_int32)]),
-    ("bmfr_server_step_query", c_int32, [ctypes.c_void_p, c_int32, ctypes.POINTER(c_int32),
+                                   ctypes.POINTER(ctypes.c_int64)]),
+    ("bmfr_server_step_query", c_int32, [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(c_int32),
                                          c_float_p]),
     ("bmfr_server_synchronize", c_int32, [ctypes.c_void_p]),
     ("bmfr_server_set_fused", c_int32, [ctypes.c_void_p, c_int32]),
+    ("bmfr_server_set_policy", c_int32, [ctypes.c_void_p, c_int32, c_int32]),
+    ("bmfr_clock_seconds", ctypes.c_double, []),
+    ("bmfr_server_submit", c_int32, [ctypes.c_void_p, ctypes.POINTER(BmfrFrameRequest)]),
+    ("bmfr_server_poll", c_int32, [ctypes.c_void_p, ctypes.POINTER(BmfrServerEvent), c_int32, c_int32,
+                                   ctypes.POINTER(c_int32)]),
     ("bmfr_scene_create", c_int32, [ctypes.c_uint64, c_int32, ctypes.c_float, c_int32, c_int32,
                                     c_int32, ctypes.POINTER(ctypes.c_void_p)]),
     ("bmfr_scene_destroy", c_int32, [ctypes.c_void_p]),
@@ -122,6 +171,7 @@ SIGNATURES = [
     ("bmfr_scene_render_host", c_int32, [ctypes.c_void_p, c_int32, c_float_p, c_float_p,
                                          c_float_p, c_float_p]),
     ("bmfr_scene_render_to_stream", c_int32, [ctypes.c_void_p, c_int32, ctypes.c_void_p]),
+    ("bmfr_scene_render_device", c_int32, [ctypes.c_void_p, c_int32, c_int32] + [ctypes.c_void_p] * 5),
     ("bmfr_exr_info", c_int32, [ctypes.c_char_p, ctypes.POINTER(c_int32), ctypes.POINTER(c_int32),
                                 ctypes.POINTER(c_int32)]),
     ("bmfr_exr_read_rgb", c_int32, [ctypes.c_char_p, c_int32, c_int32, c_float_p]),

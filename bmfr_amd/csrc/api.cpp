@@ -11,9 +11,13 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <condition_variable>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <deque>
+#include <mutex>
 #include <new>
 #include <string>
 #include <thread>
@@ -24,6 +28,7 @@
 #include "kernels.h"
 #include "scene.h"
 #include "scene_launch.h"
+#include "surface.h"
 
 namespace bmfr_host {
 
@@ -162,6 +167,12 @@ void sizes_of(const bmfr_params &p, bmfr_sizes &s, size_t *slot_bytes) {
     s.device_bytes = (int64_t)total;
 }
 
+// A frame submitted to a server's request loop, waiting for its step.
+struct ServerFrame {
+    bmfr_frame_request req;
+    hipEvent_t producer = nullptr;   // recorded on req.producer_stream at submit (or null)
+};
+
 }  // namespace
 
 struct bmfr_stream_s {
@@ -182,6 +193,23 @@ struct bmfr_stream_s {
     bool recorded[kStages]{};
     bmfr::FitterConfig fitter{16, 2};  // one wave per block, two blocks per work-group
     int fuse = BMFR_FUSE_34;   // how bmfr_run_frame runs stages 3-5 (bmfr_fuse_level)
+    // Renderer handoff: the current frame's caller-bound inputs (albedo,
+    // normals, positions, noisy; null: the stream's own buffers), the
+    // previous frame's normals and positions when they were caller-bound
+    // (null: the stream's own previous buffers), and the output surface.
+    const void *bind_in[4]{};
+    const void *prev_in[2]{};
+    void *surface = nullptr;
+    int32_t surface_format = 0;
+    int64_t surface_pitch = 0;
+    // Ownership: batches the stream is a member of (0 or 1), the server
+    // that owns it.  Guarded by the owning server's mutex: the request loop's
+    // queue of submitted frames and the frames in launched steps.
+    int batch_refs = 0;
+    bmfr_server owner = nullptr;
+    std::deque<ServerFrame> queue;
+    int frames_in_flight = 0;
+    bool closing = false;
 
     int cur(int a_slot) const { return swapped ? a_slot : a_slot + 1; }
     int prev(int a_slot) const { return swapped ? a_slot + 1 : a_slot; }
@@ -215,12 +243,12 @@ struct bmfr_stream_s {
 
     bmfr::StageBuffers buffers() const {
         bmfr::StageBuffers b;
-        b.albedo = slot[S_ALBEDO];
-        b.normals_cur = slot[cur(S_NORMALS_A)];
-        b.normals_prev = slot[prev(S_NORMALS_A)];
-        b.positions_cur = slot[cur(S_POSITIONS_A)];
-        b.positions_prev = slot[prev(S_POSITIONS_A)];
-        b.noisy_in = slot[S_NOISY_IN];
+        b.albedo = bind_in[0] ? const_cast<void *>(bind_in[0]) : slot[S_ALBEDO];
+        b.normals_cur = bind_in[1] ? const_cast<void *>(bind_in[1]) : slot[cur(S_NORMALS_A)];
+        b.normals_prev = prev_in[0] ? const_cast<void *>(prev_in[0]) : slot[prev(S_NORMALS_A)];
+        b.positions_cur = bind_in[2] ? const_cast<void *>(bind_in[2]) : slot[cur(S_POSITIONS_A)];
+        b.positions_prev = prev_in[1] ? const_cast<void *>(prev_in[1]) : slot[prev(S_POSITIONS_A)];
+        b.noisy_in = bind_in[3] ? const_cast<void *>(bind_in[3]) : slot[S_NOISY_IN];
         b.noisy_acc_cur = slot[cur(S_NOISY_ACC_A)];
         b.noisy_acc_prev = slot[prev(S_NOISY_ACC_A)];
         b.spp_cur = slot[cur(S_SPP_A)];
@@ -238,7 +266,59 @@ struct bmfr_stream_s {
         b.result_prev = slot[prev(S_RESULT_A)];
         b.noise = slot[S_NOISE];
         b.flags = slot[S_FLAGS];
+        b.surface = surface;
         return b;
+    }
+
+    // End of a frame (opencl/bmfr.cpp:482-484): the double buffers swap, and
+    // this frame's current normals and positions become the next frame's
+    // previous ones (caller-bound or the stream's own); a binding of inputs
+    // covers one frame.
+    void finish_frame() {
+        prev_in[0] = bind_in[1];
+        prev_in[1] = bind_in[2];
+        for (auto &p : bind_in) p = nullptr;
+        swapped = !swapped;
+    }
+
+    // The batched form of the frame: constants and both halves of the
+    // double buffers (BatchStream), and the step's parameters with this
+    // frame's resolved inputs and surface (BatchStep).
+    void batch_stream(bmfr::BatchStream &d) const {
+        d.args = args(0);
+        d.prev_pixel = slot[S_PREV_PIXEL];
+        d.accept = slot[S_ACCEPT];
+        d.tmp_data = slot[S_TMP];
+        d.weights = slot[S_WEIGHTS];
+        d.mins_maxs = slot[S_MINS_MAXS];
+        d.filtered = slot[S_FILTERED];
+        d.tone_mapped = slot[S_TONE];
+        d.noise = slot[S_NOISE];
+        d.flags = slot[S_FLAGS];
+        for (int h = 0; h < 2; ++h) {
+            d.noisy_acc[h] = slot[S_NOISY_ACC_A + h];
+            d.spp[h] = slot[S_SPP_A + h];
+            d.out[h] = slot[S_OUT_A + h];
+            d.result[h] = slot[S_RESULT_A + h];
+        }
+    }
+    void batch_step(int frame, const float *matrix, const float *offset, void *frame_surface, int32_t frame_format,
+                    int64_t frame_pitch, bmfr::BatchStep &st) const {
+        st.frame = frame;
+        st.cur = swapped ? 0 : 1;  // Double_buffer::current
+        std::memcpy(st.matrix, matrix, sizeof(st.matrix));
+        st.pixel_offset_x = offset[0];
+        st.pixel_offset_y = offset[1];
+        const bmfr::StageBuffers b = buffers();
+        st.albedo = b.albedo;
+        st.normals_cur = b.normals_cur;
+        st.normals_prev = b.normals_prev;
+        st.positions_cur = b.positions_cur;
+        st.positions_prev = b.positions_prev;
+        st.noisy_in = b.noisy_in;
+        st.surface = frame_surface ? frame_surface : surface;
+        st.surface_format = frame_surface ? frame_format : surface_format;
+        st.surface_pitch = frame_surface ? frame_pitch : surface_pitch;
     }
 
     bmfr::StageArgs args(int frame) const {
@@ -265,6 +345,8 @@ struct bmfr_stream_s {
             a.features.not_scaled = p.features_not_scaled;
             memcpy(a.features.factor, p.feature_factors, sizeof(a.features.factor));
         }
+        a.surface_format = surface_format;
+        a.surface_pitch = surface_pitch;
         return a;
     }
 };
@@ -285,6 +367,23 @@ int32_t check_frame(int32_t frame) {
     if (frame < 0) return fail(BMFR_ERR_INVALID_ARGUMENT, "frame_number %d < 0", frame);
     return BMFR_OK;
 }
+
+// An output surface of stream s: a known format, a pitch that holds a row
+// and keeps every pixel aligned, an aligned base.
+int32_t check_surface(bmfr_stream s, const void *surface, int32_t format, int64_t pitch) {
+    if (format < bmfr::kSurfaceRgbF32 || format > bmfr::kSurfaceRgbaU8)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "unknown surface format %d", format);
+    const int64_t row = (int64_t)s->p.width * bmfr::surface_pixel_bytes(format);
+    const int al = bmfr::surface_alignment(format);
+    if (pitch < row || pitch % al != 0)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "surface row pitch %lld: needs >= %lld bytes and a multiple of %d",
+                    (long long)pitch, (long long)row, al);
+    if (((uintptr_t)surface % (uintptr_t)al) != 0)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "surface is not %d-byte aligned", al);
+    return BMFR_OK;
+}
+
+void destroy_stream(bmfr_stream s);
 
 template <typename F>
 int32_t timed_stage(bmfr_stream s, int stage, F &&launch) {
@@ -524,7 +623,7 @@ int32_t bmfr_stream_open(const bmfr_params *params, bmfr_stream *out) {
     if (e == hipSuccess) e = hipEventCreateWithFlags(&s->sync_ev, hipEventDisableTiming);
     if (e == hipSuccess) e = hipStreamSynchronize(s->stream);
     if (e != hipSuccess) {
-        bmfr_stream_close(s);
+        destroy_stream(s);
         return fail(BMFR_ERR_HIP, "stream set-up failed: %s", hipGetErrorString(e));
     }
     *out = s;
@@ -533,6 +632,17 @@ int32_t bmfr_stream_open(const bmfr_params *params, bmfr_stream *out) {
 
 int32_t bmfr_stream_close(bmfr_stream s) {
     if (!s) return BMFR_OK;
+    if (s->owner) return fail(BMFR_ERR_STATE, "the stream belongs to a server: close it with bmfr_server_close_stream");
+    if (s->batch_refs) return fail(BMFR_ERR_STATE, "the stream is a member of a batch: destroy the batch first");
+    destroy_stream(s);
+    return BMFR_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+void destroy_stream(bmfr_stream s) {
     (void)hipSetDevice(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
     for (int i = 0; i < kStages; ++i) {
@@ -545,8 +655,11 @@ int32_t bmfr_stream_close(bmfr_stream s) {
     if (s->stream) (void)hipStreamDestroy(s->stream);
     if (s->base) (void)hipFree(s->base);
     delete s;
-    return BMFR_OK;
 }
+
+}  // namespace
+
+extern "C" {
 
 int32_t bmfr_stream_sizes(bmfr_stream s, bmfr_sizes *out) {
     if (int32_t st = check_stream(s)) return st;
@@ -563,6 +676,8 @@ int32_t bmfr_stream_reset(bmfr_stream s) {
     s->swapped = false;
     s->last_result = -1;
     s->noise_frame = -1;
+    for (auto &p : s->bind_in) p = nullptr;
+    for (auto &p : s->prev_in) p = nullptr;
     return BMFR_OK;
 }
 
@@ -619,6 +734,37 @@ int32_t bmfr_copy_frame_inputs_device(bmfr_stream s, const void *albedo, const v
     HIP_TRY(hipSetDevice(s->device));
     hipError_t e = bmfr::launch_copy4(job, 4, s->stream);
     if (e != hipSuccess) return fail(BMFR_ERR_HIP, "copy launch failed: %s", hipGetErrorString(e));
+    return BMFR_OK;
+}
+
+int32_t bmfr_bind_device_inputs(bmfr_stream s, const void *albedo, const void *normals, const void *positions,
+                                const void *noisy) {
+    if (int32_t st = check_stream(s)) return st;
+    const void *src[4] = {albedo, normals, positions, noisy};
+    const int nulls = !albedo + !normals + !positions + !noisy;
+    if (nulls != 0 && nulls != 4) return fail(BMFR_ERR_INVALID_ARGUMENT, "bind all four inputs or none");
+    for (int i = 0; i < 4; ++i)
+        if (((uintptr_t)src[i] & 15) != 0)
+            return fail(BMFR_ERR_INVALID_ARGUMENT, "device input %d is not 16-byte aligned", i);
+    for (int i = 0; i < 4; ++i) s->bind_in[i] = src[i];
+    return BMFR_OK;
+}
+
+int32_t bmfr_bind_output_surface(bmfr_stream s, void *surface, int32_t format, int64_t pitch) {
+    if (int32_t st = check_stream(s)) return st;
+    if (surface) {
+        if (int32_t st = check_surface(s, surface, format, pitch)) return st;
+    }
+    s->surface = surface;
+    s->surface_format = surface ? format : 0;
+    s->surface_pitch = surface ? pitch : 0;
+    return BMFR_OK;
+}
+
+int32_t bmfr_stream_hip_stream(bmfr_stream s, void **hip_stream) {
+    if (int32_t st = check_stream(s)) return st;
+    if (!hip_stream) return fail(BMFR_ERR_INVALID_ARGUMENT, "hip_stream is null");
+    *hip_stream = (void *)s->stream;
     return BMFR_OK;
 }
 
@@ -710,7 +856,7 @@ int32_t bmfr_taa(bmfr_stream s, int32_t frame) {
 
 int32_t bmfr_end_frame(bmfr_stream s) {
     if (int32_t st = check_stream(s)) return st;
-    s->swapped = !s->swapped;  // opencl/bmfr.cpp:482-484
+    s->finish_frame();  // opencl/bmfr.cpp:482-484
     return BMFR_OK;
 }
 
@@ -992,10 +1138,15 @@ int32_t bmfr_batch_create(const bmfr_stream *streams, int32_t count, bmfr_batch 
             if (streams[j] == streams[i]) return fail(BMFR_ERR_INVALID_ARGUMENT, "stream %d listed twice", i);
         if (streams[i]->p.feature_count)
             return fail(BMFR_ERR_INVALID_ARGUMENT, "stream %d: batches run the reference's feature set", i);
+        if (streams[i]->batch_refs)
+            return fail(BMFR_ERR_STATE, "stream %d already belongs to a batch", i);
+        if (streams[i]->owner)
+            return fail(BMFR_ERR_STATE, "stream %d belongs to a server (server steps batch it)", i);
     }
     bmfr_batch b = new (std::nothrow) bmfr_batch_s;
     if (!b) return fail(BMFR_ERR_INVALID_ARGUMENT, "host allocation failed");
     b->streams.assign(streams, streams + count);
+    for (bmfr_stream s : b->streams) ++s->batch_refs;
     b->device = streams[0]->device;
     // pipeline lanes: largest streams first, each to the lane with fewer pixels
     std::vector<int> lane_of(count, 0), order(count);
@@ -1048,27 +1199,7 @@ int32_t bmfr_batch_create(const bmfr_stream *streams, int32_t count, bmfr_batch 
         for (int k = 0; k < 3; ++k) g.starts[k][0] = 0;
         for (int m = 0; m < n; ++m) {
             bmfr_stream s = b->streams[g.members[m]];
-            bmfr::BatchStream &d = hs[m];
-            d.args = s->args(0);
-            d.albedo = s->slot[S_ALBEDO];
-            d.noisy_in = s->slot[S_NOISY_IN];
-            d.prev_pixel = s->slot[S_PREV_PIXEL];
-            d.accept = s->slot[S_ACCEPT];
-            d.tmp_data = s->slot[S_TMP];
-            d.weights = s->slot[S_WEIGHTS];
-            d.mins_maxs = s->slot[S_MINS_MAXS];
-            d.filtered = s->slot[S_FILTERED];
-            d.tone_mapped = s->slot[S_TONE];
-            d.noise = s->slot[S_NOISE];
-            d.flags = s->slot[S_FLAGS];
-            for (int h = 0; h < 2; ++h) {
-                d.normals[h] = s->slot[S_NORMALS_A + h];
-                d.positions[h] = s->slot[S_POSITIONS_A + h];
-                d.noisy_acc[h] = s->slot[S_NOISY_ACC_A + h];
-                d.spp[h] = s->slot[S_SPP_A + h];
-                d.out[h] = s->slot[S_OUT_A + h];
-                d.result[h] = s->slot[S_RESULT_A + h];
-            }
+            s->batch_stream(hs[m]);
             const bmfr_sizes &z = s->sz;
             g.starts[bmfr::kStartsPadded][m + 1] =
                 g.starts[bmfr::kStartsPadded][m] + (z.margin_width / 32) * (z.margin_height / 8);
@@ -1096,6 +1227,7 @@ int32_t bmfr_batch_destroy(bmfr_batch b) {
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     if (b->stream2) (void)hipStreamSynchronize(b->stream2);
+    for (bmfr_stream s : b->streams) --s->batch_refs;
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
     for (auto &g : b->groups) {
         if (g.d_streams) (void)hipFree(g.d_streams);
@@ -1185,12 +1317,7 @@ int32_t bmfr_batch_run(bmfr_batch b, const int32_t *frames, const float *matrice
         bmfr::BatchStep *hs = b->h_steps + ((size_t)slot * b->groups.size() + gi) * bmfr::kMaxBatch;
         for (size_t m = 0; m < g.members.size(); ++m) {
             const int i = g.members[m];
-            bmfr_stream s = b->streams[i];
-            hs[m].frame = frames[i];
-            hs[m].cur = s->swapped ? 0 : 1;  // Double_buffer::current
-            std::memcpy(hs[m].matrix, matrices + 16 * i, sizeof(hs[m].matrix));
-            hs[m].pixel_offset_x = offsets[2 * i];
-            hs[m].pixel_offset_y = offsets[2 * i + 1];
+            b->streams[i]->batch_step(frames[i], matrices + 16 * i, offsets + 2 * i, nullptr, 0, 0, hs[m]);
         }
         HIP_TRY(hipMemcpyAsync(g.d_steps, hs, sizeof(bmfr::BatchStep) * g.members.size(),
                                hipMemcpyHostToDevice, b->stream));
@@ -1227,7 +1354,7 @@ int32_t bmfr_batch_run(bmfr_batch b, const int32_t *frames, const float *matrice
         HIP_TRY(hipStreamWaitEvent(s->stream, b->done, 0));
         s->last_result = s->cur(S_RESULT_A);
         s->noise_frame = frames[i];
-        s->swapped = !s->swapped;
+        s->finish_frame();
     }
     return BMFR_OK;
 }
@@ -1342,19 +1469,66 @@ int32_t bmfr_batch_marker_elapsed(bmfr_batch b, int32_t m0, int32_t m1, float *m
 }  // extern "C"
 
 
-// ---- server: admission by HBM budget and dynamic batched steps ------------
+// ---- server: admission by HBM budget, batched steps, the request loop ------
+namespace {
+
+double clock_seconds() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// A queued bmfr_server_request_stream.
+struct AdmitRequest {
+    bmfr_params params;
+    uint64_t tag;
+    double t_request;
+    int64_t bytes;
+};
+
+// One member of a step: the stream, its frame arguments, the frame's output
+// surface (null: the stream's bound one) and the producer event to wait for.
+struct StepMember {
+    bmfr_stream s;
+    int32_t frame;
+    const float *matrix, *offset;
+    void *surface;
+    int32_t surface_format;
+    int64_t surface_pitch;
+    hipEvent_t producer;
+};
+
+// A submitted frame inside a launched step.
+struct StepFrame {
+    bmfr_stream stream;
+    int32_t frame;
+    uint64_t tag;
+    double t_arrival;
+};
+
+// A step launched by the request loop, until its completion is seen.
+struct StepRecord {
+    int64_t id;
+    int slot;
+    double t_launch;
+    std::vector<StepFrame> frames;
+};
+
+constexpr int kMaxServerStreams = 4096;
+
+}  // namespace
+
 struct bmfr_server_s {
     int device = 0;
-    int64_t budget = 0, used = 0;
+    int64_t budget = 0, used = 0;   // used counts admissions in progress too
     std::vector<bmfr_stream> open;
     hipStream_t stream = nullptr;
     // device tables: one slot per launch group (storage F32/F16 x tmp
-    // F32/F16 x qr mode)
+    // F32/F16 x qr mode), rewritten by every step on `stream` (in order)
     static constexpr int kGroups = 12;
     bmfr::BatchStream *d_streams = nullptr;  // [kGroups][kMaxBatch]
     bmfr::BatchStep *d_steps = nullptr;      // [kGroups][kMaxBatch]
     int32_t *d_starts = nullptr;             // [kGroups][kStartKinds][kMaxBatch+1]
-    static constexpr int kRing = 4;
+    // per-step records: pinned staging and events, slot = step id % kRing
+    static constexpr int kRing = 8;
     struct Staging {
         bmfr::BatchStream streams[kGroups][bmfr::kMaxBatch];
         bmfr::BatchStep steps[kGroups][bmfr::kMaxBatch];
@@ -1362,165 +1536,63 @@ struct bmfr_server_s {
     };
     Staging *h_stage = nullptr;              // [kRing], pinned
     hipEvent_t ring_ev[kRing]{};
-    int ring = 0;
     hipEvent_t step_start[kRing]{}, step_end[kRing]{};
-    int64_t steps_launched = 0;
+    int64_t ring_id[kRing]{};                // step id each slot's record belongs to (0: none)
+    int64_t next_step = 1;
     bmfr::FitterConfig fitter{16, 4};
     int grid_cap = 512;   // work-groups of 4 waves: 2 waves per SIMD on 256 CUs
     int fuse = BMFR_FUSE_345;
+    // The request loop.  `mu` guards the fields above that steps change and
+    // everything below, and the queues of the server's streams.
+    std::mutex mu;
+    std::condition_variable cv_work;     // the dispatcher: frames queued, or stop
+    std::condition_variable cv_events;   // pollers: events to read
+    std::condition_variable cv_drained;  // closers: a stream's frames have run
+    std::thread dispatcher;
+    bool stop = false;
+    int max_in_flight = 2;
+    int max_step_streams = bmfr::kMaxBatch;
+    int queued_frames = 0;
+    std::deque<StepRecord> in_flight;
+    std::deque<bmfr_server_event> events;
+    std::deque<AdmitRequest> admit_fifo;
+    std::vector<hipEvent_t> event_pool;     // producer events not in use
 };
 
-extern "C" {
+namespace {
 
-int32_t bmfr_server_create(int64_t hbm_budget_bytes, bmfr_server *out) {
-    if (!out) return fail(BMFR_ERR_INVALID_ARGUMENT, "out is null");
-    if (hbm_budget_bytes <= 0) return fail(BMFR_ERR_INVALID_ARGUMENT, "budget must be positive");
-    *out = nullptr;
-    int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(BMFR_ERR_NO_DEVICE, "no HIP device visible");
-    bmfr_server sv = new (std::nothrow) bmfr_server_s;
-    if (!sv) return fail(BMFR_ERR_INVALID_ARGUMENT, "host allocation failed");
-    sv->budget = hbm_budget_bytes;
-    hipError_t e = hipGetDevice(&sv->device);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking);
-    const size_t G = bmfr_server_s::kGroups, M = bmfr::kMaxBatch;
-    if (e == hipSuccess) e = hipMalloc((void **)&sv->d_streams, sizeof(bmfr::BatchStream) * G * M);
-    if (e == hipSuccess) e = hipMalloc((void **)&sv->d_steps, sizeof(bmfr::BatchStep) * G * M);
-    if (e == hipSuccess) e = hipMalloc((void **)&sv->d_starts, sizeof(int32_t) * G * bmfr::kStartKinds * (M + 1));
-    if (e == hipSuccess)
-        e = hipHostMalloc((void **)&sv->h_stage, sizeof(bmfr_server_s::Staging) * bmfr_server_s::kRing);
-    for (int i = 0; i < bmfr_server_s::kRing && e == hipSuccess; ++i) {
-        e = hipEventCreateWithFlags(&sv->ring_ev[i], hipEventDisableTiming);
-        if (e == hipSuccess) e = hipEventCreate(&sv->step_start[i]);
-        if (e == hipSuccess) e = hipEventCreate(&sv->step_end[i]);
-    }
-    if (e != hipSuccess) {
-        bmfr_server_destroy(sv);
-        return fail(BMFR_ERR_HIP, "server set-up failed: %s", hipGetErrorString(e));
-    }
-    *out = sv;
-    return BMFR_OK;
+int server_group(bmfr_stream s) {
+    return ((s->p.storage == BMFR_STORAGE_F16 ? 2 : 0) + (s->p.tmp_storage == BMFR_STORAGE_F16 ? 1 : 0)) * 3 +
+           s->p.qr_mode;
 }
 
-int32_t bmfr_server_destroy(bmfr_server sv) {
-    if (!sv) return BMFR_OK;
-    (void)hipSetDevice(sv->device);
-    if (sv->stream) (void)hipStreamSynchronize(sv->stream);
-    for (bmfr_stream s : sv->open) bmfr_stream_close(s);
-    sv->open.clear();
-    for (int i = 0; i < bmfr_server_s::kRing; ++i) {
-        if (sv->ring_ev[i]) (void)hipEventDestroy(sv->ring_ev[i]);
-        if (sv->step_start[i]) (void)hipEventDestroy(sv->step_start[i]);
-        if (sv->step_end[i]) (void)hipEventDestroy(sv->step_end[i]);
-    }
-    if (sv->h_stage) (void)hipHostFree(sv->h_stage);
-    if (sv->d_streams) (void)hipFree(sv->d_streams);
-    if (sv->d_steps) (void)hipFree(sv->d_steps);
-    if (sv->d_starts) (void)hipFree(sv->d_starts);
-    if (sv->stream) (void)hipStreamDestroy(sv->stream);
-    delete sv;
-    return BMFR_OK;
-}
-
-int32_t bmfr_server_usage(bmfr_server sv, int64_t *used, int64_t *budget, int32_t *streams) {
-    if (!sv) return fail(BMFR_ERR_INVALID_ARGUMENT, "server is null");
-    if (used) *used = sv->used;
-    if (budget) *budget = sv->budget;
-    if (streams) *streams = (int32_t)sv->open.size();
-    return BMFR_OK;
-}
-
-int32_t bmfr_server_open_stream(bmfr_server sv, const bmfr_params *params, bmfr_stream *out) {
-    if (!sv || !out) return fail(BMFR_ERR_INVALID_ARGUMENT, "null argument");
-    *out = nullptr;
-    bmfr_sizes z;
-    if (int32_t st = bmfr_compute_sizes(params, &z)) return st;
-    if (params->feature_count)
-        return fail(BMFR_ERR_INVALID_ARGUMENT, "servers run the reference's feature set");
-    if (sv->used + z.device_bytes > sv->budget)
-        return fail(BMFR_ERR_OUT_OF_BUDGET, "stream needs %lld bytes, %lld of %lld in use",
-                    (long long)z.device_bytes, (long long)sv->used, (long long)sv->budget);
-    if ((int)sv->open.size() >= 4 * bmfr::kMaxBatch)
-        return fail(BMFR_ERR_OUT_OF_BUDGET, "too many open streams");
+// Launches one step over `m` (distinct open streams of the server) on the
+// server's HIP stream; sv->mu is held.  Each member's frame is finished
+// (double buffers swapped) once its launches are enqueued.
+int32_t launch_step_locked(bmfr_server sv, const std::vector<StepMember> &m, int64_t *step_id) {
     HIP_TRY(hipSetDevice(sv->device));
-    bmfr_stream s;
-    if (int32_t st = bmfr_stream_open(params, &s)) return st;
-    sv->open.push_back(s);
-    sv->used += z.device_bytes;
-    *out = s;
-    return BMFR_OK;
-}
-
-int32_t bmfr_server_close_stream(bmfr_server sv, bmfr_stream s) {
-    if (!sv || !s) return fail(BMFR_ERR_INVALID_ARGUMENT, "null argument");
-    auto it = std::find(sv->open.begin(), sv->open.end(), s);
-    if (it == sv->open.end()) return fail(BMFR_ERR_INVALID_ARGUMENT, "stream does not belong to this server");
-    sv->open.erase(it);
-    sv->used -= s->sz.device_bytes;
-    // bmfr_stream_close synchronises the stream's own HIP stream, which waits
-    // for every server step the stream took part in.
-    return bmfr_stream_close(s);
-}
-
-int32_t bmfr_server_step(bmfr_server sv, int32_t count, const bmfr_stream *streams, const int32_t *frames,
-                         const float *matrices, const float *offsets, int32_t *step_id) {
-    if (!sv || !streams || !frames || !matrices || !offsets)
-        return fail(BMFR_ERR_INVALID_ARGUMENT, "null argument");
-    if (count < 1 || count > bmfr::kMaxBatch)
-        return fail(BMFR_ERR_INVALID_ARGUMENT, "step of %d streams outside 1..%d", count, bmfr::kMaxBatch);
-    for (int i = 0; i < count; ++i) {
-        if (!streams[i] || frames[i] < 0) return fail(BMFR_ERR_INVALID_ARGUMENT, "bad stream or frame %d", i);
-        if (std::find(sv->open.begin(), sv->open.end(), streams[i]) == sv->open.end())
-            return fail(BMFR_ERR_INVALID_ARGUMENT, "stream %d was not opened by this server", i);
-        for (int j = 0; j < i; ++j)
-            if (streams[j] == streams[i]) return fail(BMFR_ERR_INVALID_ARGUMENT, "stream %d listed twice", i);
-    }
-    HIP_TRY(hipSetDevice(sv->device));
-    const int slot = sv->ring;
-    sv->ring = (sv->ring + 1) % bmfr_server_s::kRing;
+    const int64_t id = sv->next_step++;
+    const int slot = (int)(id % bmfr_server_s::kRing);
     HIP_TRY(hipEventSynchronize(sv->ring_ev[slot]));  // staging slot free again
+    sv->ring_id[slot] = id;
     bmfr_server_s::Staging &hs = sv->h_stage[slot];
     int gcount[bmfr_server_s::kGroups] = {};
-    for (int i = 0; i < count; ++i) {
-        bmfr_stream s = streams[i];
-        const int g = ((s->p.storage == BMFR_STORAGE_F16 ? 2 : 0) + (s->p.tmp_storage == BMFR_STORAGE_F16 ? 1 : 0)) * 3 +
-                      s->p.qr_mode;
-        const int m = gcount[g]++;
-        bmfr::BatchStream &d = hs.streams[g][m];
-        d.args = s->args(0);
-        d.albedo = s->slot[S_ALBEDO];
-        d.noisy_in = s->slot[S_NOISY_IN];
-        d.prev_pixel = s->slot[S_PREV_PIXEL];
-        d.accept = s->slot[S_ACCEPT];
-        d.tmp_data = s->slot[S_TMP];
-        d.weights = s->slot[S_WEIGHTS];
-        d.mins_maxs = s->slot[S_MINS_MAXS];
-        d.filtered = s->slot[S_FILTERED];
-        d.tone_mapped = s->slot[S_TONE];
-        d.noise = s->slot[S_NOISE];
-        d.flags = s->slot[S_FLAGS];
-        for (int h = 0; h < 2; ++h) {
-            d.normals[h] = s->slot[S_NORMALS_A + h];
-            d.positions[h] = s->slot[S_POSITIONS_A + h];
-            d.noisy_acc[h] = s->slot[S_NOISY_ACC_A + h];
-            d.spp[h] = s->slot[S_SPP_A + h];
-            d.out[h] = s->slot[S_OUT_A + h];
-            d.result[h] = s->slot[S_RESULT_A + h];
-        }
-        bmfr::BatchStep &st = hs.steps[g][m];
-        st.frame = frames[i];
-        st.cur = s->swapped ? 0 : 1;
-        std::memcpy(st.matrix, matrices + 16 * i, sizeof(st.matrix));
-        st.pixel_offset_x = offsets[2 * i];
-        st.pixel_offset_y = offsets[2 * i + 1];
+    for (const StepMember &mb : m) {
+        bmfr_stream s = mb.s;
+        const int g = server_group(s);
+        const int k = gcount[g]++;
+        s->batch_stream(hs.streams[g][k]);
+        s->batch_step(mb.frame, mb.matrix, mb.offset, mb.surface, mb.surface_format, mb.surface_pitch,
+                      hs.steps[g][k]);
         int32_t(&sts)[bmfr::kStartKinds][bmfr::kMaxBatch + 1] = hs.starts[g];
-        if (m == 0) sts[0][0] = sts[1][0] = sts[2][0] = 0;
-        sts[bmfr::kStartsPadded][m + 1] = sts[bmfr::kStartsPadded][m] + (s->sz.margin_width / 32) * (s->sz.margin_height / 8);
-        sts[bmfr::kStartsImage][m + 1] = sts[bmfr::kStartsImage][m] + ((s->p.width + 63) / 64) * ((s->p.height + 3) / 4);
-        sts[bmfr::kStartsBlocks][m + 1] = sts[bmfr::kStartsBlocks][m] + s->sz.blocks;
-        // order after this stream's pending work (its inputs)
+        if (k == 0) sts[0][0] = sts[1][0] = sts[2][0] = 0;
+        sts[bmfr::kStartsPadded][k + 1] = sts[bmfr::kStartsPadded][k] + (s->sz.margin_width / 32) * (s->sz.margin_height / 8);
+        sts[bmfr::kStartsImage][k + 1] = sts[bmfr::kStartsImage][k] + ((s->p.width + 63) / 64) * ((s->p.height + 3) / 4);
+        sts[bmfr::kStartsBlocks][k + 1] = sts[bmfr::kStartsBlocks][k] + s->sz.blocks;
+        // order after this stream's pending work and its producer's
         HIP_TRY(hipEventRecord(s->sync_ev, s->stream));
         HIP_TRY(hipStreamWaitEvent(sv->stream, s->sync_ev, 0));
+        if (mb.producer) HIP_TRY(hipStreamWaitEvent(sv->stream, mb.producer, 0));
     }
     const size_t M = bmfr::kMaxBatch;
     for (int g = 0; g < bmfr_server_s::kGroups; ++g) {
@@ -1567,33 +1639,387 @@ int32_t bmfr_server_step(bmfr_server sv, int32_t count, const bmfr_stream *strea
         }
     }
     HIP_TRY(hipEventRecord(sv->step_end[slot], sv->stream));
-    for (int i = 0; i < count; ++i) {
-        bmfr_stream s = streams[i];
+    for (const StepMember &mb : m) {
+        bmfr_stream s = mb.s;
         HIP_TRY(hipStreamWaitEvent(s->stream, sv->step_end[slot], 0));
         s->last_result = s->cur(S_RESULT_A);
-        s->noise_frame = frames[i];
-        s->swapped = !s->swapped;
+        s->noise_frame = mb.frame;
+        s->finish_frame();
     }
-    if (step_id) *step_id = slot;
-    ++sv->steps_launched;
+    if (step_id) *step_id = id;
     return BMFR_OK;
 }
 
-// Completion of a step launched by bmfr_server_step: *done = 1 and its GPU
-// time in *ms once it has finished, *done = 0 while it runs.  Step ids are
-// reused after kRing (4) further steps.
-int32_t bmfr_server_step_query(bmfr_server sv, int32_t step_id, int32_t *done, float *ms) {
-    if (!sv || !done || step_id < 0 || step_id >= bmfr_server_s::kRing)
-        return fail(BMFR_ERR_INVALID_ARGUMENT, "bad arguments");
+hipEvent_t pool_event_locked(bmfr_server sv) {
+    if (!sv->event_pool.empty()) {
+        hipEvent_t e = sv->event_pool.back();
+        sv->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+    return e;
+}
+
+// Completions of the request loop's steps, oldest first; sv->mu is held.
+void reap_locked(bmfr_server sv) {
+    bool any = false;
+    while (!sv->in_flight.empty()) {
+        StepRecord &r = sv->in_flight.front();
+        const hipError_t q = hipEventQuery(sv->step_end[r.slot]);
+        if (q == hipErrorNotReady) break;
+        const double t = clock_seconds();
+        float ms = -1.f;
+        if (q == hipSuccess) (void)hipEventElapsedTime(&ms, sv->step_start[r.slot], sv->step_end[r.slot]);
+        for (const StepFrame &f : r.frames) {
+            bmfr_server_event ev{};
+            ev.kind = q == hipSuccess ? BMFR_EVENT_FRAME_DONE : BMFR_EVENT_FRAME_FAILED;
+            ev.status = q == hipSuccess ? BMFR_OK : BMFR_ERR_HIP;
+            ev.stream = f.stream;
+            ev.tag = f.tag;
+            ev.frame_number = f.frame;
+            ev.step_streams = (int32_t)r.frames.size();
+            ev.step_id = r.id;
+            ev.t_arrival = f.t_arrival;
+            ev.t_launch = r.t_launch;
+            ev.t_done = t;
+            ev.step_gpu_ms = ms;
+            sv->events.push_back(ev);
+            --f.stream->frames_in_flight;
+        }
+        sv->in_flight.pop_front();
+        any = true;
+    }
+    if (any) {
+        sv->cv_events.notify_all();
+        sv->cv_drained.notify_all();
+    }
+}
+
+// One step of the oldest queued frame of every stream that has one (oldest
+// arrivals first, at most max_step_streams); sv->mu is held.
+void launch_queued_locked(bmfr_server sv) {
+    std::vector<bmfr_stream> cand;
+    for (bmfr_stream s : sv->open)
+        if (!s->queue.empty()) cand.push_back(s);
+    if (cand.empty()) return;
+    std::stable_sort(cand.begin(), cand.end(), [](bmfr_stream a, bmfr_stream b) {
+        return a->queue.front().req.t_arrival < b->queue.front().req.t_arrival;
+    });
+    if ((int)cand.size() > sv->max_step_streams) cand.resize(sv->max_step_streams);
+    std::vector<StepMember> m;
+    StepRecord rec;
+    rec.t_launch = clock_seconds();
+    for (bmfr_stream s : cand) {
+        const ServerFrame &f = s->queue.front();
+        const bmfr_frame_request &r = f.req;
+        if (r.albedo) {
+            s->bind_in[0] = r.albedo;
+            s->bind_in[1] = r.normals;
+            s->bind_in[2] = r.positions;
+            s->bind_in[3] = r.noisy;
+        }
+        m.push_back({s, r.frame_number, r.prev_frame_camera_matrix, r.pixel_offset, r.surface, r.surface_format,
+                     r.surface_row_pitch, f.producer});
+        rec.frames.push_back({s, r.frame_number, r.tag, r.t_arrival});
+    }
+    const int32_t st = launch_step_locked(sv, m, &rec.id);
+    for (bmfr_stream s : cand) {
+        if (s->queue.front().producer) sv->event_pool.push_back(s->queue.front().producer);
+        s->queue.pop_front();
+        --sv->queued_frames;
+        if (st == BMFR_OK) ++s->frames_in_flight;
+        else for (auto &p : s->bind_in) p = nullptr;
+    }
+    if (st == BMFR_OK) {
+        rec.slot = (int)(rec.id % bmfr_server_s::kRing);
+        sv->in_flight.push_back(std::move(rec));
+    } else {
+        for (const StepFrame &f : rec.frames) {
+            bmfr_server_event ev{};
+            ev.kind = BMFR_EVENT_FRAME_FAILED;
+            ev.status = st;
+            ev.stream = f.stream;
+            ev.tag = f.tag;
+            ev.frame_number = f.frame;
+            ev.t_arrival = f.t_arrival;
+            ev.t_launch = rec.t_launch;
+            ev.t_done = clock_seconds();
+            sv->events.push_back(ev);
+        }
+        sv->cv_events.notify_all();
+    }
+    sv->cv_drained.notify_all();
+}
+
+// The dispatcher thread: launch whenever fewer than max_in_flight steps are
+// on the GPU and frames are queued; otherwise wait for a submit (nothing in
+// flight) or look for completions again shortly (steps in flight).
+void dispatcher_main(bmfr_server sv) {
+    (void)hipSetDevice(sv->device);
+    std::unique_lock<std::mutex> lk(sv->mu);
+    while (!sv->stop) {
+        reap_locked(sv);
+        if ((int)sv->in_flight.size() < sv->max_in_flight && sv->queued_frames > 0) {
+            launch_queued_locked(sv);
+            continue;
+        }
+        if (sv->in_flight.empty())
+            sv->cv_work.wait(lk, [&] { return sv->stop || (sv->queued_frames > 0 && sv->max_in_flight > 0); });
+        else
+            sv->cv_work.wait_for(lk, std::chrono::microseconds(20));
+    }
+}
+
+// Opens a stream owned by the server (its budget already reserved).
+int32_t open_owned_locked_out(bmfr_server sv, const bmfr_params *params, bmfr_stream *out) {
     HIP_TRY(hipSetDevice(sv->device));
-    const hipError_t q = hipEventQuery(sv->step_end[step_id]);
+    bmfr_stream s = nullptr;
+    if (int32_t st = bmfr_stream_open(params, &s)) return st;
+    s->owner = sv;
+    *out = s;
+    return BMFR_OK;
+}
+
+// Admits queued requests, in order, while the budget allows (no lock held).
+void admit_waiting(bmfr_server sv) {
+    for (;;) {
+        AdmitRequest r;
+        {
+            std::lock_guard<std::mutex> lk(sv->mu);
+            if (sv->admit_fifo.empty()) return;
+            r = sv->admit_fifo.front();
+            if (sv->used + r.bytes > sv->budget || (int)sv->open.size() >= kMaxServerStreams) return;
+            sv->admit_fifo.pop_front();
+            sv->used += r.bytes;
+        }
+        bmfr_stream s = nullptr;
+        const int32_t st = open_owned_locked_out(sv, &r.params, &s);
+        std::lock_guard<std::mutex> lk(sv->mu);
+        bmfr_server_event ev{};
+        ev.kind = BMFR_EVENT_STREAM_ADMITTED;
+        ev.status = st;
+        ev.tag = r.tag;
+        ev.t_arrival = r.t_request;
+        ev.t_done = clock_seconds();
+        if (st == BMFR_OK) {
+            sv->open.push_back(s);
+            ev.stream = s;
+        } else {
+            sv->used -= r.bytes;
+        }
+        sv->events.push_back(ev);
+        sv->cv_events.notify_all();
+    }
+}
+
+bool owns_locked(bmfr_server sv, bmfr_stream s) {
+    return std::find(sv->open.begin(), sv->open.end(), s) != sv->open.end();
+}
+
+}  // namespace
+
+extern "C" {
+
+double bmfr_clock_seconds(void) { return clock_seconds(); }
+
+int32_t bmfr_server_create(int64_t hbm_budget_bytes, bmfr_server *out) {
+    if (!out) return fail(BMFR_ERR_INVALID_ARGUMENT, "out is null");
+    if (hbm_budget_bytes <= 0) return fail(BMFR_ERR_INVALID_ARGUMENT, "budget must be positive");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(BMFR_ERR_NO_DEVICE, "no HIP device visible");
+    bmfr_server sv = new (std::nothrow) bmfr_server_s;
+    if (!sv) return fail(BMFR_ERR_INVALID_ARGUMENT, "host allocation failed");
+    sv->budget = hbm_budget_bytes;
+    hipError_t e = hipGetDevice(&sv->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking);
+    const size_t G = bmfr_server_s::kGroups, M = bmfr::kMaxBatch;
+    if (e == hipSuccess) e = hipMalloc((void **)&sv->d_streams, sizeof(bmfr::BatchStream) * G * M);
+    if (e == hipSuccess) e = hipMalloc((void **)&sv->d_steps, sizeof(bmfr::BatchStep) * G * M);
+    if (e == hipSuccess) e = hipMalloc((void **)&sv->d_starts, sizeof(int32_t) * G * bmfr::kStartKinds * (M + 1));
+    if (e == hipSuccess)
+        e = hipHostMalloc((void **)&sv->h_stage, sizeof(bmfr_server_s::Staging) * bmfr_server_s::kRing);
+    for (int i = 0; i < bmfr_server_s::kRing && e == hipSuccess; ++i) {
+        e = hipEventCreateWithFlags(&sv->ring_ev[i], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreate(&sv->step_start[i]);
+        if (e == hipSuccess) e = hipEventCreate(&sv->step_end[i]);
+    }
+    if (e != hipSuccess) {
+        bmfr_server_destroy(sv);
+        return fail(BMFR_ERR_HIP, "server set-up failed: %s", hipGetErrorString(e));
+    }
+    *out = sv;
+    return BMFR_OK;
+}
+
+int32_t bmfr_server_destroy(bmfr_server sv) {
+    if (!sv) return BMFR_OK;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->stop = true;
+    }
+    sv->cv_work.notify_all();
+    if (sv->dispatcher.joinable()) sv->dispatcher.join();
+    (void)hipSetDevice(sv->device);
+    if (sv->stream) (void)hipStreamSynchronize(sv->stream);
+    for (bmfr_stream s : sv->open) {
+        for (ServerFrame &f : s->queue)
+            if (f.producer) sv->event_pool.push_back(f.producer);
+        s->queue.clear();
+        destroy_stream(s);
+    }
+    sv->open.clear();
+    for (hipEvent_t e : sv->event_pool) (void)hipEventDestroy(e);
+    for (int i = 0; i < bmfr_server_s::kRing; ++i) {
+        if (sv->ring_ev[i]) (void)hipEventDestroy(sv->ring_ev[i]);
+        if (sv->step_start[i]) (void)hipEventDestroy(sv->step_start[i]);
+        if (sv->step_end[i]) (void)hipEventDestroy(sv->step_end[i]);
+    }
+    if (sv->h_stage) (void)hipHostFree(sv->h_stage);
+    if (sv->d_streams) (void)hipFree(sv->d_streams);
+    if (sv->d_steps) (void)hipFree(sv->d_steps);
+    if (sv->d_starts) (void)hipFree(sv->d_starts);
+    if (sv->stream) (void)hipStreamDestroy(sv->stream);
+    delete sv;
+    return BMFR_OK;
+}
+
+int32_t bmfr_server_usage(bmfr_server sv, int64_t *used, int64_t *budget, int32_t *streams) {
+    if (!sv) return fail(BMFR_ERR_INVALID_ARGUMENT, "server is null");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (used) *used = sv->used;
+    if (budget) *budget = sv->budget;
+    if (streams) *streams = (int32_t)sv->open.size();
+    return BMFR_OK;
+}
+
+int32_t bmfr_server_open_stream(bmfr_server sv, const bmfr_params *params, bmfr_stream *out) {
+    if (!sv || !out) return fail(BMFR_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    bmfr_sizes z;
+    if (int32_t st = bmfr_compute_sizes(params, &z)) return st;
+    if (params->feature_count)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "servers run the reference's feature set");
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        if (sv->used + z.device_bytes > sv->budget)
+            return fail(BMFR_ERR_OUT_OF_BUDGET, "stream needs %lld bytes, %lld of %lld in use",
+                        (long long)z.device_bytes, (long long)sv->used, (long long)sv->budget);
+        if ((int)sv->open.size() >= kMaxServerStreams) return fail(BMFR_ERR_OUT_OF_BUDGET, "too many open streams");
+        sv->used += z.device_bytes;
+    }
+    bmfr_stream s = nullptr;
+    const int32_t st = open_owned_locked_out(sv, params, &s);
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (st) {
+        sv->used -= z.device_bytes;
+        return st;
+    }
+    sv->open.push_back(s);
+    *out = s;
+    return BMFR_OK;
+}
+
+int32_t bmfr_server_request_stream(bmfr_server sv, const bmfr_params *params, uint64_t tag, bmfr_stream *out) {
+    if (!sv || !out) return fail(BMFR_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    bmfr_sizes z;
+    if (int32_t st = bmfr_compute_sizes(params, &z)) return st;
+    if (params->feature_count)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "servers run the reference's feature set");
+    if (z.device_bytes > sv->budget)
+        return fail(BMFR_ERR_OUT_OF_BUDGET, "stream needs %lld bytes, more than the whole budget of %lld",
+                    (long long)z.device_bytes, (long long)sv->budget);
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        if (!sv->admit_fifo.empty() || sv->used + z.device_bytes > sv->budget ||
+            (int)sv->open.size() >= kMaxServerStreams) {
+            sv->admit_fifo.push_back({*params, tag, clock_seconds(), z.device_bytes});
+            return BMFR_OK;   // queued: admitted later by a close
+        }
+        sv->used += z.device_bytes;
+    }
+    bmfr_stream s = nullptr;
+    const int32_t st = open_owned_locked_out(sv, params, &s);
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (st) {
+        sv->used -= z.device_bytes;
+        return st;
+    }
+    sv->open.push_back(s);
+    *out = s;
+    return BMFR_OK;
+}
+
+int32_t bmfr_server_cancel_request(bmfr_server sv, uint64_t tag, int32_t *found) {
+    if (!sv) return fail(BMFR_ERR_INVALID_ARGUMENT, "server is null");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    auto it = std::find_if(sv->admit_fifo.begin(), sv->admit_fifo.end(),
+                           [&](const AdmitRequest &r) { return r.tag == tag; });
+    const bool f = it != sv->admit_fifo.end();
+    if (f) sv->admit_fifo.erase(it);
+    if (found) *found = f ? 1 : 0;
+    return BMFR_OK;
+}
+
+int32_t bmfr_server_close_stream(bmfr_server sv, bmfr_stream s) {
+    if (!sv || !s) return fail(BMFR_ERR_INVALID_ARGUMENT, "null argument");
+    {
+        std::unique_lock<std::mutex> lk(sv->mu);
+        if (!owns_locked(sv, s)) return fail(BMFR_ERR_INVALID_ARGUMENT, "stream does not belong to this server");
+        if (s->closing) return fail(BMFR_ERR_STATE, "the stream is already being closed");
+        s->closing = true;
+        sv->cv_drained.wait(lk, [&] { return s->queue.empty() && s->frames_in_flight == 0; });
+        sv->open.erase(std::find(sv->open.begin(), sv->open.end(), s));
+        sv->used -= s->sz.device_bytes;
+    }
+    // destroy_stream synchronises the stream's own HIP stream, which waits for
+    // every server step the stream took part in.
+    destroy_stream(s);
+    admit_waiting(sv);
+    return BMFR_OK;
+}
+
+int32_t bmfr_server_step(bmfr_server sv, int32_t count, const bmfr_stream *streams, const int32_t *frames,
+                         const float *matrices, const float *offsets, int64_t *step_id) {
+    if (!sv || !streams || !frames || !matrices || !offsets)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "null argument");
+    if (count < 1 || count > bmfr::kMaxBatch)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "step of %d streams outside 1..%d", count, bmfr::kMaxBatch);
+    std::lock_guard<std::mutex> lk(sv->mu);
+    std::vector<StepMember> m;
+    for (int i = 0; i < count; ++i) {
+        if (!streams[i] || frames[i] < 0) return fail(BMFR_ERR_INVALID_ARGUMENT, "bad stream or frame %d", i);
+        if (!owns_locked(sv, streams[i]))
+            return fail(BMFR_ERR_INVALID_ARGUMENT, "stream %d was not opened by this server", i);
+        for (int j = 0; j < i; ++j)
+            if (streams[j] == streams[i]) return fail(BMFR_ERR_INVALID_ARGUMENT, "stream %d listed twice", i);
+        if (!streams[i]->queue.empty() || streams[i]->closing)
+            return fail(BMFR_ERR_STATE, "stream %d has frames queued in the request loop or is being closed", i);
+        m.push_back({streams[i], frames[i], matrices + 16 * i, offsets + 2 * i, nullptr, 0, 0, nullptr});
+    }
+    return launch_step_locked(sv, m, step_id);
+}
+
+int32_t bmfr_server_step_query(bmfr_server sv, int64_t step_id, int32_t *done, float *ms) {
+    if (!sv || !done) return fail(BMFR_ERR_INVALID_ARGUMENT, "null argument");
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (step_id < 1 || step_id >= sv->next_step)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "step %lld was never launched", (long long)step_id);
+    const int slot = (int)(step_id % bmfr_server_s::kRing);
+    if (sv->ring_id[slot] != step_id)
+        return fail(BMFR_ERR_STATE, "step %lld has expired: its record holds step %lld now", (long long)step_id,
+                    (long long)sv->ring_id[slot]);
+    HIP_TRY(hipSetDevice(sv->device));
+    const hipError_t q = hipEventQuery(sv->step_end[slot]);
     if (q == hipErrorNotReady) {
         *done = 0;
         return BMFR_OK;
     }
     if (q != hipSuccess) return fail(BMFR_ERR_HIP, "step query failed: %s", hipGetErrorString(q));
     *done = 1;
-    if (ms) HIP_TRY(hipEventElapsedTime(ms, sv->step_start[step_id], sv->step_end[step_id]));
+    if (ms) HIP_TRY(hipEventElapsedTime(ms, sv->step_start[slot], sv->step_end[slot]));
     return BMFR_OK;
 }
 
@@ -1601,12 +2027,91 @@ int32_t bmfr_server_set_fused(bmfr_server sv, int32_t level) {
     if (!sv) return fail(BMFR_ERR_INVALID_ARGUMENT, "server is null");
     if (level != BMFR_FUSE_34 && level != BMFR_FUSE_345)
         return fail(BMFR_ERR_INVALID_ARGUMENT, "server steps run fuse level 1 or 2, not %d", level);
+    std::lock_guard<std::mutex> lk(sv->mu);
     sv->fuse = level;
+    return BMFR_OK;
+}
+
+int32_t bmfr_server_set_policy(bmfr_server sv, int32_t max_in_flight, int32_t max_streams_per_step) {
+    if (!sv) return fail(BMFR_ERR_INVALID_ARGUMENT, "server is null");
+    if (max_in_flight < 0 || max_in_flight > 4 || max_streams_per_step < 1 || max_streams_per_step > bmfr::kMaxBatch)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "policy %d steps in flight, %d streams per step", max_in_flight,
+                    max_streams_per_step);
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sv->max_in_flight = max_in_flight;
+    sv->max_step_streams = max_streams_per_step;
+    sv->cv_work.notify_all();
+    return BMFR_OK;
+}
+
+int32_t bmfr_server_submit(bmfr_server sv, const bmfr_frame_request *r) {
+    if (!sv || !r || !r->stream) return fail(BMFR_ERR_INVALID_ARGUMENT, "null argument");
+    if (int32_t st = check_frame(r->frame_number)) return st;
+    bmfr_stream s = r->stream;
+    const void *in[4] = {r->albedo, r->normals, r->positions, r->noisy};
+    const int nulls = !in[0] + !in[1] + !in[2] + !in[3];
+    if (nulls != 0 && nulls != 4) return fail(BMFR_ERR_INVALID_ARGUMENT, "give all four inputs or none");
+    for (int i = 0; i < 4; ++i)
+        if (((uintptr_t)in[i] & 15) != 0)
+            return fail(BMFR_ERR_INVALID_ARGUMENT, "device input %d is not 16-byte aligned", i);
+    std::unique_lock<std::mutex> lk(sv->mu);
+    if (!owns_locked(sv, s)) return fail(BMFR_ERR_INVALID_ARGUMENT, "stream does not belong to this server");
+    if (s->closing) return fail(BMFR_ERR_STATE, "the stream is being closed");
+    if (r->surface)
+        if (int32_t st = check_surface(s, r->surface, r->surface_format, r->surface_row_pitch)) return st;
+    if (nulls == 4 && !s->queue.empty())
+        return fail(BMFR_ERR_STATE, "the stream's own input buffers hold one frame and one is queued: "
+                                    "give device inputs to queue more");
+    HIP_TRY(hipSetDevice(sv->device));
+    ServerFrame f;
+    f.req = *r;
+    if (r->producer_stream) {
+        f.producer = pool_event_locked(sv);
+        if (!f.producer) return fail(BMFR_ERR_HIP, "event creation failed");
+        const hipError_t e = hipEventRecord(f.producer, (hipStream_t)r->producer_stream);
+        if (e != hipSuccess) {
+            sv->event_pool.push_back(f.producer);
+            return fail(BMFR_ERR_HIP, "recording the producer event failed: %s", hipGetErrorString(e));
+        }
+    }
+    if (!sv->dispatcher.joinable()) {
+        try {
+            sv->dispatcher = std::thread(dispatcher_main, sv);
+        } catch (...) {
+            if (f.producer) sv->event_pool.push_back(f.producer);
+            return fail(BMFR_ERR_STATE, "cannot start the dispatcher thread");
+        }
+    }
+    s->queue.push_back(f);
+    ++sv->queued_frames;
+    lk.unlock();
+    sv->cv_work.notify_one();
+    return BMFR_OK;
+}
+
+int32_t bmfr_server_poll(bmfr_server sv, bmfr_server_event *out, int32_t max_events, int32_t timeout_us,
+                         int32_t *count) {
+    if (!sv || !count || max_events < 0 || (max_events > 0 && !out) || timeout_us < 0)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "bad poll arguments");
+    std::unique_lock<std::mutex> lk(sv->mu);
+    if (sv->events.empty() && timeout_us > 0)
+        sv->cv_events.wait_for(lk, std::chrono::microseconds(timeout_us), [&] { return !sv->events.empty(); });
+    int n = 0;
+    while (n < max_events && !sv->events.empty()) {
+        out[n++] = sv->events.front();
+        sv->events.pop_front();
+    }
+    *count = n;
     return BMFR_OK;
 }
 
 int32_t bmfr_server_synchronize(bmfr_server sv) {
     if (!sv) return fail(BMFR_ERR_INVALID_ARGUMENT, "server is null");
+    {
+        // the request loop's queued frames are launched first
+        std::unique_lock<std::mutex> lk(sv->mu);
+        sv->cv_drained.wait(lk, [&] { return sv->queued_frames == 0; });
+    }
     HIP_TRY(hipSetDevice(sv->device));
     HIP_TRY(hipStreamSynchronize(sv->stream));
     return BMFR_OK;
@@ -1692,6 +2197,27 @@ int32_t bmfr_scene_render_to_stream(bmfr_scene sc, int32_t frame, bmfr_stream s)
                                        s->p.storage == BMFR_STORAGE_F16, s->slot[S_ALBEDO],
                                        s->slot[s->cur(S_NORMALS_A)], s->slot[s->cur(S_POSITIONS_A)],
                                        s->slot[S_NOISY_IN], s->stream);
+    if (e != hipSuccess) return fail(BMFR_ERR_HIP, "scene render launch failed: %s", hipGetErrorString(e));
+    return BMFR_OK;
+}
+
+int32_t bmfr_scene_render_device(bmfr_scene sc, int32_t frame, int32_t storage, void *albedo, void *normals,
+                                 void *positions, void *noisy, void *hip_stream) {
+    if (!sc) return fail(BMFR_ERR_INVALID_ARGUMENT, "scene is null");
+    if (!albedo || !normals || !positions || !noisy) return fail(BMFR_ERR_INVALID_ARGUMENT, "null output buffer");
+    if (storage != BMFR_STORAGE_F32 && storage != BMFR_STORAGE_F16)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "unknown storage %d", storage);
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (!sc->device_copy) {
+        HIP_TRY(hipMalloc((void **)&sc->device_copy, sizeof(bmfr::SceneData)));
+        HIP_TRY(hipMemcpy(sc->device_copy, &sc->data, sizeof(bmfr::SceneData), hipMemcpyHostToDevice));
+    }
+    bmfr::CameraFrame c;
+    bmfr::scene_camera(sc->data, frame, c);
+    hipError_t e = bmfr::launch_render(sc->device_copy, sc->data.width, sc->data.height, c, frame,
+                                       storage == BMFR_STORAGE_F16, albedo, normals, positions, noisy,
+                                       (hipStream_t)hip_stream);
     if (e != hipSuccess) return fail(BMFR_ERR_HIP, "scene render launch failed: %s", hipGetErrorString(e));
     return BMFR_OK;
 }

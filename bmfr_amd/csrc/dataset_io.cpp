@@ -23,6 +23,7 @@
 
 #include "bmfr_hip.h"
 #include "host_error.h"
+#include "surface.h"
 
 namespace {
 
@@ -567,11 +568,7 @@ int32_t bmfr_png_write_rgb(const char *path, int32_t width, int32_t height, cons
         uint8_t *d = rows.data() + (size_t)y * (1 + 3 * (size_t)width);
         d[0] = 0;
         const float *src = rgb + (size_t)y * (size_t)row_stride * 3;
-        for (int i = 0; i < 3 * width; ++i) {
-            float v = src[i];
-            v = v > 0.f ? (v < 1.f ? v : 1.f) : 0.f;   // NaN -> 0
-            d[1 + i] = (uint8_t)(v * 255.f + 0.5f);
-        }
+        for (int i = 0; i < 3 * width; ++i) d[1 + i] = bmfr::quantize_u8(src[i]);   // NaN -> 0
     }
     uLongf n = compressBound((uLong)rows.size());
     std::vector<uint8_t> z(n);

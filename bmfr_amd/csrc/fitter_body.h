@@ -520,14 +520,17 @@ __device__ __forceinline__ void fitter_block(const TT *__restrict__ src, float *
     }
 
     // conditioning flag for the FP64 re-factorisation (BMFR_QR_KAHAN_FP64)
+    // fminf/fmaxf skip a NaN operand, so a NaN pivot norm is tracked on its own
     if (flag.flag && active && t == 0) {
-        float rmin = norms[0], rmax = norms[0];
+        float rmin = fabsf(norms[0]), rmax = fabsf(norms[0]);
+        bool nan = !(norms[0] == norms[0]);
 #pragma unroll
         for (int i = 1; i < kFeatures; ++i) {
             rmin = fminf(rmin, fabsf(norms[i]));
             rmax = fmaxf(rmax, fabsf(norms[i]));
+            nan |= !(norms[i] == norms[i]);
         }
-        *flag.flag = (rmin < flag.threshold * rmax || !(rmin == rmin)) ? 1 : 0;
+        *flag.flag = (rmin < flag.threshold * rmax || nan) ? 1 : 0;
     }
 
     // ---- back substitution, bmfr.cl:658-699, in wave 0 ----
@@ -889,7 +892,8 @@ __global__ __launch_bounds__(64 * (16 / RPL) * BPG) void kb_fitter(const BatchLa
         const int s = batch_stream_of(starts, L.n, u);
         const int blk = u - starts[s];
         const BatchStream &bs = L.streams[s];
-        int32_t *fl = (int32_t *)bs.flags;
+        // flags only for BMFR_QR_KAHAN_FP64 streams, as the single-stream launch
+        int32_t *fl = bs.args.qr_mode == 2 ? (int32_t *)bs.flags : nullptr;
         if constexpr (!COMP) {
             fitter_block_fast<RPL, BPG, TT, HALF_ROUND>(
                 reinterpret_cast<const TT *>(bs.tmp_data) + (int64_t)blk * (kBufferCount * kBlockPixels),

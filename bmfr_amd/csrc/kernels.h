@@ -35,6 +35,8 @@ struct StageArgs {
     int32_t qr_mode;       // bmfr_qr_mode
     float refactor_threshold;
     FeatureSet features;   // read by the kernels of a configured feature set only
+    int32_t surface_format;   // bmfr_surface_format of StageBuffers::surface
+    int64_t surface_pitch;    // its row pitch in bytes
 };
 
 // Device pointers of one stream's buffer set for the current frame (the
@@ -52,6 +54,7 @@ struct StageBuffers {
     void *result_cur, *result_prev;
     void *noise;          // float[(N-1)*1024]: the fitter's feature noise of one frame
     void *flags;          // int32[blocks]: FP64 re-factorisation flags
+    void *surface;        // caller's output surface (null: none)
 };
 
 // Fitter launch shape, chosen per stream (or by the autotuner).
@@ -65,17 +68,23 @@ struct FitterConfig {
 // both halves of every double buffer.
 struct BatchStream {
     StageArgs args;  // constants; frame/matrix/offset come from BatchStep
-    void *albedo, *noisy_in, *prev_pixel, *accept, *tmp_data, *weights, *mins_maxs;
+    void *prev_pixel, *accept, *tmp_data, *weights, *mins_maxs;
     void *filtered, *tone_mapped, *noise, *flags;
-    void *normals[2], *positions[2], *noisy_acc[2], *spp[2], *out[2], *result[2];
+    void *noisy_acc[2], *spp[2], *out[2], *result[2];
 };
 
-// Per-step parameters of one stream (written by the host before each step).
+// Per-step parameters of one stream (written by the host before each step):
+// the frame's arguments, the double-buffer parity, and the frame's input
+// G-buffers (the stream's own or caller-bound ones) and output surface.
 struct BatchStep {
     int32_t frame;
     int32_t cur;  // index of the current half of the double buffers
     float matrix[16];
     float pixel_offset_x, pixel_offset_y;
+    const void *albedo, *normals_cur, *normals_prev, *positions_cur, *positions_prev, *noisy_in;
+    void *surface;
+    int32_t surface_format;
+    int64_t surface_pitch;
 };
 
 enum { kStartsPadded = 0, kStartsImage = 1, kStartsBlocks = 2, kStartKinds = 3 };

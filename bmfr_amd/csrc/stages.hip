@@ -8,9 +8,12 @@
 // wave writes whole 32-pixel rows of one block of tmp_data (block-major,
 // [by][bx][feature][y][x]); stages 3-5 run over the image in 64x4 tiles so
 // that a wave reads 64 consecutive AoS pixels (768 contiguous bytes).
+#include <type_traits>
+
 #include "device_common.h"
 #include "kernels.h"
 #include "stage1.h"
+#include "surface.h"
 #include "tone_curve.h"
 
 namespace bmfr {
@@ -498,6 +501,19 @@ __device__ __forceinline__ float3 taa_resolve(const StageArgs &a, float2 pf, flo
     return r;
 }
 
+// The frame's TAA result into the result history (result_buffer.current())
+// and, when one is bound, the caller's output surface, which receives the
+// stored value (rounded to the history's storage type).
+template <typename GT>
+__device__ __forceinline__ void store_result(const StageArgs &a, const StageBuffers &bufs, int64_t lp, int x, int y,
+                                             float3 r) {
+    store3((GT *)bufs.result_cur, lp, r);
+    if (bufs.surface) {
+        if constexpr (!std::is_same<GT, float>::value) r = make_float3(round_half(r.x), round_half(r.y), round_half(r.z));
+        store_surface(bufs.surface, a.surface_format, a.surface_pitch, x, y, r);
+    }
+}
+
 __device__ __forceinline__ float4 ycocg_texel(float3 rgb) {
     const float3 yc = rgb_to_ycocg(rgb);
     return make_float4(yc.x, yc.y, yc.z, 0.f);
@@ -530,7 +546,7 @@ __device__ __forceinline__ void stage5_body(const StageArgs &a, const StageBuffe
     }
     __syncthreads();
     if (!inside) return;
-    store3((GT *)bufs.result_cur, lp, taa_resolve<kTaaTileW, GT>(a, pf, mine, h, &tile[ty][tx]));
+    store_result<GT>(a, bufs, lp, x, y, taa_resolve<kTaaTileW, GT>(a, pf, mine, h, &tile[ty][tx]));
 }
 
 // ---------------------------------------------------------------------------
@@ -627,7 +643,7 @@ __device__ __forceinline__ void stage345_body(const StageArgs &a, const StageBuf
         if (x >= 0 && y >= 0 && x < W && y < H) {
             const int64_t lp = (int64_t)y * W + x;
             const TaaHistory<GT> h = taa_history<GT>(a, bufs, lp, pf[r]);
-            store3((GT *)bufs.result_cur, lp, taa_resolve<kRing, GT>(a, pf[r], mine[r], h, &tile[ty + 8 * r][tx]));
+            store_result<GT>(a, bufs, lp, x, y, taa_resolve<kRing, GT>(a, pf[r], mine[r], h, &tile[ty + 8 * r][tx]));
         }
     }
 }
@@ -658,13 +674,16 @@ __device__ __forceinline__ void resolve_batch(const BatchLaunch &L, int s, Stage
     for (int i = 0; i < 16; ++i) a.matrix[i] = st.matrix[i];
     a.pixel_offset_x = st.pixel_offset_x;
     a.pixel_offset_y = st.pixel_offset_y;
+    a.surface_format = st.surface_format;
+    a.surface_pitch = st.surface_pitch;
     const int c = st.cur, p = 1 - st.cur;
-    b.albedo = bs.albedo;
-    b.normals_cur = bs.normals[c];
-    b.normals_prev = bs.normals[p];
-    b.positions_cur = bs.positions[c];
-    b.positions_prev = bs.positions[p];
-    b.noisy_in = bs.noisy_in;
+    b.albedo = const_cast<void *>(st.albedo);
+    b.normals_cur = const_cast<void *>(st.normals_cur);
+    b.normals_prev = const_cast<void *>(st.normals_prev);
+    b.positions_cur = const_cast<void *>(st.positions_cur);
+    b.positions_prev = const_cast<void *>(st.positions_prev);
+    b.noisy_in = const_cast<void *>(st.noisy_in);
+    b.surface = st.surface;
     b.noisy_acc_cur = bs.noisy_acc[c];
     b.noisy_acc_prev = bs.noisy_acc[p];
     b.spp_cur = bs.spp[c];

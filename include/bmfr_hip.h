@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define BMFR_API_VERSION 2
+#define BMFR_API_VERSION 3
 
 typedef enum bmfr_status {
     BMFR_OK = 0,
@@ -196,6 +196,53 @@ int32_t bmfr_upload_frame(bmfr_stream stream, const float *albedo, const float *
 int32_t bmfr_copy_frame_inputs_device(bmfr_stream stream, const void *albedo, const void *normals,
                                       const void *positions, const void *noisy);
 
+/* ---- renderer handoff (opencl/bmfr.cpp:415-416, 478-480) -------------------
+ * The reference uploads every frame from the host and reads the result back,
+ * and notes that "in real use case there would not be WriteBuffer and
+ * ReadBuffer function calls because the input data comes from the path
+ * tracer and output goes to the screen".  These calls are that case. */
+/* Zero-copy inputs: the stream's next frame (its next
+ * bmfr_accumulate_noisy_data, bmfr_run_frame, batch step or server step, up
+ * to the bmfr_end_frame that closes it) reads these caller-owned device
+ * buffers instead of the stream's own input buffers: albedo as float[3P],
+ * normals, positions and noisy colour in the stream's storage type (float or
+ * half [3P]), AoS with row stride `width`, 16-byte aligned.  No copy is made.
+ * The binding covers one frame; after it the stream uses its own buffers
+ * again unless bound anew.  The frame's normals and positions are also the
+ * NEXT frame's previous G-buffers (read by its accumulate_noisy_data, as
+ * normals_buffer.previous() in opencl/bmfr.cpp:431-434): keep them unchanged
+ * until that work has completed (a renderer with two G-buffer sets used in
+ * turn does this naturally).  All four null: drop the binding.  The debug
+ * accessors (bmfr_read_buffer and friends) still name the stream's own
+ * buffers. */
+int32_t bmfr_bind_device_inputs(bmfr_stream stream, const void *albedo, const void *normals,
+                                const void *positions, const void *noisy);
+
+/* Output surfaces: every later frame's TAA result (the value stored in the
+ * result history, i.e. what bmfr_download_result returns) is also written
+ * into a caller-owned device surface, by the kernel that computes it: row y
+ * of the image starts at surface + y * row_pitch_bytes. */
+typedef enum bmfr_surface_format {
+    BMFR_SURFACE_RGB_F32 = 0,   /* float[3] per pixel (the reference's out_data) */
+    BMFR_SURFACE_RGBA_F32 = 1,  /* float[4], alpha 1 */
+    BMFR_SURFACE_RGBA_F16 = 2,  /* half[4] (round to nearest even), alpha 1 */
+    BMFR_SURFACE_RGBA_U8 = 3    /* uint8[4]: round(clamp(v, 0, 1) * 255) per channel, the
+                                   8-bit conversion of the reference's PNG output
+                                   (opencl/bmfr.cpp:519-547, bmfr_png_write_rgb), alpha 255 */
+} bmfr_surface_format;
+/* Binds (surface non-null) or unbinds (null) the stream's output surface;
+ * the binding lasts until changed.  row_pitch_bytes must hold a row
+ * (>= width x pixel size) and be a multiple of the pixel's alignment (4 for
+ * U8 and F32 formats, 8 for RGBA_F16); the surface must be aligned likewise.
+ * The surface is written on the stream's HIP stream (bmfr_stream_hip_stream)
+ * or, for batch and server steps, on theirs, which the stream's later work
+ * waits for. */
+int32_t bmfr_bind_output_surface(bmfr_stream stream, void *surface, int32_t format,
+                                 int64_t row_pitch_bytes);
+/* The HIP stream (hipStream_t) the stream's work is enqueued on, so that a
+ * renderer or compositor can order its own work against it with events. */
+int32_t bmfr_stream_hip_stream(bmfr_stream stream, void **hip_stream);
+
 /* ---- the five stages (opencl/bmfr.cpp:429-476) ---------------------------- */
 /* prev_frame_camera_matrix: 16 floats, the previous frame's view-projection
  * in the reference's row-vector layout (camera_matrices[max(f-1,0)],
@@ -309,27 +356,115 @@ int32_t bmfr_batch_record_marker(bmfr_batch batch, int32_t marker);
 int32_t bmfr_batch_marker_elapsed(bmfr_batch batch, int32_t m0, int32_t m1, float *ms);
 
 /* ---- server: streams admitted by HBM budget, continuous batching ------------
- * The server owns the streams it admits (bmfr_server_open_stream refuses with
- * BMFR_ERR_OUT_OF_BUDGET when the stream's device footprint would exceed the
- * budget) and runs dynamic batched steps: each bmfr_server_step takes one
- * frame of any subset of its streams (the ones with work pending), one
- * launch per stage per storage group, ordered after each member's pending
- * work (its inputs) and before its later work.  Steps are asynchronous; a
- * step id (reused after 4 steps) can be polled for completion and GPU time.
- * No counterpart in the reference (one stream, opencl/bmfr.cpp:417-485). */
+ * No counterpart in the reference, which runs one stream with five launches
+ * per frame (opencl/bmfr.cpp:417-485); a server runs many render streams of
+ * any resolution and storage precision.  It owns the streams it admits (their
+ * device footprint counts against its HBM budget) and runs batched steps: one
+ * frame of each of a set of its streams, one launch per stage per launch
+ * group (storage x tmp storage x QR mode), ordered after each member's
+ * pending work (its inputs) and before its later work.  A stream's results
+ * are the same in any step as when it runs alone.  Two ways to drive it:
+ *  - explicit steps: bmfr_server_step launches one step now;
+ *  - the request loop: bmfr_server_submit queues a frame of a stream, and a
+ *    dispatcher thread inside the library forms steps continuously: whenever
+ *    fewer than `max_in_flight` steps (default 2) are on the GPU it launches
+ *    one holding the oldest queued frame of every stream that has one (at
+ *    most `max_streams_per_step`, oldest arrivals first); completions come
+ *    back through bmfr_server_poll with their timestamps.
+ * Step ids are a monotonically increasing count (the first step is 1). */
 int32_t bmfr_server_create(int64_t hbm_budget_bytes, bmfr_server *out);
+/* Waits for the steps on the GPU, drops frames still queued, closes every
+ * stream the server owns and frees the server. */
 int32_t bmfr_server_destroy(bmfr_server server);
 int32_t bmfr_server_usage(bmfr_server server, int64_t *used_bytes, int64_t *budget_bytes,
                           int32_t *open_streams);
+/* Opens a stream now, or refuses with BMFR_ERR_OUT_OF_BUDGET. */
 int32_t bmfr_server_open_stream(bmfr_server server, const bmfr_params *params, bmfr_stream *out);
+/* Admission FIFO: opens the stream now if nothing is waiting and it fits
+ * (*out = the stream); otherwise queues the request (*out = NULL) and the
+ * server opens it, in request order, once closes have freed enough of the
+ * budget, reporting it as a BMFR_EVENT_STREAM_ADMITTED event with `tag`.  A
+ * stream larger than the whole budget is refused (BMFR_ERR_OUT_OF_BUDGET). */
+int32_t bmfr_server_request_stream(bmfr_server server, const bmfr_params *params, uint64_t tag,
+                                   bmfr_stream *out);
+/* Withdraws a queued admission request (*found = 1 if it was still queued). */
+int32_t bmfr_server_cancel_request(bmfr_server server, uint64_t tag, int32_t *found);
+/* Waits until the stream's queued frames have run, then closes it and frees
+ * its budget (which may admit queued requests). */
 int32_t bmfr_server_close_stream(bmfr_server server, bmfr_stream stream);
+/* One step now over `count` distinct streams of the server (none of which may
+ * have frames queued through bmfr_server_submit); frames[i], matrices[16*i..]
+ * and offsets[2*i..] are stream i's bmfr_accumulate_noisy_data arguments. */
 int32_t bmfr_server_step(bmfr_server server, int32_t count, const bmfr_stream *streams,
                          const int32_t *frames, const float *matrices, const float *offsets,
-                         int32_t *step_id);
-int32_t bmfr_server_step_query(bmfr_server server, int32_t step_id, int32_t *done, float *gpu_ms);
+                         int64_t *step_id);
+/* *done = 1 and the step's GPU time once it has completed, 0 while it runs.
+ * BMFR_ERR_STATE for a step whose record has been reused (the server keeps
+ * the last few steps' records). */
+int32_t bmfr_server_step_query(bmfr_server server, int64_t step_id, int32_t *done, float *gpu_ms);
 int32_t bmfr_server_synchronize(bmfr_server server);
 /* BMFR_FUSE_34 or BMFR_FUSE_345 (default) for later steps' stages 3-5. */
 int32_t bmfr_server_set_fused(bmfr_server server, int32_t level);
+/* Request-loop policy: steps on the GPU at once (1..4, default 2; 0 pauses
+ * launching, during which bmfr_server_close_stream and
+ * bmfr_server_synchronize wait for it to resume) and streams per step
+ * (1..64, default 64). */
+int32_t bmfr_server_set_policy(bmfr_server server, int32_t max_in_flight, int32_t max_streams_per_step);
+
+/* Seconds on the clock the request loop's timestamps use (a monotonic clock,
+ * CLOCK_MONOTONIC on Linux). */
+double bmfr_clock_seconds(void);
+
+/* One frame of a stream for the request loop. */
+typedef struct bmfr_frame_request {
+    bmfr_stream stream;
+    int32_t frame_number;                   /* as bmfr_accumulate_noisy_data's */
+    float prev_frame_camera_matrix[16];
+    float pixel_offset[2];
+    /* Device inputs, as bmfr_bind_device_inputs (the frame's normals and
+     * positions stay in use until the stream's next frame has run); all null:
+     * the stream's own input buffers, which hold one frame, so such a request
+     * is refused (BMFR_ERR_STATE) while another frame of the stream waits. */
+    const void *albedo, *normals, *positions, *noisy;
+    /* Output surface for this frame (null: the stream's bound surface, if any). */
+    void *surface;
+    int32_t surface_format;                 /* bmfr_surface_format */
+    int64_t surface_row_pitch;              /* bytes */
+    /* hipStream_t on which the inputs are being written: the step waits for
+     * the work enqueued on it before this call (null: the inputs are complete;
+     * the stream's own HIP stream is always waited for). */
+    void *producer_stream;
+    double t_arrival;                       /* seconds, bmfr_clock_seconds() */
+    uint64_t tag;                           /* returned with the completion */
+} bmfr_frame_request;
+/* Queues a frame; frames of one stream run in submission order, one per step.
+ * The caller keeps the inputs and the surface valid until the frame's
+ * BMFR_EVENT_FRAME_DONE (normals and positions: until the stream's next
+ * frame's). */
+int32_t bmfr_server_submit(bmfr_server server, const bmfr_frame_request *request);
+
+typedef enum bmfr_event_kind {
+    BMFR_EVENT_FRAME_DONE = 1,       /* a submitted frame's step has completed */
+    BMFR_EVENT_FRAME_FAILED = 2,     /* its step could not be launched (status) */
+    BMFR_EVENT_STREAM_ADMITTED = 3   /* a queued bmfr_server_request_stream was opened */
+} bmfr_event_kind;
+typedef struct bmfr_server_event {
+    int32_t kind;                    /* bmfr_event_kind */
+    int32_t status;                  /* bmfr_status (FRAME_FAILED) */
+    bmfr_stream stream;
+    uint64_t tag;                    /* the request's tag */
+    int32_t frame_number;
+    int32_t step_streams;            /* streams in the frame's step */
+    int64_t step_id;
+    double t_arrival;                /* the request's t_arrival (admission: request time) */
+    double t_launch;                 /* when the step was enqueued */
+    double t_done;                   /* when the dispatcher saw it complete (admission time) */
+    float step_gpu_ms;               /* the step's GPU time, first launch to last end */
+} bmfr_server_event;
+/* Up to max_events events, oldest first; waits up to timeout_us for the
+ * first one (0: does not wait). */
+int32_t bmfr_server_poll(bmfr_server server, bmfr_server_event *events, int32_t max_events,
+                         int32_t timeout_us, int32_t *count);
 
 /* ---- procedural scene (replaces the EXR dataset, opencl/bmfr.cpp:145-172) ---- */
 typedef struct bmfr_scene_s *bmfr_scene;
@@ -350,6 +485,12 @@ int32_t bmfr_scene_render_host(bmfr_scene scene, int32_t frame, float *albedo,
 /* GPU render of one frame straight into the stream's current input buffers
  * (albedo, normals, positions, noisy colour), enqueued on the stream. */
 int32_t bmfr_scene_render_to_stream(bmfr_scene scene, int32_t frame, bmfr_stream stream);
+/* GPU render of one frame into caller-owned device buffers (a renderer's own
+ * G-buffers, for bmfr_bind_device_inputs or bmfr_server_submit): albedo as
+ * float[3P], the other three as float (storage BMFR_STORAGE_F32) or half
+ * [3P]; enqueued on `hip_stream` (a hipStream_t; null: the null stream). */
+int32_t bmfr_scene_render_device(bmfr_scene scene, int32_t frame, int32_t storage, void *albedo,
+                                 void *normals, void *positions, void *noisy, void *hip_stream);
 
 /* ---- dataset-format IO (the reference's OpenImageIO and camera header use) ----
  * Host-only helpers so that the published scenes run when a user supplies

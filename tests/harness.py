@@ -95,3 +95,100 @@ def rel_err(a, b, floor=1e-2):
 def psnr(img, ref):
     mse = np.mean((np.asarray(img, np.float64) - np.asarray(ref, np.float64)) ** 2)
     return float("inf") if mse == 0 else 10 * np.log10(1.0 / mse)
+
+
+class Hip:
+    """The few HIP runtime calls the tests make themselves (device buffers a
+    renderer would own, and a producer HIP stream), through the runtime's C
+    API; plumbing only."""
+    _rt = None
+
+    @classmethod
+    def rt(cls):
+        if cls._rt is None:
+            import ctypes
+            rt = ctypes.CDLL("libamdhip64.so")
+            vp = ctypes.c_void_p
+            rt.hipMalloc.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t]
+            rt.hipFree.argtypes = [vp]
+            rt.hipMemcpy.argtypes = [vp, vp, ctypes.c_size_t, ctypes.c_int]
+            rt.hipMemset.argtypes = [vp, ctypes.c_int, ctypes.c_size_t]
+            rt.hipStreamCreate.argtypes = [ctypes.POINTER(vp)]
+            rt.hipStreamDestroy.argtypes = [vp]
+            rt.hipStreamSynchronize.argtypes = [vp]
+            rt.hipDeviceSynchronize.argtypes = []
+            cls._rt = rt
+        return cls._rt
+
+    @classmethod
+    def check(cls, err, what):
+        assert err == 0, f"{what} failed: hipError {err}"
+
+
+class DeviceBuffer:
+    """A caller-owned device allocation (int pointer in .ptr)."""
+
+    def __init__(self, nbytes, fill=None):
+        import ctypes
+        p = ctypes.c_void_p()
+        Hip.check(Hip.rt().hipMalloc(ctypes.byref(p), int(nbytes)), "hipMalloc")
+        self.ptr, self.nbytes = p.value, int(nbytes)
+        if fill is not None:
+            Hip.check(Hip.rt().hipMemset(self.ptr, int(fill), self.nbytes), "hipMemset")
+
+    def read(self, dtype=np.uint8):
+        out = np.empty(self.nbytes // np.dtype(dtype).itemsize, dtype)
+        Hip.check(Hip.rt().hipDeviceSynchronize(), "hipDeviceSynchronize")
+        Hip.check(Hip.rt().hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, 2), "hipMemcpy D2H")
+        return out
+
+    def free(self):
+        if self.ptr:
+            Hip.rt().hipFree(self.ptr)
+            self.ptr = None
+
+
+class HipStream:
+    def __init__(self):
+        import ctypes
+        p = ctypes.c_void_p()
+        Hip.check(Hip.rt().hipStreamCreate(ctypes.byref(p)), "hipStreamCreate")
+        self.handle = p.value
+
+    def synchronize(self):
+        Hip.check(Hip.rt().hipStreamSynchronize(self.handle), "hipStreamSynchronize")
+
+    def destroy(self):
+        if self.handle:
+            Hip.rt().hipStreamDestroy(self.handle)
+            self.handle = None
+
+
+class FrameInputs:
+    """One renderer-owned set of a frame's G-buffers: albedo float[3P], the
+    others float or half [3P] per the stream's storage."""
+
+    def __init__(self, w, h, storage_f16):
+        n = w * h * 3
+        s = 2 if storage_f16 else 4
+        self.storage = 1 if storage_f16 else 0
+        self.bufs = [DeviceBuffer(4 * n), DeviceBuffer(s * n), DeviceBuffer(s * n), DeviceBuffer(s * n)]
+
+    @property
+    def ptrs(self):
+        return tuple(b.ptr for b in self.bufs)
+
+    def render(self, sc, frame, hip_stream=None):
+        sc.render_device(frame, self.storage, *self.ptrs, hip_stream=hip_stream)
+
+    def free(self):
+        for b in self.bufs:
+            b.free()
+
+
+def quantize_u8(v):
+    """round(clamp(v, 0, 1) * 255) in float32 with NaN -> 0: the 8-bit
+    conversion of the reference's PNG output (opencl/bmfr.cpp:519-547)."""
+    v = np.asarray(v, np.float32)
+    c = np.where(v > 0, np.where(v < 1, v, np.float32(1)), np.float32(0)).astype(np.float32)
+    return (c * np.float32(255) + np.float32(0.5)).astype(np.float32).astype(np.uint8)

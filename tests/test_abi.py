@@ -5,6 +5,7 @@ configuration constrains them, and derives the reference's sizes
 import ctypes
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -33,7 +34,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_api_version_and_status_strings():
     lib = _lib.load()
-    assert lib.bmfr_api_version() == 2
+    assert lib.bmfr_api_version() == 3
     for st in range(6):
         assert lib.bmfr_status_string(st)
     assert lib.bmfr_status_string(99) == b"unknown status"
@@ -108,3 +109,52 @@ def test_device_footprint_720p():
     assert 0.95 * (14 * 12 * P + 11 * P + 26 * Q) < f32 < 1.05 * (14 * 12 * P + 11 * P + 26 * Q)
     p16 = bmfr_amd.default_params(1280, 720, storage=bmfr_amd.STORAGE_F16)
     assert bmfr_amd.compute_sizes(p16).device_bytes < 0.7 * f32
+
+
+def test_header_compiles_as_c99_and_cxx(tmp_path):
+    """include/bmfr_hip.h is plain C: C99 with every warning, and C++."""
+    inc = os.path.join(ROOT, "include")
+    src = tmp_path / "h.c"
+    src.write_text('#include "bmfr_hip.h"\nint main(void) { return bmfr_api_version() == 3 ? 0 : 1; }\n')
+    for cmd in (["gcc", "-x", "c", "-std=c99", "-pedantic", "-Wall", "-Wextra", "-Werror"],
+                ["g++", "-x", "c++", "-std=c++11", "-pedantic", "-Wall", "-Wextra", "-Werror"]):
+        r = subprocess.run(cmd + ["-fsyntax-only", "-I", inc, str(src)], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+
+
+def test_struct_layouts_match_the_python_mirror(tmp_path):
+    """The ctypes structures have the C compiler's sizes and offsets."""
+    inc = os.path.join(ROOT, "include")
+    structs = {"bmfr_params": _lib.BmfrParams, "bmfr_sizes": _lib.BmfrSizes,
+               "bmfr_frame_request": _lib.BmfrFrameRequest, "bmfr_server_event": _lib.BmfrServerEvent}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "bmfr_hip.h"', 'int main(void) {']
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    r = subprocess.run(["gcc", "-std=c99", "-I", inc, str(src), "-o", str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    got = {}
+    for line in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines():
+        c, f, v = line.split()
+        got[(c, f)] = int(v)
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)
+
+
+def test_new_entry_points_validate_without_a_gpu():
+    lib = _lib.load()
+    assert lib.bmfr_bind_device_inputs(None, None, None, None, None) == bmfr_amd.ERR_INVALID_ARGUMENT
+    assert lib.bmfr_bind_output_surface(None, None, 0, 0) == bmfr_amd.ERR_INVALID_ARGUMENT
+    assert lib.bmfr_server_submit(None, None) == bmfr_amd.ERR_INVALID_ARGUMENT
+    n = ctypes.c_int32()
+    assert lib.bmfr_server_poll(None, None, 0, 0, ctypes.byref(n)) == bmfr_amd.ERR_INVALID_ARGUMENT
+    assert lib.bmfr_server_step_query(None, 1, None, None) == bmfr_amd.ERR_INVALID_ARGUMENT
+    t0 = lib.bmfr_clock_seconds()
+    assert lib.bmfr_clock_seconds() >= t0 > 0

@@ -11,20 +11,27 @@ from tests.harness import gpu_params, scene
 
 pytestmark = pytest.mark.gpu
 
-MIX = [  # (w, h, storage_f16, tmp_f16, seed offset)
-    (96, 64, False, True, 0),
-    (130, 70, False, True, 1),     # ragged: W, H not multiples of 32
-    (64, 48, True, True, 2),       # FP16 storage: a second launch group
-    (200, 120, False, False, 3),   # FP32 tmp_data: a third group
+MIX = [  # (w, h, storage_f16, tmp_f16, seed offset, qr mode)
+    (96, 64, False, True, 0, bmfr_amd.QR_FP32),
+    (130, 70, False, True, 1, bmfr_amd.QR_FP32),     # ragged: W, H not multiples of 32
+    (64, 48, True, True, 2, bmfr_amd.QR_FP32),       # FP16 storage: a second launch group
+    (200, 120, False, False, 3, bmfr_amd.QR_FP32),   # FP32 tmp_data: a third group
+    (96, 64, False, False, 4, bmfr_amd.QR_KAHAN),    # compensated QR
+    (130, 70, True, False, 5, bmfr_amd.QR_KAHAN_FP64),   # + FP64 re-factorisation
 ]
+# A threshold at which many blocks are re-factorised in FP64 (the default,
+# 1e-6, flags almost none of these scenes' blocks).
+REFACTOR_THRESHOLD = 0.5
 
 
 def _streams(mix):
-    return [bmfr_amd.Stream(gpu_params(w, h, tmp_f16=t16, storage_f16=g16)) for (w, h, g16, t16, _) in mix]
+    return [bmfr_amd.Stream(gpu_params(w, h, tmp_f16=t16, storage_f16=g16, qr_mode=qr,
+                                       refactor_threshold=REFACTOR_THRESHOLD))
+            for (w, h, g16, t16, _, qr) in mix]
 
 
 def _scenes(mix):
-    return [scene(w, h, seed=0xB3F40000 + 3 + k) for (w, h, _, _, k) in mix]
+    return [scene(w, h, seed=0xB3F40000 + 3 + k) for (w, h, _, _, k, _) in mix]
 
 
 def _buffers(st, fuse):
@@ -32,7 +39,9 @@ def _buffers(st, fuse):
              "prev_pixel", "accept"]
     if fuse == bmfr_amd.FUSE_34:  # the 3+4+5 kernel keeps tone_mapped on chip
         names.append("tone_mapped")
-    return {n: st.read_buffer(n) for n in names}
+    d = {n: st.read_buffer(n) for n in names}
+    d["refactored"] = np.array([st.refactor_count()])
+    return d
 
 
 def _run(mix, frames, batched, graph=False, eager_between=False, fuse=bmfr_amd.FUSE_345):
@@ -86,11 +95,13 @@ def test_batch_equals_each_stream_alone(gpu, fuse):
     for k, (a, m) in enumerate(zip(alone, mixed)):
         for name in a:
             assert _same(a[name], m[name]), (k, name)
+    # the compensated streams: only the FP64 one re-factorises, and it does
+    assert [int(a["refactored"][0]) > 0 for a in alone] == [False] * 5 + [True]
 
 
 def test_batch_graph_replay_equals_eager(gpu):
-    eager = _run(MIX[:3], 6, batched=True)
-    graph = _run(MIX[:3], 6, batched=True, graph=True, eager_between=True)
+    eager = _run(MIX[:3] + MIX[4:], 6, batched=True)
+    graph = _run(MIX[:3] + MIX[4:], 6, batched=True, graph=True, eager_between=True)
     for k, (a, m) in enumerate(zip(eager, graph)):
         for name in a:
             assert _same(a[name], m[name]), (k, name)

@@ -117,11 +117,20 @@ def run_plain(args):
             totals.append(float(st.stage_times()[5]))
         elif f >= args.warmup + args.steps:
             per_stage.append(st.stage_times())
+    # the fitter's kernel time without dispatch gaps: the last frame's fitter
+    # launched back to back, events only around the whole run
     st.synchronize()
+    st.set_timing(0)
+    reps = 50
+    st.record_marker(0)
+    for _ in range(reps):
+        st.fitter(n_frames - 1)
+    st.record_marker(1)
+    fitter_b2b_ms = st.marker_elapsed(0, 1) / reps
     s = st.sizes
     st.close()
     sc.close()
-    return np.array(totals), np.array(per_stage, np.float64), wall, s
+    return np.array(totals), np.array(per_stage, np.float64), wall, s, fitter_b2b_ms
 
 
 BATCH_MIX = [(640, 360)] * 4 + [(1280, 720)] * 6 + [(1920, 1080)] * 4 + [(2560, 1440)] * 2
@@ -283,177 +292,44 @@ def run_feature_sets(frames, fuse):
     return out
 
 
-SERVE_RES = [(640, 360), (1280, 720), (1920, 1080), (2560, 1440)]
-SERVE_P = [0.2, 0.5, 0.25, 0.05]
+SERVE_LOAD = os.path.join(ROOT, "bmfr_amd", "bmfr_serve_load")
 
 
-def poisson_arrivals(rng, t_open, t_close, rate):
-    arr, t = [], t_open
-    while True:
-        t += rng.exponential(1.0 / rate)
-        if t >= t_close:
-            return arr
-        arr.append(t)
+def run_serve_load(*opts, timeout=600):
+    """One run of bmfr_serve_load, the C++ client of the library's request
+    loop (bmfr_server_submit / bmfr_server_poll): its JSON line as a dict."""
+    import subprocess
+    r = subprocess.run([SERVE_LOAD] + [str(o) for o in opts], capture_output=True, text=True, timeout=timeout)
+    if r.returncode != 0:
+        raise RuntimeError(f"bmfr_serve_load {' '.join(map(str, opts))} failed ({r.returncode}): {r.stderr[-2000:]}")
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
 
-def run_served(duration, budget_gib, seed=0xB3F40000 + 4):
-    """64 render streams opened and closed over `duration` seconds of wall
-    time; per stream a resolution from SERVE_RES with probabilities SERVE_P,
-    Poisson frame arrivals at 30 or 60 Hz and FP16 or FP32 storage."""
-    rng = np.random.default_rng(seed)
-    specs = []
-    for k in range(64):
-        w, h = SERVE_RES[rng.choice(4, p=SERVE_P)]
-        t_open = rng.uniform(0.0, 0.8 * duration)
-        t_close = min(duration, t_open + rng.uniform(0.15, 0.4) * duration)
-        rate = 30.0 if rng.random() < 0.5 else 60.0
-        f16 = bool(rng.random() < 0.5)
-        specs.append(dict(w=w, h=h, open=t_open, close=t_close, rate=rate, f16=f16,
-                          arrivals=poisson_arrivals(rng, t_open, t_close, rate)))
-    return serve(specs, duration, budget_gib, seed)
+def run_served(duration, budget_gib):
+    """64 render streams opened and closed over `duration` seconds: per stream
+    640x360 / 1280x720 / 1920x1080 / 2560x1440 with probabilities 0.2 / 0.5 /
+    0.25 / 0.05, Poisson frame arrivals at 30 or 60 Hz, FP16 or FP32
+    storage; admitted by the HBM budget (admission FIFO); every frame handed
+    over as device G-buffers with an RGBA8 output surface."""
+    return run_serve_load("--mode", "served", "--streams", 64, "--seconds", duration, "--budget-gib", budget_gib)
 
 
-def run_serve_sweep(counts, duration, seed=0xB3F40000 + 5):
-    """p50/p99 frame latency against the number of concurrent streams: N
-    streams of 1280x720 (FP32 storage), all open for `duration` seconds, each
-    with Poisson arrivals at 60 Hz; latency of the frames arriving after the
-    first second (all N streams are admitted and allocated by then)."""
+def run_serve_sweep(counts, duration):
+    """p50/p99 frame latency against the number of concurrent 1280x720 FP32
+    streams at 60 Hz (latency of the frames arriving after the first second),
+    up to and past the GPU's saturation: a stream holds at most 3 frames in
+    the denoiser and drops one that arrives beyond that."""
     out = []
     for n in counts:
-        rng = np.random.default_rng(seed + n)
-        specs = [dict(w=1280, h=720, open=0.0, close=duration, rate=60.0, f16=False,
-                      arrivals=poisson_arrivals(rng, 0.0, duration, 60.0)) for _ in range(n)]
-        r = serve(specs, duration, 64.0, seed + n, settle=1.0)
-        out.append({k: r[k] for k in ("streams", "admitted", "frames", "aggregate_frames_per_s", "mpixel_per_s",
-                                      "latency_ms_p50", "latency_ms_p99", "mean_streams_per_step",
-                                      "step_gpu_ms_mean")})
-        progress(f"serve sweep: {n} streams, p99 {r['latency_ms_p99']} ms, "
-                 f"{r['mean_streams_per_step']} streams/step")
+        budget = max(4.0, 0.2 * n + 4.0)   # 191 MB a stream
+        r = run_serve_load("--mode", "sweep", "--streams", n, "--seconds", duration, "--settle", 1.0,
+                           "--budget-gib", budget)
+        out.append({k: r[k] for k in ("streams", "frames_arrived", "frames", "dropped", "aggregate_frames_per_s",
+                                      "mpixel_per_s", "latency_ms_p50", "latency_ms_p99", "latency_ms_max",
+                                      "mean_streams_per_step", "step_gpu_ms_mean", "gpu_busy")})
+        progress(f"serve sweep: {n} streams, p99 {r['latency_ms_p99']} ms, {r['mean_streams_per_step']} "
+                 f"streams/step, {r['dropped']} of {r['frames_arrived']} frames dropped, busy {r['gpu_busy']}")
     return out
-
-
-def serve(specs, duration, budget_gib, seed, settle=0.0):
-    """Streams are admitted by the server's HBM budget (a FIFO waits for
-    memory), and every step batches the oldest pending frame of each stream
-    with work (continuous batching, at most two steps in flight).  Latency =
-    host time at which a frame's step is seen complete minus the frame's
-    arrival, for frames arriving at or after `settle` seconds."""
-    import bmfr_amd
-    n = len(specs)
-    sv = bmfr_amd.Server(int(budget_gib * (1 << 30)))
-    state = [dict(stream=None, scene=None, next=0, pending=[], frame=0, done=False) for _ in specs]
-    waiting, admitted, waited = [], 0, 0
-    latencies, step_ms, step_sizes, in_flight = [], [], [], []
-    frames_done, pixels_done = 0, 0
-    t0 = time.perf_counter()
-    order = sorted(range(n), key=lambda k: specs[k]["open"])
-    oi = 0
-    while True:
-        now = time.perf_counter() - t0
-        # streams that open now join the admission queue
-        while oi < n and specs[order[oi]]["open"] <= now:
-            waiting.append(order[oi])
-            oi += 1
-        still = []
-        for k in waiting:
-            sp = specs[k]
-            if now >= sp["close"]:
-                continue  # never admitted before its end
-            try:
-                p = bmfr_amd.default_params(sp["w"], sp["h"],
-                                            storage=bmfr_amd.STORAGE_F16 if sp["f16"] else bmfr_amd.STORAGE_F32)
-                st = sv.open_stream(p)
-            except bmfr_amd.BmfrError as e:
-                if e.status != bmfr_amd.ERR_OUT_OF_BUDGET:
-                    raise
-                still.append(k)
-                continue
-            if now - sp["open"] > 0.005:
-                waited += 1
-            state[k]["stream"] = st
-            state[k]["scene"] = bmfr_amd.Scene(seed + 100 + k, SPHERES, 1e-3, sp["w"], sp["h"])
-            # frames that arrived while the stream waited for memory are dropped
-            while state[k]["next"] < len(sp["arrivals"]) and sp["arrivals"][state[k]["next"]] < now:
-                state[k]["next"] += 1
-            admitted += 1
-        waiting = still
-        # arrivals
-        for k in range(n):
-            s_ = state[k]
-            if s_["stream"] is None:
-                continue
-            arr = specs[k]["arrivals"]
-            while s_["next"] < len(arr) and arr[s_["next"]] <= now:
-                s_["pending"].append(arr[s_["next"]])
-                s_["next"] += 1
-        # completions
-        keep = []
-        for (sid, members) in in_flight:
-            done, ms = sv.query(sid)
-            if done:
-                tnow = time.perf_counter() - t0
-                step_ms.append(ms)
-                for (k, t_arr, px) in members:
-                    if t_arr >= settle:
-                        latencies.append(tnow - t_arr)
-                    frames_done += 1
-                    pixels_done += px
-            else:
-                keep.append((sid, members))
-        in_flight = keep
-        # closes (after the stream's in-flight work, which close waits for)
-        for k in range(n):
-            s_ = state[k]
-            if s_["stream"] is not None and now >= specs[k]["close"] and not s_["pending"] and \
-                    not any(k == m[0] for (_, ms_) in in_flight for m in ms_):
-                s_["stream"].close()
-                s_["scene"].close()
-                s_["stream"] = None
-                s_["done"] = True
-        # launch a step when fewer than two are in flight
-        if len(in_flight) < 2:
-            members = [k for k in range(n) if state[k]["stream"] is not None and state[k]["pending"]]
-            members = members[:64]
-            if members:
-                sts, frs, mats, offs, info = [], [], [], [], []
-                for k in members:
-                    s_ = state[k]
-                    t_arr = s_["pending"].pop(0)
-                    fr = s_["frame"]
-                    s_["frame"] += 1
-                    s_["scene"].render_to_stream(fr, s_["stream"])
-                    vp, off = s_["scene"].frame_args(fr)
-                    sts.append(s_["stream"])
-                    frs.append(fr)
-                    mats.append(vp)
-                    offs.append(off)
-                    info.append((k, t_arr, specs[k]["w"] * specs[k]["h"]))
-                sid = sv.step(sts, frs, np.concatenate(mats), np.concatenate(offs))
-                in_flight.append((sid, info))
-                step_sizes.append(len(members))
-        if now >= duration and not in_flight and oi >= n:
-            break
-        if now >= duration + 5.0:
-            break
-    wall = time.perf_counter() - t0
-    sv.synchronize()
-    for s_ in state:
-        if s_["stream"] is not None:
-            s_["stream"].close()
-            s_["scene"].close()
-    sv.close()
-    lat = np.array(latencies) * 1e3
-    return {
-        "streams": n, "duration_s": round(wall, 2), "budget_gib": budget_gib,
-        "admitted": admitted, "admitted_after_waiting": waited,
-        "frames": frames_done, "aggregate_frames_per_s": round(frames_done / wall, 1),
-        "mpixel_per_s": round(pixels_done / wall / 1e6, 1),
-        "latency_ms_p50": round(float(np.percentile(lat, 50)), 3) if len(lat) else None,
-        "latency_ms_p99": round(float(np.percentile(lat, 99)), 3) if len(lat) else None,
-        "latency_ms_max": round(float(lat.max()), 3) if len(lat) else None,
-        "steps": len(step_sizes), "mean_streams_per_step": round(float(np.mean(step_sizes)), 2),
-        "step_gpu_ms_mean": round(float(np.mean(step_ms)), 4) if step_ms else None,
-    }
 
 
 def measured_traffic(kernel):
@@ -509,6 +385,44 @@ def cpu_baseline(frames=8):
                       f"1280x720 scene; mean of frames 1..{frames - 1}"}
 
 
+def reference_kernels_same_gpu(frames=10):
+    """The reference's own opencl/bmfr.cl, compiled unmodified for gfx950
+    (oracle/_ref, built where the reference tree is present), run by an
+    OpenCL host with the reference's buffers and bindings on this GPU through
+    the ROCm OpenCL runtime: the same 1280x720 frames (24 spheres, FP16
+    tmp_data), the sum of its five kernels' event times averaged over frames
+    1..N-1 (opencl/bmfr.cpp:488-517).  A baseline beside the CPU one, never
+    the measured path."""
+    from oracle import ref
+    if not ref.available():
+        return {"ms_per_frame": None, "note": "oracle/_ref not built (the reference tree was absent at build)"}
+    try:
+        if ref.load().ref_device_count() < 1:
+            return {"ms_per_frame": None, "note": "no OpenCL GPU device visible"}
+        import bmfr_amd
+        sc = bmfr_amd.Scene(SEED, SPHERES, 1e-3, W, H)
+        r = ref.RefRun(W, H, True)
+        per = []
+        for f in range(frames):
+            g = sc.render_host(f)
+            r.upload(g["albedo"], g["normals"], g["positions"], g["noisy"])
+            vp, off = sc.frame_args(f)
+            for i in range(5):
+                r.stage(i, f, vp, off)
+            if f > 0:
+                per.append([r.stage_ms(i) for i in range(5)])
+            r.end_frame()
+        r.close()
+        sc.close()
+    except (OSError, RuntimeError) as e:
+        return {"ms_per_frame": None, "note": f"reference run unavailable: {e}"}
+    per = np.array(per).mean(axis=0)
+    return {"ms_per_frame": round(float(per.sum()), 4), "unit": "ms/frame",
+            "stage_ms": [round(float(x), 4) for x in per], "kind": "reference",
+            "sample": f"opencl/bmfr.cl on this MI355X (ROCm OpenCL), {frames} frames 1280x720, "
+                      "sum of its five kernels' event times, frames 1..N-1"}
+
+
 def progress(msg):
     """One line per finished phase on stderr (the JSON line stays alone on stdout)."""
     print(f"bench.py: {msg}", file=sys.stderr, flush=True)
@@ -528,7 +442,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=30.0)
     ap.add_argument("--serve-budget-gib", type=float, default=4.0)
     ap.add_argument("--no-served", action="store_true")
-    ap.add_argument("--serve-sweep", default="1,16,64,128",
+    ap.add_argument("--serve-sweep", default="1,16,64,128,192,256,512,1024",
                     help="stream counts of the latency-vs-streams sweep (empty: skip)")
     ap.add_argument("--serve-sweep-seconds", type=float, default=5.0)
     ap.add_argument("--precision-frames", type=int, default=60)
@@ -541,7 +455,7 @@ def main():
               "exchange)", file=sys.stderr)
         sys.exit(2)
 
-    totals, t, wall, s = run_plain(args)
+    totals, t, wall, s, fitter_b2b_ms = run_plain(args)
     ms_frame = float(totals.mean())
     progress(f"plain 1280x720: {ms_frame:.4f} ms/frame over {len(totals)} frames")
     P, Q, nB = s.pixels, s.padded_pixels, s.blocks
@@ -555,7 +469,14 @@ def main():
     roof = {"kernel": dom, "bound": "hbm", "achieved": round(stage_gbs[dom], 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(stage_gbs[dom] / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(dom), "algorithmic_bytes": int(sb[dom]),
-            "stage_frac": {k: round(v / HBM_PEAK_GBS, 4) for k, v in stage_gbs.items()}}
+            "stage_frac": {k: round(v / HBM_PEAK_GBS, 4) for k, v in stage_gbs.items()},
+            # What actually binds the fitter (one wave per 32x32 block, ~1 wave
+            # per SIMD at 720p): its dependent VALU instruction chain, not HBM.
+            "binding_resource": "valu issue / latency (see issue)",
+            # the same bytes over the kernel's time launched back to back (no
+            # dispatch gap), the figure comparable to a kernel trace's duration
+            "kernel_ms_back_to_back": round(fitter_b2b_ms, 5),
+            "frac_back_to_back": round(sb[dom] / (fitter_b2b_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     # The fitter's own limit is instruction issue: its code is straight-line,
     # FITTER_VALU_PER_WAVE instructions per wave (one wave per block, the
     # launch padded to whole work-groups per XCD), against one wave64 VALU
@@ -601,10 +522,14 @@ def main():
         sweep = run_serve_sweep([int(x) for x in args.serve_sweep.split(",")], args.serve_sweep_seconds)
     copy_gbs = copy_peak()
     roof["copy_peak_gbs"] = round(copy_gbs, 1)
-    cpu = None
+    cpu, reference_gpu = None, None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline()
         progress(f"cpu baseline: {cpu['value']:.1f} ms/frame on {cpu['cores']} threads")
+        reference_gpu = reference_kernels_same_gpu()
+        if reference_gpu.get("ms_per_frame"):
+            reference_gpu["ratio_to_this_chain"] = round(reference_gpu["ms_per_frame"] / ms_frame, 2)
+        progress(f"reference bmfr.cl on this GPU: {reference_gpu}")
     out = {
         "metric": "720p ms/frame plain vs batched; p99 ms vs streams; fitter % LDS/HBM peak",
         "value": round(ms_frame, 4),
@@ -633,6 +558,7 @@ def main():
         "precision": precision,
         "feature_sets": feature_sets,
         "cpu_baseline": cpu,
+        "reference_gpu": reference_gpu,
     }
     print(json.dumps(out), flush=True)
 

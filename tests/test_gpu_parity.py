@@ -102,9 +102,13 @@ def test_fitter_matches_oracle_and_fp64(gpu, cfg, tmp_f16):
         fe, we = _fitted_error(ls.o.p, tmp0, fr, s.blocks, Wg, Wo)
         # FP32: both QRs are FP32 Householder with different summation
         # orders; FP16 tmp rounds every update to binary16 on both sides, so
-        # one differently-rounded element shifts later columns by ~1e-3.
-        ftol = 3e-2 if tmp_f16 else 1e-4
+        # one differently-rounded element shifts later columns by ~1e-3
+        # (measured: 14-18 of 81 blocks above 1e-4, at most 1.06e-3; each
+        # explained by _fp16_outliers_are_rounding_flips).
+        ftol = 2e-3 if tmp_f16 else 1e-4
         assert fe < ftol, (fe, we)
+        if tmp_f16:
+            _fp16_outliers_are_rounding_flips(ls, tmp0, fr, Wg, Wo, cfg)
         # against FP64 least squares on the identical (scaled, noised) matrix
         Wd = np.stack([np.linalg.lstsq(*bind.fitter_inputs(ls.o.p, tmp0, b, fr)[:2], rcond=None)[0]
                        for b in range(s.blocks)])
@@ -113,6 +117,44 @@ def test_fitter_matches_oracle_and_fp64(gpu, cfg, tmp_f16):
         assert fe64 < max(2.0 * fo64, 1e-4), (fe64, fo64)
     finally:
         ls.close()
+
+
+def _fp16_outliers_are_rounding_flips(ls, tmp0, fr, Wg, Wo, cfg):
+    """The blocks whose FP16-tmp fitted colours differ from the oracle's by
+    more than the FP32 bar: the same block matrices (binary16 values) fitted
+    by both sides with FP32 updates instead of binary16-rounded ones agree
+    within the FP32 bar, so the difference comes from the binary16 rounding
+    of the updates (an element landing on the other side of a rounding
+    boundary after a differently ordered sum), not from the method."""
+    s = ls.o.s
+    per = []
+    for blk in range(s.blocks):
+        A, _, _ = bind.fitter_inputs(ls.o.p, tmp0, blk, fr)
+        fw = A @ Wo[blk]
+        per.append(float(np.abs(A @ Wg[blk] - fw).max() / (np.abs(fw).max() + 1e-3)))
+    per = np.array(per)
+    out = np.nonzero(per > 1e-4)[0]
+    print(f"FP16 tmp fitter, cfg {cfg}: {len(out)} of {s.blocks} blocks above 1e-4 (max {per.max():.2e})")
+    if not len(out):
+        return
+    w, h = ls.w, ls.h
+    g32 = bmfr_amd.Stream(bmfr_amd.default_params(w, h, tmp_storage=bmfr_amd.STORAGE_F32))
+    try:
+        g32.set_fitter_config(*cfg)
+        g32.write_buffer("tmp_data", tmp0.astype(np.float32))
+        g32.fitter(fr)
+        W32 = g32.read_buffer("weights").reshape(s.blocks, 10, 3).astype(np.float64)
+    finally:
+        g32.close()
+    o32 = bind.OracleStream(bind.params(w, h, half_tmp=False))
+    o32.tmp[:] = tmp0.astype(np.float32)
+    o32.fitter(fr)
+    Wo32 = o32.weights.reshape(s.blocks, 10, 3).astype(np.float64)
+    for blk in out:
+        A, _, _ = bind.fitter_inputs(o32.p, o32.tmp, int(blk), fr)
+        fw = A @ Wo32[blk]
+        e32 = float(np.abs(A @ W32[blk] - fw).max() / (np.abs(fw).max() + 1e-3))
+        assert e32 < 1e-4, (int(blk), per[blk], e32)
 
 
 def _run_chain(ls, frames):
@@ -182,11 +224,18 @@ def test_end_to_end_final_frame(gpu, w, h, frames, tmp_f16):
         finite = np.isfinite(res_o) & np.isfinite(res_g)
         assert np.array_equal(np.isfinite(res_o), np.isfinite(res_g))
         err = rel_err(res_g[finite], res_o[finite])
+        print(f"end to end {w}x{h} tmp_f16={tmp_f16}: median {np.median(err):.2e}, p99.9 "
+              f"{np.percentile(err, 99.9):.2e}, p99.99 {np.percentile(err, 99.99):.2e}, max {err.max():.2e}")
         if tmp_f16:
             # binary16 working matrix: a few pixels of a few blocks move by
-            # more than 1e-3 when one update rounds the other way
-            assert np.percentile(err, 99.9) < 1e-2 and np.median(err) < 1e-4, \
+            # more than 1e-3 when one update rounds the other way (the fitter
+            # test shows those blocks agree within 1e-4 without the binary16
+            # rounding of updates); measured at 256x256: median 2.3e-6, p99.9
+            # 3.9e-3, p99.99 2.0e-2, max 7.4e-2 (a dark pixel), the FP32 run
+            # of the same frames max 7.2e-4
+            assert np.median(err) < 1e-4 and np.percentile(err, 99.9) < 1e-2, \
                 (float(np.median(err)), float(err.max()))
+            assert np.percentile(err, 99.99) < 3e-2 and err.max() < 0.1, float(err.max())
         else:
             assert err.max() < 1e-3, float(err.max())
         # PSNR of each against the noise-free render of the last frame

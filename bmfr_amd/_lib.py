@@ -113,6 +113,8 @@ SIGNATURES = [
     ("bmfr_weighted_sum", c_int32, [ctypes.c_void_p, c_int32]),
     ("bmfr_fitter_fp64_weights", c_int32, [ctypes.c_void_p, c_float_p]),
     ("bmfr_refactor_count", c_int32, [ctypes.c_void_p, ctypes.POINTER(c_int32)]),
+    ("bmfr_fit_blocks", c_int32, [ctypes.c_void_p, c_int32, c_int32, c_int32, ctypes.c_double, c_int32,
+                                  ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     ("bmfr_accumulate_filtered_data", c_int32, [ctypes.c_void_p, c_int32]),
     ("bmfr_taa", c_int32, [ctypes.c_void_p, c_int32]),
     ("bmfr_end_frame", c_int32, [ctypes.c_void_p]),

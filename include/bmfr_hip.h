@@ -256,6 +256,21 @@ int32_t bmfr_weighted_sum(bmfr_stream stream, int32_t frame_number);
  * the same scaled and noised matrix (W = float[30 nB]); for stating the
  * fitter's weight error against FP64.  No counterpart in the reference. */
 int32_t bmfr_fitter_fp64_weights(bmfr_stream stream, float *weights_out);
+/* The fitter over caller-owned buffers, allocating nothing.  No counterpart
+ * in the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Fits `blocks` 32x32 blocks of tmp_data (block-major, the
+ * reference's 13 columns, float or half per tmp_storage, 16-byte aligned)
+ * for frame `frame_number` exactly as bmfr_fitter does (same kernels, same
+ * bits) and writes weights (float[30 blocks]) and mins_maxs (float[12
+ * blocks]); enqueued on `hip_stream` (a hipStream_t; null: the null stream).
+ * Its scratch (the frame's noise table and the FP64 re-factorisation flags,
+ * 100 KB) is a fixed device array each GPU holds until the process ends:
+ * at most 16384 blocks per call (larger is BMFR_ERR_INVALID_ARGUMENT), and
+ * calls that overlap on one GPU race on it. */
+int32_t bmfr_fit_blocks(const void *tmp_data, int32_t tmp_storage, int32_t blocks, int32_t frame_number,
+                        double noise_amount, int32_t qr_mode, float refactor_threshold, float *weights,
+                        float *mins_maxs, void *hip_stream);
 /* Blocks the last BMFR_QR_KAHAN_FP64 fitter call re-factorised in FP64. */
 int32_t bmfr_refactor_count(bmfr_stream stream, int32_t *count);
 int32_t bmfr_accumulate_filtered_data(bmfr_stream stream, int32_t frame_number);

@@ -210,6 +210,8 @@ struct bmfr_stream_s {
     std::deque<ServerFrame> queue;
     int frames_in_flight = 0;
     bool closing = false;
+    int64_t last_step_id = 0;   // the last server step the stream took part in
+    int last_lane = 0;          // and the server lane it ran on
 
     int cur(int a_slot) const { return swapped ? a_slot : a_slot + 1; }
     int prev(int a_slot) const { return swapped ? a_slot + 1 : a_slot; }
@@ -1520,22 +1522,28 @@ struct bmfr_server_s {
     int device = 0;
     int64_t budget = 0, used = 0;   // used counts admissions in progress too
     std::vector<bmfr_stream> open;
-    hipStream_t stream = nullptr;
-    // device tables: one slot per launch group (storage F32/F16 x tmp
-    // F32/F16 x qr mode), rewritten by every step on `stream` (in order)
+    // Two HIP streams ("lanes"): a step goes to the lane its members' last
+    // steps ran on (ordered there for free) or, with no such member, to the
+    // other lane than the last step's, so that consecutive steps of disjoint
+    // streams overlap on the GPU (one's VALU-bound fitter beside the other's
+    // streaming stages).  Steps that share a stream are ordered through that
+    // stream's own HIP stream whichever lanes they run on.
+    hipStream_t lane[2]{};
+    int last_lane = 0;
+    // per-step records, slot = step id % kRing: pinned staging, device tables
+    // (one set per launch group: storage F32/F16 x tmp F32/F16 x qr mode)
+    // and events; a slot is reused once its previous step has ended
     static constexpr int kGroups = 12;
-    bmfr::BatchStream *d_streams = nullptr;  // [kGroups][kMaxBatch]
-    bmfr::BatchStep *d_steps = nullptr;      // [kGroups][kMaxBatch]
-    int32_t *d_starts = nullptr;             // [kGroups][kStartKinds][kMaxBatch+1]
-    // per-step records: pinned staging and events, slot = step id % kRing
     static constexpr int kRing = 8;
+    bmfr::BatchStream *d_streams = nullptr;  // [kRing][kGroups][kMaxBatch]
+    bmfr::BatchStep *d_steps = nullptr;      // [kRing][kGroups][kMaxBatch]
+    int32_t *d_starts = nullptr;             // [kRing][kGroups][kStartKinds][kMaxBatch+1]
     struct Staging {
         bmfr::BatchStream streams[kGroups][bmfr::kMaxBatch];
         bmfr::BatchStep steps[kGroups][bmfr::kMaxBatch];
         int32_t starts[kGroups][bmfr::kStartKinds][bmfr::kMaxBatch + 1];
     };
     Staging *h_stage = nullptr;              // [kRing], pinned
-    hipEvent_t ring_ev[kRing]{};
     hipEvent_t step_start[kRing]{}, step_end[kRing]{};
     int64_t ring_id[kRing]{};                // step id each slot's record belongs to (0: none)
     int64_t next_step = 1;
@@ -1573,9 +1581,20 @@ int32_t launch_step_locked(bmfr_server sv, const std::vector<StepMember> &m, int
     HIP_TRY(hipSetDevice(sv->device));
     const int64_t id = sv->next_step++;
     const int slot = (int)(id % bmfr_server_s::kRing);
-    HIP_TRY(hipEventSynchronize(sv->ring_ev[slot]));  // staging slot free again
+    HIP_TRY(hipEventSynchronize(sv->step_end[slot]));  // the slot's previous step has ended
     sv->ring_id[slot] = id;
     bmfr_server_s::Staging &hs = sv->h_stage[slot];
+    // the lane: where most members' recent steps ran, else the other one
+    int score[2] = {0, 0};
+    for (const StepMember &mb : m)
+        if (mb.s->last_step_id > 0 && mb.s->last_step_id + bmfr_server_s::kRing > id) ++score[mb.s->last_lane];
+    const int ln = score[0] || score[1] ? (score[1] > score[0] ? 1 : 0) : 1 - sv->last_lane;
+    sv->last_lane = ln;
+    const hipStream_t ls = sv->lane[ln];
+    const size_t G = bmfr_server_s::kGroups, M = bmfr::kMaxBatch;
+    bmfr::BatchStream *d_streams = sv->d_streams + (size_t)slot * G * M;
+    bmfr::BatchStep *d_steps = sv->d_steps + (size_t)slot * G * M;
+    int32_t *d_starts = sv->d_starts + (size_t)slot * G * bmfr::kStartKinds * (M + 1);
     int gcount[bmfr_server_s::kGroups] = {};
     for (const StepMember &mb : m) {
         bmfr_stream s = mb.s;
@@ -1591,28 +1610,26 @@ int32_t launch_step_locked(bmfr_server sv, const std::vector<StepMember> &m, int
         sts[bmfr::kStartsBlocks][k + 1] = sts[bmfr::kStartsBlocks][k] + s->sz.blocks;
         // order after this stream's pending work and its producer's
         HIP_TRY(hipEventRecord(s->sync_ev, s->stream));
-        HIP_TRY(hipStreamWaitEvent(sv->stream, s->sync_ev, 0));
-        if (mb.producer) HIP_TRY(hipStreamWaitEvent(sv->stream, mb.producer, 0));
+        HIP_TRY(hipStreamWaitEvent(ls, s->sync_ev, 0));
+        if (mb.producer) HIP_TRY(hipStreamWaitEvent(ls, mb.producer, 0));
     }
-    const size_t M = bmfr::kMaxBatch;
     for (int g = 0; g < bmfr_server_s::kGroups; ++g) {
         if (!gcount[g]) continue;
-        HIP_TRY(hipMemcpyAsync(sv->d_streams + g * M, hs.streams[g], sizeof(bmfr::BatchStream) * gcount[g],
-                               hipMemcpyHostToDevice, sv->stream));
-        HIP_TRY(hipMemcpyAsync(sv->d_steps + g * M, hs.steps[g], sizeof(bmfr::BatchStep) * gcount[g],
-                               hipMemcpyHostToDevice, sv->stream));
-        HIP_TRY(hipMemcpyAsync(sv->d_starts + g * bmfr::kStartKinds * (M + 1), hs.starts[g],
-                               sizeof(hs.starts[g]), hipMemcpyHostToDevice, sv->stream));
+        HIP_TRY(hipMemcpyAsync(d_streams + g * M, hs.streams[g], sizeof(bmfr::BatchStream) * gcount[g],
+                               hipMemcpyHostToDevice, ls));
+        HIP_TRY(hipMemcpyAsync(d_steps + g * M, hs.steps[g], sizeof(bmfr::BatchStep) * gcount[g],
+                               hipMemcpyHostToDevice, ls));
+        HIP_TRY(hipMemcpyAsync(d_starts + g * bmfr::kStartKinds * (M + 1), hs.starts[g], sizeof(hs.starts[g]),
+                               hipMemcpyHostToDevice, ls));
     }
-    HIP_TRY(hipEventRecord(sv->ring_ev[slot], sv->stream));
-    HIP_TRY(hipEventRecord(sv->step_start[slot], sv->stream));
+    HIP_TRY(hipEventRecord(sv->step_start[slot], ls));
     for (int st = 0; st < kStages; ++st) {
         for (int g = 0; g < bmfr_server_s::kGroups; ++g) {
             if (!gcount[g]) continue;
             bmfr::BatchLaunch L;
-            L.streams = sv->d_streams + g * M;
-            L.steps = sv->d_steps + g * M;
-            L.starts = sv->d_starts + g * bmfr::kStartKinds * (M + 1);
+            L.streams = d_streams + g * M;
+            L.steps = d_steps + g * M;
+            L.starts = d_starts + g * bmfr::kStartKinds * (M + 1);
             L.n = gcount[g];
             const int sg = g / 3;
             const bool g16 = sg >= 2, t16 = sg & 1;
@@ -1620,30 +1637,30 @@ int32_t launch_step_locked(bmfr_server sv, const std::vector<StepMember> &m, int
             const int32_t(&sts)[bmfr::kStartKinds][bmfr::kMaxBatch + 1] = hs.starts[g];
             hipError_t e;
             switch (st) {
-                case 0: e = bmfr::launch_batch_accumulate_noisy_data(L, sts[0][L.n], g16, t16, sv->stream); break;
-                case 1: e = bmfr::launch_batch_fitter(L, sts[2][L.n], t16, sv->fitter, sv->grid_cap, qr, sv->stream); break;
+                case 0: e = bmfr::launch_batch_accumulate_noisy_data(L, sts[0][L.n], g16, t16, ls); break;
+                case 1: e = bmfr::launch_batch_fitter(L, sts[2][L.n], t16, sv->fitter, sv->grid_cap, qr, ls); break;
                 case 2: e = hipSuccess; break;  // fused into stage 4
                 case 3:
                     e = sv->fuse == BMFR_FUSE_345
-                            ? bmfr::launch_batch_filter_taa(L, sts[bmfr::kStartsBlocks][L.n], g16, sv->stream)
-                            : bmfr::launch_batch_weighted_sum_filtered(L, sts[bmfr::kStartsPadded][L.n], g16,
-                                                                       sv->stream);
+                            ? bmfr::launch_batch_filter_taa(L, sts[bmfr::kStartsBlocks][L.n], g16, ls)
+                            : bmfr::launch_batch_weighted_sum_filtered(L, sts[bmfr::kStartsPadded][L.n], g16, ls);
                     break;
                 default:
-                    e = sv->fuse == BMFR_FUSE_345
-                            ? hipSuccess
-                            : bmfr::launch_batch_taa(L, sts[bmfr::kStartsImage][L.n], g16, sv->stream);
+                    e = sv->fuse == BMFR_FUSE_345 ? hipSuccess
+                                                  : bmfr::launch_batch_taa(L, sts[bmfr::kStartsImage][L.n], g16, ls);
                     break;
             }
             if (e != hipSuccess) return fail(BMFR_ERR_HIP, "server stage %d launch failed: %s", st, hipGetErrorString(e));
         }
     }
-    HIP_TRY(hipEventRecord(sv->step_end[slot], sv->stream));
+    HIP_TRY(hipEventRecord(sv->step_end[slot], ls));
     for (const StepMember &mb : m) {
         bmfr_stream s = mb.s;
         HIP_TRY(hipStreamWaitEvent(s->stream, sv->step_end[slot], 0));
         s->last_result = s->cur(S_RESULT_A);
         s->noise_frame = mb.frame;
+        s->last_step_id = id;
+        s->last_lane = ln;
         s->finish_frame();
     }
     if (step_id) *step_id = id;
@@ -1833,16 +1850,15 @@ int32_t bmfr_server_create(int64_t hbm_budget_bytes, bmfr_server *out) {
     if (!sv) return fail(BMFR_ERR_INVALID_ARGUMENT, "host allocation failed");
     sv->budget = hbm_budget_bytes;
     hipError_t e = hipGetDevice(&sv->device);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&sv->stream, hipStreamNonBlocking);
-    const size_t G = bmfr_server_s::kGroups, M = bmfr::kMaxBatch;
-    if (e == hipSuccess) e = hipMalloc((void **)&sv->d_streams, sizeof(bmfr::BatchStream) * G * M);
-    if (e == hipSuccess) e = hipMalloc((void **)&sv->d_steps, sizeof(bmfr::BatchStep) * G * M);
-    if (e == hipSuccess) e = hipMalloc((void **)&sv->d_starts, sizeof(int32_t) * G * bmfr::kStartKinds * (M + 1));
+    for (int l = 0; l < 2 && e == hipSuccess; ++l) e = hipStreamCreateWithFlags(&sv->lane[l], hipStreamNonBlocking);
+    const size_t G = bmfr_server_s::kGroups, M = bmfr::kMaxBatch, R = bmfr_server_s::kRing;
+    if (e == hipSuccess) e = hipMalloc((void **)&sv->d_streams, sizeof(bmfr::BatchStream) * R * G * M);
+    if (e == hipSuccess) e = hipMalloc((void **)&sv->d_steps, sizeof(bmfr::BatchStep) * R * G * M);
     if (e == hipSuccess)
-        e = hipHostMalloc((void **)&sv->h_stage, sizeof(bmfr_server_s::Staging) * bmfr_server_s::kRing);
+        e = hipMalloc((void **)&sv->d_starts, sizeof(int32_t) * R * G * bmfr::kStartKinds * (M + 1));
+    if (e == hipSuccess) e = hipHostMalloc((void **)&sv->h_stage, sizeof(bmfr_server_s::Staging) * R);
     for (int i = 0; i < bmfr_server_s::kRing && e == hipSuccess; ++i) {
-        e = hipEventCreateWithFlags(&sv->ring_ev[i], hipEventDisableTiming);
-        if (e == hipSuccess) e = hipEventCreate(&sv->step_start[i]);
+        e = hipEventCreate(&sv->step_start[i]);
         if (e == hipSuccess) e = hipEventCreate(&sv->step_end[i]);
     }
     if (e != hipSuccess) {
@@ -1862,7 +1878,8 @@ int32_t bmfr_server_destroy(bmfr_server sv) {
     sv->cv_work.notify_all();
     if (sv->dispatcher.joinable()) sv->dispatcher.join();
     (void)hipSetDevice(sv->device);
-    if (sv->stream) (void)hipStreamSynchronize(sv->stream);
+    for (hipStream_t l : sv->lane)
+        if (l) (void)hipStreamSynchronize(l);
     for (bmfr_stream s : sv->open) {
         for (ServerFrame &f : s->queue)
             if (f.producer) sv->event_pool.push_back(f.producer);
@@ -1872,7 +1889,6 @@ int32_t bmfr_server_destroy(bmfr_server sv) {
     sv->open.clear();
     for (hipEvent_t e : sv->event_pool) (void)hipEventDestroy(e);
     for (int i = 0; i < bmfr_server_s::kRing; ++i) {
-        if (sv->ring_ev[i]) (void)hipEventDestroy(sv->ring_ev[i]);
         if (sv->step_start[i]) (void)hipEventDestroy(sv->step_start[i]);
         if (sv->step_end[i]) (void)hipEventDestroy(sv->step_end[i]);
     }
@@ -1880,7 +1896,8 @@ int32_t bmfr_server_destroy(bmfr_server sv) {
     if (sv->d_streams) (void)hipFree(sv->d_streams);
     if (sv->d_steps) (void)hipFree(sv->d_steps);
     if (sv->d_starts) (void)hipFree(sv->d_starts);
-    if (sv->stream) (void)hipStreamDestroy(sv->stream);
+    for (hipStream_t l : sv->lane)
+        if (l) (void)hipStreamDestroy(l);
     delete sv;
     return BMFR_OK;
 }
@@ -2113,7 +2130,7 @@ int32_t bmfr_server_synchronize(bmfr_server sv) {
         sv->cv_drained.wait(lk, [&] { return sv->queued_frames == 0; });
     }
     HIP_TRY(hipSetDevice(sv->device));
-    HIP_TRY(hipStreamSynchronize(sv->stream));
+    for (hipStream_t l : sv->lane) HIP_TRY(hipStreamSynchronize(l));
     return BMFR_OK;
 }
 

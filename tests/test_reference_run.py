@@ -11,9 +11,12 @@ FMAs (OpenCL's default FP_CONTRACT ON, dot() as an FMA chain) and lowers
 every stage with separately rounded operations and IEEE division (bit-exact
 against the CPU oracle).  So:
   frame 0, stage 1          bit-exact (no reprojection arithmetic involved)
-  integer decisions         accept bits and spp: mismatch fraction <= 1e-4
-                            (only pixels whose floor/threshold sits within a
-                            few ulp of a boundary can differ)
+  integer decisions         accept bits and spp: every mismatch shown to be a
+                            boundary case (the FP64 value of the floor
+                            argument, tap distance or weighted spp within
+                            2^-17 of the integer, limit or rounding midpoint;
+                            _explain_stage1_mismatches), and at most 1e-4 of
+                            the decisions
   floats, stage by stage    each stage fed the reference's own inputs; the
                             per-stage bounds below
   end to end                final frame of 8 frames, per-pixel relative
@@ -43,6 +46,84 @@ def _ref_ready(w, h, half, feature_set=None):
         pytest.skip("no OpenCL GPU device")
 
 
+# Boundary tolerances for explaining an integer decision that differs from
+# the reference's (its build contracts a*b+c into FMAs and divides with a
+# relaxed reciprocal; this library rounds every operation): the FP64 value
+# of the quantity the decision is taken on lies within these relative
+# distances of the boundary it is compared with.
+FLOOR_TOL = 2.0 ** -17     # floor() argument from the nearest integer (x max(|v|, 1))
+THRESH_TOL = 2.0 ** -18    # squared distance from POSITION/NORMAL_LIMIT_SQUARED
+SPP_TOL = 2.0 ** -17       # sample spp from the rounding midpoint k + 1/2 (x max(|s|, 1))
+
+
+def _explain_stage1_mismatches(w, h, f, prev, vp, off, ga, ra, gs, rs, gp, rp, prev_spp, pls, nls):
+    """Every pixel whose accept bits or spp differ from the reference's after
+    stage 1 (bmfr.cl:336-442), classified by the decision that flipped, with
+    that decision's quantity recomputed in FP64 from the same float inputs:
+      floor    the reprojected position's floor differs (the integer lies
+               between the two float values, and the FP64 value is within
+               FLOOR_TOL of it);
+      tap      same floor, a tap's acceptance differs: its position or
+               normal squared distance is within THRESH_TOL of the limit;
+      spp      same taps: the weighted previous spp is within SPP_TOL of a
+               rounding midpoint (convert_uchar_sat_rte).
+    Returns (counts per class, list of unexplained pixels)."""
+    M = np.asarray(vp, np.float64)
+    pos = f["positions"].reshape(-1, 3).astype(np.float64)
+    nrm = f["normals"].reshape(-1, 3).astype(np.float64)
+    ppos = prev["positions"].reshape(-1, 3).astype(np.float64)
+    pnrm = prev["normals"].reshape(-1, 3).astype(np.float64)
+    counts = {"floor": 0, "tap": 0, "spp": 0}
+    bad = []
+    for lp in np.nonzero((ga != ra) | (gs != rs))[0]:
+        p4 = np.array([*pos[lp], 1.0])
+        wdiv = M[3::4] @ p4
+        vx = ((M[0::4] @ p4) / wdiv + 1.0) / 2.0 * w - float(off[0])
+        vy = ((M[1::4] @ p4) / wdiv + 1.0) / 2.0 * h - (1.0 - float(off[1]))
+        g2, r2 = gp[2 * lp:2 * lp + 2], rp[2 * lp:2 * lp + 2]
+        fg, fr = np.floor(g2), np.floor(r2)
+        if (fg != fr).any():
+            ok = True
+            for v, a, b in ((vx, g2[0], r2[0]), (vy, g2[1], r2[1])):
+                if np.floor(a) != np.floor(b):
+                    k = max(np.floor(a), np.floor(b))
+                    ok &= abs(v - k) <= FLOOR_TOL * max(abs(v), 1.0) and min(a, b) < k <= max(a, b)
+            counts["floor"] += ok
+            if not ok:
+                bad.append(("floor", int(lp), vx, vy, g2.tolist(), r2.tolist()))
+            continue
+        ix, iy = int(fg[0]), int(fg[1])
+        if ga[lp] != ra[lp]:
+            ok = True
+            for i in range(4):
+                if not ((int(ga[lp]) ^ int(ra[lp])) >> i) & 1:
+                    continue
+                sx, sy = ix + (i & 1), iy + (i >> 1)
+                ls = sy * w + sx
+                dp = float(((ppos[ls] - pos[lp]) ** 2).sum())
+                dn = float(((pnrm[ls] - nrm[lp]) ** 2).sum())
+                ok &= abs(dp - pls) <= THRESH_TOL * pls or (dp < pls and abs(dn - nls) <= THRESH_TOL * nls)
+            counts["tap"] += ok
+            if not ok:
+                bad.append(("tap", int(lp), int(ga[lp]), int(ra[lp])))
+            continue
+        # same accept bits, spp differs: the rounding of the weighted spp
+        fx, fy = float(g2[0]) - ix, float(g2[1]) - iy
+        wts = [(1 - fx) * (1 - fy), fx * (1 - fy), (1 - fx) * fy, fx * fy]
+        num = den = 0.0
+        for i in range(4):
+            if (int(ga[lp]) >> i) & 1:
+                ls = (iy + (i >> 1)) * w + ix + (i & 1)
+                num += wts[i] * float(prev_spp[ls])
+                den += wts[i]
+        sv = num / den if den > 0 else 0.0
+        ok = abs((sv - np.floor(sv)) - 0.5) <= SPP_TOL * max(abs(sv), 1.0)
+        counts["spp"] += ok
+        if not ok:
+            bad.append(("spp", int(lp), sv, int(gs[lp]), int(rs[lp])))
+    return counts, bad
+
+
 # The reference's set at three sizes, and two configured sets (its
 # FEATURE_BUFFERS strings in oracle/Makefile; 5 and 16 features).
 @pytest.mark.parametrize("w,h,half,fs", [(96, 64, True, None), (256, 256, True, None), (256, 256, False, None),
@@ -62,7 +143,10 @@ def test_stage_by_stage_against_reference(gpu, w, h, half, fs):
     g = bmfr_amd.Stream(gparams)
     sc = scene(w, h)
     int_mismatch = 0
+    explained = {"floor": 0, "tap": 0, "spp": 0}
+    unexplained = []
     worst = {}
+    prev_f, prev_spp = None, None
     try:
         for fr in range(6):
             f = sc.render_host(fr)
@@ -82,6 +166,14 @@ def test_stage_by_stage_against_reference(gpu, w, h, half, fs):
                 assert np.array_equal(rn, gn)
                 assert np.array_equal(rt, g.read_buffer("tmp_data"))
             int_mismatch += int((ra != ga).sum() + (rs != gs).sum())
+            if fr > 0:
+                c, bad = _explain_stage1_mismatches(w, h, f, prev_f, vp, off, ga, ra, gs, rs, gp, rp, prev_spp,
+                                                    float(gparams.position_limit_squared),
+                                                    float(gparams.normal_limit_squared))
+                for k in c:
+                    explained[k] += c[k]
+                unexplained += bad
+            prev_f, prev_spp = f, rs.copy()
             # absolute: NDC error of a few ulp (the reference's relaxed
             # divide) scaled by W/2 or H/2, then a subtraction
             worst["prev_pixel"] = max(worst.get("prev_pixel", 0),
@@ -134,7 +226,8 @@ def test_stage_by_stage_against_reference(gpu, w, h, half, fs):
             r.end_frame()
             g.end_frame()
         print(f"{w}x{h} half={half} features={fs or 'reference'}: integer mismatches "
-              f"{int_mismatch}/{12 * P}, worst {worst}")
+              f"{int_mismatch}/{12 * P}, explained as boundary cases {explained}, worst {worst}")
+        assert not unexplained, unexplained[:5]
         assert int_mismatch <= 1e-4 * 12 * P
         assert worst["prev_pixel"] < 1e-6          # |dx| < 1e-6 * max(W, H)
         assert worst["acc"] < 5e-3
@@ -191,6 +284,10 @@ def test_end_to_end_against_reference(gpu, half):
         assert np.median(err) < 1e-4
         assert np.percentile(err, 99) < (2e-2 if half else 5e-3)
         assert np.percentile(err_in, 99.9) < (2e-2 if half else 5e-3)
+        if not half:
+            # FP32 working matrix: every race-free pixel within 1e-3 of the
+            # reference (measured max 2.4e-4 interior, 5.2e-4 whole image)
+            assert err_in.max() < 1e-3, float(err_in.max())
         assert d < 0.01
     finally:
         r.close()

@@ -62,18 +62,51 @@ __device__ __forceinline__ int mirror(int index, int size) {
     return index;
 }
 
+// Global-memory access through an explicitly global pointer.  Every buffer
+// the kernels touch is global device memory, but a pointer read from device
+// memory (the batched launches' per-stream tables) is a generic pointer to
+// the compiler, which then uses flat instructions: those also count against
+// the LDS counter, so every wait on an LDS access waits for the loads in
+// flight too, and block-uniform loads cannot become scalar loads.  The cast
+// states the address space, so the accesses compile to global (or scalar)
+// loads and stores whichever way the pointer was obtained.
+// Class types (__half, the HIP vector types) go through a builtin type of
+// their size: their constructors and assignment take generic references.
+typedef unsigned int gword4 __attribute__((ext_vector_type(4)));
+template <int N> struct GlobalRaw;
+template <> struct GlobalRaw<1> { typedef unsigned char type; };
+template <> struct GlobalRaw<2> { typedef unsigned short type; };
+template <> struct GlobalRaw<4> { typedef unsigned int type; };
+template <> struct GlobalRaw<8> { typedef unsigned long long type; };
+template <> struct GlobalRaw<16> { typedef gword4 type; };
+template <typename T>
+__device__ __forceinline__ T gload(const T *p, int64_t i) {
+    typedef typename GlobalRaw<sizeof(T)>::type U;
+    const U u = ((const __attribute__((address_space(1))) U *)p)[i];
+    T v;
+    __builtin_memcpy(&v, &u, sizeof(T));
+    return v;
+}
+template <typename T>
+__device__ __forceinline__ void gstore(T *p, int64_t i, T v) {
+    typedef typename GlobalRaw<sizeof(T)>::type U;
+    U u;
+    __builtin_memcpy(&u, &v, sizeof(T));
+    ((__attribute__((address_space(1))) U *)p)[i] = u;
+}
+
 // Storage-type access for G-buffers and histories (float or half).
 template <typename T> struct Store;
 template <> struct Store<float> {
-    __device__ __forceinline__ static float ld(const float *p, int64_t i) { return p[i]; }
-    __device__ __forceinline__ static void st(float *p, int64_t i, float v) { p[i] = v; }
+    __device__ __forceinline__ static float ld(const float *p, int64_t i) { return gload(p, i); }
+    __device__ __forceinline__ static void st(float *p, int64_t i, float v) { gstore(p, i, v); }
 };
 template <> struct Store<__half> {
     __device__ __forceinline__ static float ld(const __half *p, int64_t i) {
-        return __half2float(p[i]);
+        return __half2float(gload(p, i));
     }
     __device__ __forceinline__ static void st(__half *p, int64_t i, float v) {
-        p[i] = __float2half_rn(v);
+        gstore(p, i, __float2half_rn(v));
     }
 };
 

@@ -89,30 +89,29 @@ __device__ __forceinline__ Stage1Pixel stage1_pixel(const StageArgs &a, const St
             tpp[i] = load3(prev_positions, tap_ls[i]);
             tpn[i] = load3(prev_normals, tap_ls[i]);
             tpc[i] = load3(prev_noisy, tap_ls[i]);
-            tsp[i] = (float)prev_spp[tap_ls[i]];
+            tsp[i] = (float)gload(prev_spp, tap_ls[i]);
         }
+        // The tests are evaluated for every tap and the sums selected rather
+        // than branched over (the same values, in the reference's order):
+        // with a branch per tap the compiler moves that tap's loads into it,
+        // after a wait for every load in flight, one round trip per test.
         float total_weight = 0.f;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            if (tap_in[i]) {
-                const float3 pp = tpp[i];
-                const float dx = pp.x - wp.x, dy = pp.y - wp.y, dz = pp.z - wp.z;
-                const float pd = (dx * dx + dy * dy) + dz * dz;
-                if (pd < a.position_limit_squared) {
-                    const float3 pn = tpn[i];
-                    const float nx = pn.x - nrm.x, ny = pn.y - nrm.y, nz = pn.z - nrm.z;
-                    const float nd = (nx * nx + ny * ny) + nz * nz;
-                    if (nd < a.normal_limit_squared) {
-                        store_accept |= 1u << i;
-                        sample_spp = sample_spp + wts[i] * tsp[i];
-                        const float3 pc = tpc[i];
-                        prev_col.x = prev_col.x + wts[i] * pc.x;
-                        prev_col.y = prev_col.y + wts[i] * pc.y;
-                        prev_col.z = prev_col.z + wts[i] * pc.z;
-                        total_weight = total_weight + wts[i];
-                    }
-                }
-            }
+            const float3 pp = tpp[i];
+            const float dx = pp.x - wp.x, dy = pp.y - wp.y, dz = pp.z - wp.z;
+            const float pd = (dx * dx + dy * dy) + dz * dz;
+            const float3 pn = tpn[i];
+            const float nx = pn.x - nrm.x, ny = pn.y - nrm.y, nz = pn.z - nrm.z;
+            const float nd = (nx * nx + ny * ny) + nz * nz;
+            const bool ok = tap_in[i] & (pd < a.position_limit_squared) & (nd < a.normal_limit_squared);
+            const float3 pc = tpc[i];
+            store_accept |= ok ? 1u << i : 0u;
+            sample_spp = ok ? sample_spp + wts[i] * tsp[i] : sample_spp;
+            prev_col.x = ok ? prev_col.x + wts[i] * pc.x : prev_col.x;
+            prev_col.y = ok ? prev_col.y + wts[i] * pc.y : prev_col.y;
+            prev_col.z = ok ? prev_col.z + wts[i] * pc.z : prev_col.z;
+            total_weight = ok ? total_weight + wts[i] : total_weight;
         }
         if (total_weight > 0.f) {  // bmfr.cl:421-429
             const double rw = quotient_reciprocal(total_weight);   // IEEE bits of x / total_weight
@@ -145,9 +144,9 @@ __device__ __forceinline__ Stage1Pixel stage1_pixel(const StageArgs &a, const St
     // work-items would only rewrite with the same value)
     if (pwx >= 0 && pwx < W && pwy >= 0 && pwy < H) {
         store3(acc_noisy, lp, nc);
-        out_prev_pixel[lp] = make_float2(ppf_x, ppf_y);
-        accept_bools[lp] = (uint8_t)store_accept;
-        cur_spp[lp] = (uint8_t)new_spp;
+        gstore(out_prev_pixel, lp, make_float2(ppf_x, ppf_y));
+        gstore(accept_bools, lp, (uint8_t)store_accept);
+        gstore(cur_spp, lp, (uint8_t)new_spp);
     }
     return Stage1Pixel{wp, nrm, nc};
 }

@@ -37,11 +37,11 @@ __device__ __forceinline__ void stage1_body(const StageArgs &a, const StageBuffe
         if (write_noise) {
             const int nt = (nf - 1) * kBlockPixels, stride = a.margin_width * a.margin_height;
             for (int i = lin; i < nt; i += stride)
-                ((float *)bufs.noise)[i] = noise_random(i & (kBlockPixels - 1), 1 + (i >> 10), a.frame, ncol);
+                gstore((float *)bufs.noise, i, noise_random(i & (kBlockPixels - 1), 1 + (i >> 10), a.frame, ncol));
         }
     } else if (write_noise && lin < kNoiseTable) {
         // the padded grid has at least 96x96 = 9216 work-items (W, H >= 47)
-        ((float *)bufs.noise)[lin] = noise_random(lin & (kBlockPixels - 1), 1 + (lin >> 10), a.frame);
+        gstore((float *)bufs.noise, lin, noise_random(lin & (kBlockPixels - 1), 1 + (lin >> 10), a.frame));
     }
     const Stage1Pixel q = stage1_pixel<GT>(a, bufs, gx, gy);
     // The work-group's 32x8 tile is rows yin0..yin0+7 of one block, and
@@ -70,8 +70,8 @@ __device__ __forceinline__ void stage1_body(const StageArgs &a, const StageBuffe
         constexpr int kPerFeature = 256 / kVec;
         for (int c = threadIdx.x; c < ncol * kPerFeature; c += 256) {
             const int f = c / kPerFeature, e = (c - f * kPerFeature) * kVec;
-            *reinterpret_cast<uint4 *>(dst + f * kBlockPixels + e) =
-                *reinterpret_cast<const uint4 *>(&feat_tile[f][e]);
+            gstore(reinterpret_cast<uint4 *>(dst + f * kBlockPixels + e), 0,
+                   *reinterpret_cast<const uint4 *>(&feat_tile[f][e]));
         }
     }
 }
@@ -141,7 +141,7 @@ __device__ __forceinline__ ScaleRecips scale_recips(const StageBuffers &bufs, in
     const float *mm = (const float *)bufs.mins_maxs + g * (kScaled * 2);
     const int l = (int)(threadIdx.x & 63);
     float d = 1.f;
-    if (l < kScaled) d = mm[l * 2 + 1] - mm[l * 2];
+    if (l < kScaled) d = gload(mm, l * 2 + 1) - gload(mm, l * 2);
     const double y = quotient_reciprocal(d);
     ScaleRecips r;
 #pragma unroll
@@ -175,11 +175,11 @@ __device__ __forceinline__ float3 stage3_color_of(const StageArgs &a, const Stag
             float v = feature_value(fs, f, n, wp);
             if (f >= fs.not_scaled) {
                 const int i = f - fs.not_scaled;
-                v = scale_feature(v, mm[i * 2], mm[i * 2 + 1]);
+                v = scale_feature(v, gload(mm, i * 2), gload(mm, i * 2 + 1));
             }
-            cx = cx + w[f * 3 + 0] * v;
-            cy = cy + w[f * 3 + 1] * v;
-            cz = cz + w[f * 3 + 2] * v;
+            cx = cx + gload(w, f * 3 + 0) * v;
+            cy = cy + gload(w, f * 3 + 1) * v;
+            cz = cz + gload(w, f * 3 + 2) * v;
         }
     } else {
         const float feat[kFeatures] = {1.f,  n.x,  n.y,         n.z,         wp.x,
@@ -191,12 +191,12 @@ __device__ __forceinline__ float3 stage3_color_of(const StageArgs &a, const Stag
             float v = feat[f];
             if (f >= kNotScaled) {
                 const int i = f - kNotScaled;
-                v = rec ? scale_feature_r(v, mm[i * 2], mm[i * 2 + 1], rec->s[i])
-                        : scale_feature(v, mm[i * 2], mm[i * 2 + 1]);
+                const float mn = gload(mm, i * 2), mx = gload(mm, i * 2 + 1);
+                v = rec ? scale_feature_r(v, mn, mx, rec->s[i]) : scale_feature(v, mn, mx);
             }
-            cx = cx + w[f * 3 + 0] * v;
-            cy = cy + w[f * 3 + 1] * v;
-            cz = cz + w[f * 3 + 2] * v;
+            cx = cx + gload(w, f * 3 + 0) * v;
+            cy = cy + gload(w, f * 3 + 1) * v;
+            cz = cz + gload(w, f * 3 + 2) * v;
         }
     }
     return make_float3(cx < 0.f ? 0.f : cx, cy < 0.f ? 0.f : cy, cz < 0.f ? 0.f : cz);
@@ -220,9 +220,9 @@ __device__ __forceinline__ void stage3_body(const StageArgs &a, const StageBuffe
     const int64_t lp = (int64_t)y * a.width + x;
     const float3 c = stage3_color<GT, CUSTOM>(a, bufs, lp, g, CUSTOM ? nullptr : &rec);
     float *__restrict__ output = (float *)bufs.filtered;
-    output[lp * 3 + 0] = c.x;
-    output[lp * 3 + 1] = c.y;
-    output[lp * 3 + 2] = c.z;
+    gstore(output, lp * 3 + 0, c.x);
+    gstore(output, lp * 3 + 1, c.y);
+    gstore(output, lp * 3 + 2, c.z);
 }
 
 template <typename GT, bool CUSTOM>
@@ -255,11 +255,11 @@ __device__ __forceinline__ Stage4Gather<GT> stage4_gather(const StageArgs &a, co
     const GT *__restrict__ acc_prev = (const GT *)bufs.out_prev;
     const int W = a.width;
     Stage4Gather<GT> g;
-    const uint32_t acc = a.frame > 0 ? (uint32_t)accept_bools[lp] : 0u;
-    const float2 pf = prev_pixel[lp];
+    const uint32_t acc = a.frame > 0 ? (uint32_t)gload(accept_bools, lp) : 0u;
+    const float2 pf = gload(prev_pixel, lp);
     g.pf = pf;
-    g.spp = cur_spp[lp];
-    g.albedo = make_float3(albedo[lp * 3 + 0], albedo[lp * 3 + 1], albedo[lp * 3 + 2]);
+    g.spp = gload(cur_spp, lp);
+    g.albedo = make_float3(gload(albedo, lp * 3 + 0), gload(albedo, lp * 3 + 1), gload(albedo, lp * 3 + 2));
     const int ix = cvt_rtn(pf.x), iy = cvt_rtn(pf.y);
     const float fx = pf.x - (float)ix, fy = pf.y - (float)iy;
     const float omx = 1.f - fx, omy = 1.f - fy;
@@ -329,9 +329,9 @@ __device__ __forceinline__ void stage4_finish(const StageArgs &a, const StageBuf
     const float3 ac = stage4_blend<GT>(a, fc, g);
     store3((GT *)bufs.out_cur, lp, ac);
     const float3 t = tone_map(g.albedo, ac);
-    tone_mapped[lp * 3 + 0] = t.x;
-    tone_mapped[lp * 3 + 1] = t.y;
-    tone_mapped[lp * 3 + 2] = t.z;
+    gstore(tone_mapped, lp * 3 + 0, t.x);
+    gstore(tone_mapped, lp * 3 + 1, t.y);
+    gstore(tone_mapped, lp * 3 + 2, t.z);
 }
 
 template <typename GT>
@@ -340,7 +340,8 @@ __device__ __forceinline__ void stage4_body(const StageArgs &a, const StageBuffe
     const int64_t lp = (int64_t)y * a.width + x;
     const float *__restrict__ filtered = (const float *)bufs.filtered;
     const Stage4Gather<GT> gth = stage4_gather<GT>(a, bufs, lp);
-    stage4_finish<GT>(a, bufs, lp, make_float3(filtered[lp * 3 + 0], filtered[lp * 3 + 1], filtered[lp * 3 + 2]),
+    stage4_finish<GT>(a, bufs, lp,
+                      make_float3(gload(filtered, lp * 3 + 0), gload(filtered, lp * 3 + 1), gload(filtered, lp * 3 + 2)),
                       gth);
 }
 
@@ -531,8 +532,8 @@ __device__ __forceinline__ void stage5_body(const StageArgs &a, const StageBuffe
     // tile's, so that they are in flight during the tile load and the barrier
     const bool inside = x < W && y < H;
     const int64_t lp = inside ? (int64_t)y * W + x : 0;
-    const float2 pf = prev_pixel[lp];
-    const float3 mine = make_float3(new_frame[lp * 3 + 0], new_frame[lp * 3 + 1], new_frame[lp * 3 + 2]);
+    const float2 pf = gload(prev_pixel, lp);
+    const float3 mine = make_float3(gload(new_frame, lp * 3 + 0), gload(new_frame, lp * 3 + 1), gload(new_frame, lp * 3 + 2));
     const TaaHistory<GT> h = taa_history<GT>(a, bufs, lp, pf);
     for (int e = threadIdx.x; e < kTaaTileW * kTaaTileH; e += 256) {
         const int ry = e / kTaaTileW, rx = e - ry * kTaaTileW;
@@ -540,7 +541,7 @@ __device__ __forceinline__ void stage5_body(const StageArgs &a, const StageBuffe
         float4 t = make_float4(__builtin_nanf(""), __builtin_nanf(""), __builtin_nanf(""), 0.f);
         if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
             const int64_t ls = (int64_t)gy * W + gx;
-            t = ycocg_texel(make_float3(new_frame[ls * 3 + 0], new_frame[ls * 3 + 1], new_frame[ls * 3 + 2]));
+            t = ycocg_texel(make_float3(gload(new_frame, ls * 3 + 0), gload(new_frame, ls * 3 + 1), gload(new_frame, ls * 3 + 2)));
         }
         tile[ry][rx] = t;
     }
@@ -768,7 +769,7 @@ hipError_t launch_accumulate_noisy_data(const StageArgs &a, const StageBuffers &
 __global__ __launch_bounds__(256) void k_noise_table(int frame, int features, float *__restrict__ noise) {
     const int lin = blockIdx.x * 256 + threadIdx.x;
     if (lin < (features - 1) * kBlockPixels)
-        noise[lin] = noise_random(lin & (kBlockPixels - 1), 1 + (lin >> 10), frame, features + 3);
+        gstore(noise, lin, noise_random(lin & (kBlockPixels - 1), 1 + (lin >> 10), frame, features + 3));
 }
 
 hipError_t launch_noise_table(const StageArgs &a, const StageBuffers &b, hipStream_t s) {

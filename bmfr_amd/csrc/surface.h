@@ -8,6 +8,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "device_common.h"
+
 namespace bmfr {
 
 enum SurfaceFormat { kSurfaceRgbF32 = 0, kSurfaceRgbaF32 = 1, kSurfaceRgbaF16 = 2, kSurfaceRgbaU8 = 3 };
@@ -37,24 +39,25 @@ __device__ __forceinline__ void store_surface(void *surface, int format, int64_t
     switch (format) {
         case kSurfaceRgbF32: {
             float *p = (float *)row + 3 * (int64_t)x;
-            p[0] = v.x;
-            p[1] = v.y;
-            p[2] = v.z;
+            gstore(p, 0, v.x);
+            gstore(p, 1, v.y);
+            gstore(p, 2, v.z);
             break;
         }
         case kSurfaceRgbaF32:
-            *(float4 *)(row + 16 * (int64_t)x) = make_float4(v.x, v.y, v.z, 1.f);
+            gstore((float4 *)(row + 16 * (int64_t)x), 0, make_float4(v.x, v.y, v.z, 1.f));
             break;
         case kSurfaceRgbaF16: {
             const uint32_t lo = (uint32_t)__half_as_ushort(__float2half_rn(v.x)) |
                                 ((uint32_t)__half_as_ushort(__float2half_rn(v.y)) << 16);
             const uint32_t hi = (uint32_t)__half_as_ushort(__float2half_rn(v.z)) | (0x3c00u << 16);
-            *(uint2 *)(row + 8 * (int64_t)x) = make_uint2(lo, hi);
+            gstore((uint2 *)(row + 8 * (int64_t)x), 0, make_uint2(lo, hi));
             break;
         }
         default:
-            *(uint32_t *)(row + 4 * (int64_t)x) = (uint32_t)quantize_u8(v.x) | ((uint32_t)quantize_u8(v.y) << 8) |
-                                                  ((uint32_t)quantize_u8(v.z) << 16) | 0xff000000u;
+            gstore((uint32_t *)(row + 4 * (int64_t)x), 0,
+                   (uint32_t)quantize_u8(v.x) | ((uint32_t)quantize_u8(v.y) << 8) | ((uint32_t)quantize_u8(v.z) << 16) |
+                       0xff000000u);
             break;
     }
 }

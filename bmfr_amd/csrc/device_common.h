@@ -122,6 +122,42 @@ __device__ __forceinline__ void store3(T *p, int64_t pixel, float3 v) {
     Store<T>::st(p, pixel * 3 + 2, v.z);
 }
 
+// Per-pixel buffers (one to three elements a pixel) of the largest frame the
+// library accepts, 16384 x 16384, hold at most 3 GiB, so a byte offset into
+// one fits in 32 bits.  Formed in 32 bits and added to the buffer's uniform
+// base, it lets the compiler use the scalar-base form of the global
+// instructions (one VGPR of offset) instead of a 64-bit VGPR address per
+// access.  `px` is the element index (pixel * elements per pixel for the
+// single-element forms, the pixel for the three-element ones).
+template <typename T>
+__device__ __forceinline__ T *px_ptr(T *p, uint32_t byte_offset) {
+    return (T *)((char *)p + byte_offset);
+}
+template <typename T>
+__device__ __forceinline__ const T *px_ptr(const T *p, uint32_t byte_offset) {
+    return (const T *)((const char *)p + byte_offset);
+}
+template <typename T>
+__device__ __forceinline__ T gload_px(const T *p, uint32_t px) {
+    return gload(px_ptr(p, px * (uint32_t)sizeof(T)), 0);
+}
+template <typename T>
+__device__ __forceinline__ void gstore_px(T *p, uint32_t px, T v) {
+    gstore(px_ptr(p, px * (uint32_t)sizeof(T)), 0, v);
+}
+template <typename T>
+__device__ __forceinline__ float3 load3_px(const T *p, uint32_t pixel) {
+    const T *q = px_ptr(p, pixel * (uint32_t)(3 * sizeof(T)));
+    return make_float3(Store<T>::ld(q, 0), Store<T>::ld(q, 1), Store<T>::ld(q, 2));
+}
+template <typename T>
+__device__ __forceinline__ void store3_px(T *p, uint32_t pixel, float3 v) {
+    T *q = px_ptr(p, pixel * (uint32_t)(3 * sizeof(T)));
+    Store<T>::st(q, 0, v.x);
+    Store<T>::st(q, 1, v.y);
+    Store<T>::st(q, 2, v.z);
+}
+
 // convert_int_rtn (opencl/bmfr.cl:356, 794, 879) with an explicitly
 // saturating conversion, NaN -> 0 (what v_cvt_i32_f32 does); written out so
 // that no out-of-range float->int conversion is left undefined.

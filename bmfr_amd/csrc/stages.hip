@@ -155,18 +155,63 @@ __device__ __forceinline__ float scale_feature_r(float v, float mn, float mx, do
     return v - mn;
 }
 
-// Filtered colour of in-image pixel lp in block g: sum of block weights
-// times the pixel's (scaled) features, negatives clamped (bmfr.cl:712-750).
+// The weights and scaling bounds of one block for the reference's feature
+// set, read once: a kernel that stores before it has used them all would
+// otherwise read them again for every pixel (the compiler cannot tell that
+// its stores leave them unchanged).
+struct BlockFit {
+    float w[kFeatures * 3];
+    float mn[kScaled], mx[kScaled];
+};
+
+__device__ __forceinline__ BlockFit block_fit(const StageBuffers &bufs, int64_t g) {
+    const float *w = (const float *)bufs.weights + g * (kFeatures * 3);
+    const float *mm = (const float *)bufs.mins_maxs + g * (kScaled * 2);
+    BlockFit f;
+#pragma unroll
+    for (int i = 0; i < kFeatures * 3; ++i) f.w[i] = gload(w, i);
+#pragma unroll
+    for (int i = 0; i < kScaled; ++i) {
+        f.mn[i] = gload(mm, i * 2);
+        f.mx[i] = gload(mm, i * 2 + 1);
+    }
+    return f;
+}
+
+// Filtered colour of a pixel with world position wp and normal n from its
+// block's fit: sum of weights times the (scaled) features, negatives clamped
+// (bmfr.cl:712-750).  With `rec` (the wave's reciprocals for the block) the
+// scaling divisions become products.
+__device__ __forceinline__ float3 stage3_color_fit(float3 wp, float3 n, const BlockFit &fit,
+                                                   const ScaleRecips *rec) {
+    const float feat[kFeatures] = {1.f,  n.x,  n.y,         n.z,         wp.x,
+                                   wp.y, wp.z, wp.x * wp.x, wp.y * wp.y, wp.z * wp.z};
+    float cx = 0.f, cy = 0.f, cz = 0.f;
+#pragma unroll
+    for (int f = 0; f < kFeatures; ++f) {
+        float v = feat[f];
+        if (f >= kNotScaled) {
+            const int i = f - kNotScaled;
+            v = rec ? scale_feature_r(v, fit.mn[i], fit.mx[i], rec->s[i]) : scale_feature(v, fit.mn[i], fit.mx[i]);
+        }
+        cx = cx + fit.w[f * 3 + 0] * v;
+        cy = cy + fit.w[f * 3 + 1] * v;
+        cz = cz + fit.w[f * 3 + 2] * v;
+    }
+    return make_float3(cx < 0.f ? 0.f : cx, cy < 0.f ? 0.f : cy, cz < 0.f ? 0.f : cz);
+}
+
+// Filtered colour of in-image pixel lp in block g (bmfr.cl:712-750).
 // CUSTOM evaluates the feature set in `a.features`.  With `rec` (the wave's
 // reciprocals for block g, reference set only) the scaling divisions become
 // products.
 template <typename GT, bool CUSTOM = false>
 __device__ __forceinline__ float3 stage3_color_of(const StageArgs &a, const StageBuffers &bufs, float3 wp, float3 n,
                                                  int64_t g, const ScaleRecips *rec = nullptr) {
-    const float *__restrict__ weights = (const float *)bufs.weights;
-    const float *__restrict__ mins_maxs = (const float *)bufs.mins_maxs;
-    float cx = 0.f, cy = 0.f, cz = 0.f;
     if constexpr (CUSTOM) {
+        const float *__restrict__ weights = (const float *)bufs.weights;
+        const float *__restrict__ mins_maxs = (const float *)bufs.mins_maxs;
+        float cx = 0.f, cy = 0.f, cz = 0.f;
         const FeatureSet &fs = a.features;
         const int nf = fs.count, ns = fs.count - fs.not_scaled;
         const float *w = weights + g * (nf * 3);
@@ -181,32 +226,17 @@ __device__ __forceinline__ float3 stage3_color_of(const StageArgs &a, const Stag
             cy = cy + gload(w, f * 3 + 1) * v;
             cz = cz + gload(w, f * 3 + 2) * v;
         }
+        return make_float3(cx < 0.f ? 0.f : cx, cy < 0.f ? 0.f : cy, cz < 0.f ? 0.f : cz);
     } else {
-        const float feat[kFeatures] = {1.f,  n.x,  n.y,         n.z,         wp.x,
-                                       wp.y, wp.z, wp.x * wp.x, wp.y * wp.y, wp.z * wp.z};
-        const float *w = weights + g * (kFeatures * 3);
-        const float *mm = mins_maxs + g * (kScaled * 2);
-#pragma unroll
-        for (int f = 0; f < kFeatures; ++f) {
-            float v = feat[f];
-            if (f >= kNotScaled) {
-                const int i = f - kNotScaled;
-                const float mn = gload(mm, i * 2), mx = gload(mm, i * 2 + 1);
-                v = rec ? scale_feature_r(v, mn, mx, rec->s[i]) : scale_feature(v, mn, mx);
-            }
-            cx = cx + gload(w, f * 3 + 0) * v;
-            cy = cy + gload(w, f * 3 + 1) * v;
-            cz = cz + gload(w, f * 3 + 2) * v;
-        }
+        return stage3_color_fit(wp, n, block_fit(bufs, g), rec);
     }
-    return make_float3(cx < 0.f ? 0.f : cx, cy < 0.f ? 0.f : cy, cz < 0.f ? 0.f : cz);
 }
 
 template <typename GT, bool CUSTOM = false>
 __device__ __forceinline__ float3 stage3_color(const StageArgs &a, const StageBuffers &bufs, int64_t lp,
                                               int64_t g, const ScaleRecips *rec = nullptr) {
-    return stage3_color_of<GT, CUSTOM>(a, bufs, load3((const GT *)bufs.positions_cur, lp),
-                                       load3((const GT *)bufs.normals_cur, lp), g, rec);
+    return stage3_color_of<GT, CUSTOM>(a, bufs, load3_px((const GT *)bufs.positions_cur, (uint32_t)lp),
+                                       load3_px((const GT *)bufs.normals_cur, (uint32_t)lp), g, rec);
 }
 
 template <typename GT, bool CUSTOM = false>
@@ -220,9 +250,7 @@ __device__ __forceinline__ void stage3_body(const StageArgs &a, const StageBuffe
     const int64_t lp = (int64_t)y * a.width + x;
     const float3 c = stage3_color<GT, CUSTOM>(a, bufs, lp, g, CUSTOM ? nullptr : &rec);
     float *__restrict__ output = (float *)bufs.filtered;
-    gstore(output, lp * 3 + 0, c.x);
-    gstore(output, lp * 3 + 1, c.y);
-    gstore(output, lp * 3 + 2, c.z);
+    store3_px(output, (uint32_t)lp, c);
 }
 
 template <typename GT, bool CUSTOM>
@@ -246,20 +274,42 @@ struct Stage4Gather {
     float2 pf;       // prev_pixel (stage 5 reads it too)
 };
 
-template <typename GT>
-__device__ __forceinline__ Stage4Gather<GT> stage4_gather(const StageArgs &a, const StageBuffers &bufs, int64_t lp) {
+// Stage 4's per-pixel inputs (the first memory round trip): stage 1's accept
+// bits, spp and prev_pixel, and the albedo.
+struct Stage4Inputs {
+    float3 albedo;
+    float2 pf;
+    uint32_t acc;
+    uint32_t spp;
+};
+
+__device__ __forceinline__ Stage4Inputs stage4_inputs(const StageArgs &a, const StageBuffers &bufs, int64_t lp) {
     const float2 *__restrict__ prev_pixel = (const float2 *)bufs.prev_pixel;
     const uint8_t *__restrict__ accept_bools = (const uint8_t *)bufs.accept;
     const float *__restrict__ albedo = (const float *)bufs.albedo;
     const uint8_t *__restrict__ cur_spp = (const uint8_t *)bufs.spp_cur;
+    const uint32_t px = (uint32_t)lp;
+    Stage4Inputs in;
+    in.acc = a.frame > 0 ? (uint32_t)gload_px(accept_bools, px) : 0u;
+    in.pf = gload_px(prev_pixel, px);
+    in.spp = gload_px(cur_spp, px);
+    in.albedo = load3_px(albedo, px);
+    return in;
+}
+
+// The history taps of pixel lp (the second round trip, addressed by the
+// inputs).
+template <typename GT>
+__device__ __forceinline__ Stage4Gather<GT> stage4_taps(const StageArgs &a, const StageBuffers &bufs, int64_t lp,
+                                                        const Stage4Inputs &in) {
     const GT *__restrict__ acc_prev = (const GT *)bufs.out_prev;
     const int W = a.width;
     Stage4Gather<GT> g;
-    const uint32_t acc = a.frame > 0 ? (uint32_t)gload(accept_bools, lp) : 0u;
-    const float2 pf = gload(prev_pixel, lp);
+    const uint32_t acc = in.acc;
+    const float2 pf = in.pf;
     g.pf = pf;
-    g.spp = gload(cur_spp, lp);
-    g.albedo = make_float3(gload(albedo, lp * 3 + 0), gload(albedo, lp * 3 + 1), gload(albedo, lp * 3 + 2));
+    g.spp = in.spp;
+    g.albedo = in.albedo;
     const int ix = cvt_rtn(pf.x), iy = cvt_rtn(pf.y);
     const float fx = pf.x - (float)ix, fy = pf.y - (float)iy;
     const float omx = 1.f - fx, omy = 1.f - fy;
@@ -276,9 +326,14 @@ __device__ __forceinline__ Stage4Gather<GT> stage4_gather(const StageArgs &a, co
         // the debug API) from reading outside.
         const bool use = (acc & (1u << i)) && ls >= 0 && ls < (int64_t)W * a.height;
         g.use |= use ? 1u << i : 0u;
-        g.tap[i] = load3(acc_prev, use ? ls : lp);
+        g.tap[i] = load3_px(acc_prev, (uint32_t)(use ? ls : lp));
     }
     return g;
+}
+
+template <typename GT>
+__device__ __forceinline__ Stage4Gather<GT> stage4_gather(const StageArgs &a, const StageBuffers &bufs, int64_t lp) {
+    return stage4_taps<GT>(a, bufs, lp, stage4_inputs(a, bufs, lp));
 }
 
 // The accumulated colour of a pixel from its filtered colour `fc` and its
@@ -327,11 +382,8 @@ __device__ __forceinline__ void stage4_finish(const StageArgs &a, const StageBuf
                                               const Stage4Gather<GT> &g) {
     float *__restrict__ tone_mapped = (float *)bufs.tone_mapped;
     const float3 ac = stage4_blend<GT>(a, fc, g);
-    store3((GT *)bufs.out_cur, lp, ac);
-    const float3 t = tone_map(g.albedo, ac);
-    gstore(tone_mapped, lp * 3 + 0, t.x);
-    gstore(tone_mapped, lp * 3 + 1, t.y);
-    gstore(tone_mapped, lp * 3 + 2, t.z);
+    store3_px((GT *)bufs.out_cur, (uint32_t)lp, ac);
+    store3_px(tone_mapped, (uint32_t)lp, tone_map(g.albedo, ac));
 }
 
 template <typename GT>
@@ -341,7 +393,7 @@ __device__ __forceinline__ void stage4_body(const StageArgs &a, const StageBuffe
     const float *__restrict__ filtered = (const float *)bufs.filtered;
     const Stage4Gather<GT> gth = stage4_gather<GT>(a, bufs, lp);
     stage4_finish<GT>(a, bufs, lp,
-                      make_float3(gload(filtered, lp * 3 + 0), gload(filtered, lp * 3 + 1), gload(filtered, lp * 3 + 2)),
+                      load3_px(filtered, (uint32_t)lp),
                       gth);
 }
 
@@ -423,7 +475,7 @@ __device__ __forceinline__ TaaHistory<GT> taa_history(const StageArgs &a, const 
     h.use[3] = h.history && iy < H - 1 && ix < W - 1;
     const int64_t tap_ls[4] = {r0 + ix, r0 + ix + 1, r1 + ix, r1 + ix + 1};
 #pragma unroll
-    for (int i = 0; i < 4; ++i) h.tap[i] = load3(prev_result, h.use[i] ? tap_ls[i] : lp);
+    for (int i = 0; i < 4; ++i) h.tap[i] = load3_px(prev_result, (uint32_t)(h.use[i] ? tap_ls[i] : lp));
     return h;
 }
 
@@ -444,13 +496,37 @@ __device__ __forceinline__ float vmax3(float a, float b, float c) {
     return r;
 }
 
-// TAA of one pixel (bmfr.cl:884-973): `mine` is its tone-mapped colour, `nb`
-// the top-left texel of its 3x3 YCoCg neighbourhood in an LDS tile of row
+// The bilinear history colour of a pixel with history, in YCoCg
+// (bmfr.cl:922-963).
+template <typename GT>
+__device__ __forceinline__ float3 taa_history_ycocg(float2 pf, const TaaHistory<GT> &h) {
+    float3 pc = make_float3(0.f, 0.f, 0.f);
+    float tw = 0.f;
+    const float fx = pf.x - (float)h.ix, fy = pf.y - (float)h.iy;
+    const float omx = 1.f - fx, omy = 1.f - fy;
+    const float wts[4] = {omx * omy, fx * omy, omx * fy, fx * fy};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        if (h.use[i]) {
+            pc.x = pc.x + wts[i] * h.tap[i].x;
+            pc.y = pc.y + wts[i] * h.tap[i].y;
+            pc.z = pc.z + wts[i] * h.tap[i].z;
+            tw = tw + wts[i];
+        }
+    }
+    const double rtw = quotient_reciprocal(tw);   // pc / tw, IEEE bits; unguarded like bmfr.cl:962
+    pc.x = quotient(pc.x, rtw);
+    pc.y = quotient(pc.y, rtw);
+    pc.z = quotient(pc.z, rtw);
+    return rgb_to_ycocg(pc);
+}
+
+// The history colour `pyc` clamped to the pixel's 3x3 neighbourhood and
+// blended with its tone-mapped colour `mine` (bmfr.cl:893-920, 964-973).
+// `nbp` is the top-left texel of the neighbourhood in an LDS tile of row
 // stride STRIDE (NaN outside the image).
-template <int STRIDE, typename GT>
-__device__ __forceinline__ float3 taa_resolve(const StageArgs &a, float2 pf, float3 mine, const TaaHistory<GT> &h,
-                                              const float4 *nbp) {
-    if (!h.history) return mine;
+template <int STRIDE>
+__device__ __forceinline__ float3 taa_clamp_blend(const StageArgs &a, float3 mine, float3 pyc, const float4 *nbp) {
     float4 nb[3][3];
 #pragma unroll
     for (int dy = 0; dy < 3; ++dy)
@@ -470,25 +546,6 @@ __device__ __forceinline__ float3 taa_resolve(const StageArgs &a, float2 pf, flo
     mxb.x = vmax3(vmax3(mxc.x, nb[0][0].x, nb[0][2].x), nb[2][0].x, nb[2][2].x);
     mxb.y = vmax3(vmax3(mxc.y, nb[0][0].y, nb[0][2].y), nb[2][0].y, nb[2][2].y);
     mxb.z = vmax3(vmax3(mxc.z, nb[0][0].z, nb[0][2].z), nb[2][0].z, nb[2][2].z);
-    float3 pc = make_float3(0.f, 0.f, 0.f);
-    float tw = 0.f;
-    const float fx = pf.x - (float)h.ix, fy = pf.y - (float)h.iy;
-    const float omx = 1.f - fx, omy = 1.f - fy;
-    const float wts[4] = {omx * omy, fx * omy, omx * fy, fx * fy};
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        if (h.use[i]) {
-            pc.x = pc.x + wts[i] * h.tap[i].x;
-            pc.y = pc.y + wts[i] * h.tap[i].y;
-            pc.z = pc.z + wts[i] * h.tap[i].z;
-            tw = tw + wts[i];
-        }
-    }
-    const double rtw = quotient_reciprocal(tw);   // pc / tw, IEEE bits; unguarded like bmfr.cl:962
-    pc.x = quotient(pc.x, rtw);
-    pc.y = quotient(pc.y, rtw);
-    pc.z = quotient(pc.z, rtw);
-    const float3 pyc = rgb_to_ycocg(pc);
     float3 cl;
     cl.x = fminf(fmaxf(pyc.x, (mnb.x + mnc.x) / 2.f), (mxb.x + mxc.x) / 2.f);
     cl.y = fminf(fmaxf(pyc.y, (mnb.y + mnc.y) / 2.f), (mxb.y + mxc.y) / 2.f);
@@ -502,13 +559,22 @@ __device__ __forceinline__ float3 taa_resolve(const StageArgs &a, float2 pf, flo
     return r;
 }
 
+// TAA of one pixel (bmfr.cl:884-973): `mine` is its tone-mapped colour, `nbp`
+// as for taa_clamp_blend.
+template <int STRIDE, typename GT>
+__device__ __forceinline__ float3 taa_resolve(const StageArgs &a, float2 pf, float3 mine, const TaaHistory<GT> &h,
+                                              const float4 *nbp) {
+    if (!h.history) return mine;
+    return taa_clamp_blend<STRIDE>(a, mine, taa_history_ycocg<GT>(pf, h), nbp);
+}
+
 // The frame's TAA result into the result history (result_buffer.current())
 // and, when one is bound, the caller's output surface, which receives the
 // stored value (rounded to the history's storage type).
 template <typename GT>
 __device__ __forceinline__ void store_result(const StageArgs &a, const StageBuffers &bufs, int64_t lp, int x, int y,
                                              float3 r) {
-    store3((GT *)bufs.result_cur, lp, r);
+    store3_px((GT *)bufs.result_cur, (uint32_t)lp, r);
     if (bufs.surface) {
         if constexpr (!std::is_same<GT, float>::value) r = make_float3(round_half(r.x), round_half(r.y), round_half(r.z));
         store_surface(bufs.surface, a.surface_format, a.surface_pitch, x, y, r);
@@ -532,8 +598,8 @@ __device__ __forceinline__ void stage5_body(const StageArgs &a, const StageBuffe
     // tile's, so that they are in flight during the tile load and the barrier
     const bool inside = x < W && y < H;
     const int64_t lp = inside ? (int64_t)y * W + x : 0;
-    const float2 pf = gload(prev_pixel, lp);
-    const float3 mine = make_float3(gload(new_frame, lp * 3 + 0), gload(new_frame, lp * 3 + 1), gload(new_frame, lp * 3 + 2));
+    const float2 pf = gload_px(prev_pixel, (uint32_t)lp);
+    const float3 mine = load3_px(new_frame, (uint32_t)lp);
     const TaaHistory<GT> h = taa_history<GT>(a, bufs, lp, pf);
     for (int e = threadIdx.x; e < kTaaTileW * kTaaTileH; e += 256) {
         const int ry = e / kTaaTileW, rx = e - ry * kTaaTileW;
@@ -541,7 +607,7 @@ __device__ __forceinline__ void stage5_body(const StageArgs &a, const StageBuffe
         float4 t = make_float4(__builtin_nanf(""), __builtin_nanf(""), __builtin_nanf(""), 0.f);
         if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
             const int64_t ls = (int64_t)gy * W + gx;
-            t = ycocg_texel(make_float3(gload(new_frame, ls * 3 + 0), gload(new_frame, ls * 3 + 1), gload(new_frame, ls * 3 + 2)));
+            t = ycocg_texel(load3_px(new_frame, (uint32_t)ls));
         }
         tile[ry][rx] = t;
     }
@@ -588,7 +654,11 @@ __device__ __forceinline__ void stage345_body(const StageArgs &a, const StageBuf
     __syncthreads();
     const ToneTablesLds tt{tone_log, tone_pow2};
     ScaleRecips rec;
-    if constexpr (!CUSTOM) rec = scale_recips(bufs, blk);   // all lanes active; blk is the work-group's
+    BlockFit fit;
+    if constexpr (!CUSTOM) {
+        rec = scale_recips(bufs, blk);   // all lanes active; blk is the work-group's
+        fit = block_fit(bufs, blk);
+    }
     const int W = a.width, H = a.height;
     const int2 off = block_offset(a.frame);
     const int by = blk / a.blocks_x, bx = blk - by * a.blocks_x;
@@ -597,54 +667,95 @@ __device__ __forceinline__ void stage345_body(const StageArgs &a, const StageBuf
     const int y0 = by * kBlock - kBlock / 2 + off.y;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
     const float4 nan4 = make_float4(__builtin_nanf(""), __builtin_nanf(""), __builtin_nanf(""), 0.f);
-    float3 mine[4];
-    float2 pf[4];
+
+    // Pixel r (0..3) of this thread is (x0 + tx, y0 + ty + 8 r).  A pixel
+    // outside the image reads the nearest image pixel and its results are
+    // dropped, so that no load sits behind a branch: the loop below is a
+    // software pipeline in which pixel r + 1's inputs are in flight while
+    // pixel r's history taps are, and each pixel's TAA history taps are
+    // issued with its stage-4 taps (the second round trip) instead of after
+    // the tile barrier.
+    uint32_t lp[4];
+    bool inside[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int x = x0 + tx, y = y0 + ty + 8 * r;
-        float4 t = nan4;
-        pf[r] = make_float2(0.f, 0.f);
-        mine[r] = make_float3(0.f, 0.f, 0.f);
-        if (x >= 0 && y >= 0 && x < W && y < H) {
-            const int64_t lp = (int64_t)y * W + x;
-            const Stage4Gather<GT> gth = stage4_gather<GT>(a, bufs, lp);
-            const float3 ac =
-                stage4_blend<GT>(a, stage3_color<GT, CUSTOM>(a, bufs, lp, blk, CUSTOM ? nullptr : &rec), gth);
-            store3((GT *)bufs.out_cur, lp, ac);
-            mine[r] = tone_map(gth.albedo, ac, tt);
-            pf[r] = gth.pf;
-            t = ycocg_texel(mine[r]);
-        }
-        tile[ty + 8 * r + 1][tx + 1] = t;
+        inside[r] = x >= 0 && y >= 0 && x < W && y < H;
+        lp[r] = (uint32_t)(min(max(y, 0), H - 1) * W + min(max(x, 0), W - 1));
     }
-    if (threadIdx.x < kRingTexels) {
+    // The one-pixel ring around the block (pixels of the neighbouring blocks,
+    // whose weights and bounds are vector loads), one texel per thread of the
+    // first 132; it only goes into the tile.
+    const bool ring = threadIdx.x < kRingTexels;
+    int rx, ry;   // ring position in -1..32
+    {
         const int h = threadIdx.x;
-        int rx, ry;   // ring position in -1..32
         if (h < kRing) { rx = h - 1; ry = -1; }
         else if (h < 2 * kRing) { rx = h - kRing - 1; ry = kBlock; }
         else if (h < 2 * kRing + kBlock) { rx = -1; ry = h - 2 * kRing; }
         else { rx = kBlock; ry = h - 2 * kRing - kBlock; }
-        const int x = x0 + rx, y = y0 + ry;
-        float4 t = nan4;
-        if (x >= 0 && y >= 0 && x < W && y < H) {
-            const int64_t lp = (int64_t)y * W + x;
-            // the pixel's own block (bmfr.cl:715-718)
-            const int64_t g = (int64_t)((y + kBlock / 2 - off.y) / kBlock) * a.blocks_x +
-                              (x + kBlock / 2 - off.x) / kBlock;
-            const Stage4Gather<GT> gth = stage4_gather<GT>(a, bufs, lp);
-            const float3 ac = stage4_blend<GT>(a, stage3_color<GT, CUSTOM>(a, bufs, lp, g), gth);
-            t = ycocg_texel(tone_map(gth.albedo, ac, tt));
+    }
+    const int rgx = x0 + rx, rgy = y0 + ry;
+    const bool ring_inside = ring && rgx >= 0 && rgy >= 0 && rgx < W && rgy < H;
+    const int rcx = min(max(rgx, 0), W - 1), rcy = min(max(rgy, 0), H - 1);
+    const uint32_t ring_lp = (uint32_t)(rcy * W + rcx);
+
+    float3 mine[4], hist[4];
+    uint32_t has_history = 0;   // bit r: pixel r has a TAA history (bmfr.cl:884-890)
+    Stage4Inputs in = stage4_inputs(a, bufs, lp[0]);
+    float3 wp = load3_px((const GT *)bufs.positions_cur, lp[0]);
+    float3 nrm = load3_px((const GT *)bufs.normals_cur, lp[0]);
+    Stage4Inputs ring_in;
+    float3 ring_wp, ring_n;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const Stage4Gather<GT> gth = stage4_taps<GT>(a, bufs, lp[r], in);
+        const TaaHistory<GT> h = taa_history<GT>(a, bufs, lp[r], in.pf);
+        Stage4Inputs nin;
+        float3 nwp, nn;
+        if (r < 3) {
+            nin = stage4_inputs(a, bufs, lp[r + 1]);
+            nwp = load3_px((const GT *)bufs.positions_cur, lp[r + 1]);
+            nn = load3_px((const GT *)bufs.normals_cur, lp[r + 1]);
+        } else if (ring) {
+            ring_in = stage4_inputs(a, bufs, ring_lp);
+            ring_wp = load3_px((const GT *)bufs.positions_cur, ring_lp);
+            ring_n = load3_px((const GT *)bufs.normals_cur, ring_lp);
         }
+        float3 fc;
+        if constexpr (CUSTOM) fc = stage3_color_of<GT, true>(a, bufs, wp, nrm, blk);
+        else fc = stage3_color_fit(wp, nrm, fit, &rec);
+        const float3 ac = stage4_blend<GT>(a, fc, gth);
+        if (inside[r]) store3_px((GT *)bufs.out_cur, lp[r], ac);
+        mine[r] = tone_map(gth.albedo, ac, tt);
+        hist[r] = taa_history_ycocg<GT>(in.pf, h);
+        has_history |= inside[r] && h.history ? 1u << r : 0u;
+        tile[ty + 8 * r + 1][tx + 1] = inside[r] ? ycocg_texel(mine[r]) : nan4;
+        if (r < 3) {
+            in = nin;
+            wp = nwp;
+            nrm = nn;
+        }
+    }
+    if (ring) {
+        float4 t = nan4;
+        // the pixel's own block (bmfr.cl:715-718), from the clamped position
+        // so that it stays a block of the frame
+        const int64_t g = (int64_t)((rcy + kBlock / 2 - off.y) / kBlock) * a.blocks_x +
+                          (rcx + kBlock / 2 - off.x) / kBlock;
+        const Stage4Gather<GT> gth = stage4_taps<GT>(a, bufs, ring_lp, ring_in);
+        const float3 ac = stage4_blend<GT>(a, stage3_color_of<GT, CUSTOM>(a, bufs, ring_wp, ring_n, g), gth);
+        if (ring_inside) t = ycocg_texel(tone_map(gth.albedo, ac, tt));
         tile[ry + 1][rx + 1] = t;
     }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        const int x = x0 + tx, y = y0 + ty + 8 * r;
-        if (x >= 0 && y >= 0 && x < W && y < H) {
-            const int64_t lp = (int64_t)y * W + x;
-            const TaaHistory<GT> h = taa_history<GT>(a, bufs, lp, pf[r]);
-            store_result<GT>(a, bufs, lp, x, y, taa_resolve<kRing, GT>(a, pf[r], mine[r], h, &tile[ty + 8 * r][tx]));
+        if (inside[r]) {
+            const float3 res = (has_history >> r) & 1u
+                                   ? taa_clamp_blend<kRing>(a, mine[r], hist[r], &tile[ty + 8 * r][tx])
+                                   : mine[r];
+            store_result<GT>(a, bufs, lp[r], x0 + tx, y0 + ty + 8 * r, res);
         }
     }
 }

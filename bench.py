@@ -434,7 +434,7 @@ def main():
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--fitter-config", default="")
-    ap.add_argument("--fuse", type=int, default=1, help="stages 3-5: 0 separate, 1 3+4 fused, 2 3+4+5 fused")
+    ap.add_argument("--fuse", type=int, default=2, help="stages 3-5: 0 separate, 1 3+4 fused, 2 3+4+5 fused")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=40)
     ap.add_argument("--no-batched", action="store_true")

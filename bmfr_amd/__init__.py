@@ -229,8 +229,8 @@ class Stream:
         _check(self._lib.bmfr_run_frame(self._h, frame, _fp(m), _fp(o)))
 
     def set_fused(self, level=True):
-        """How run_frame runs stages 3-5: FUSE_NONE (False), FUSE_34 (a new
-        stream's level) or FUSE_345 (True)."""
+        """How run_frame runs stages 3-5: FUSE_NONE (False), FUSE_34 or
+        FUSE_345 (True, a new stream's level)."""
         if isinstance(level, bool):
             level = FUSE_345 if level else FUSE_NONE
         _check(self._lib.bmfr_set_fused(self._h, int(level)))

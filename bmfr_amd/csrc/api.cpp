@@ -192,7 +192,7 @@ struct bmfr_stream_s {
     hipEvent_t sync_ev = nullptr;   // orders this stream against batch launches
     bool recorded[kStages]{};
     bmfr::FitterConfig fitter{16, 2};  // one wave per block, two blocks per work-group
-    int fuse = BMFR_FUSE_34;   // how bmfr_run_frame runs stages 3-5 (bmfr_fuse_level)
+    int fuse = BMFR_FUSE_345;  // how bmfr_run_frame runs stages 3-5 (bmfr_fuse_level)
     // Renderer handoff: the current frame's caller-bound inputs (albedo,
     // normals, positions, noisy; null: the stream's own buffers), the
     // previous frame's normals and positions when they were caller-bound

@@ -36,11 +36,11 @@ __device__ __forceinline__ Stage1Pixel stage1_pixel(const StageArgs &a, const St
     const int pwx = gx - kBlock / 2 + off.x;
     const int pwy = gy - kBlock / 2 + off.y;
     const int px = mirror(pwx, W), py = mirror(pwy, H);
-    const int64_t lp = (int64_t)py * W + px;
+    const uint32_t lp = (uint32_t)(py * W + px);
 
-    const float3 wp = load3(cur_positions, lp);
-    const float3 nrm = load3(cur_normals, lp);
-    const float3 cur = load3(cur_noisy, lp);
+    const float3 wp = load3_px(cur_positions, lp);
+    const float3 nrm = load3_px(cur_normals, lp);
+    const float3 cur = load3_px(cur_noisy, lp);
 
     float ppf_x = (float)px, ppf_y = (float)py;
     uint32_t store_accept = 0;
@@ -74,22 +74,22 @@ __device__ __forceinline__ Stage1Pixel stage1_pixel(const StageArgs &a, const St
         // texel and is never used.  The tests and sums below keep the
         // reference's order (bmfr.cl:374-419).
         bool tap_in[4];
-        int64_t tap_ls[4];
+        uint32_t tap_ls[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int sx = wrap_add(ipx, i & 1);
             const int sy = wrap_add(ipy, i >> 1);
             tap_in[i] = sx >= 0 && sy >= 0 && sx < W && sy < H;
-            tap_ls[i] = tap_in[i] ? (int64_t)wrap_mad(sy, W, sx) : lp;
+            tap_ls[i] = tap_in[i] ? (uint32_t)wrap_mad(sy, W, sx) : lp;
         }
         float3 tpp[4], tpn[4], tpc[4];
         float tsp[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            tpp[i] = load3(prev_positions, tap_ls[i]);
-            tpn[i] = load3(prev_normals, tap_ls[i]);
-            tpc[i] = load3(prev_noisy, tap_ls[i]);
-            tsp[i] = (float)gload(prev_spp, tap_ls[i]);
+            tpp[i] = load3_px(prev_positions, tap_ls[i]);
+            tpn[i] = load3_px(prev_normals, tap_ls[i]);
+            tpc[i] = load3_px(prev_noisy, tap_ls[i]);
+            tsp[i] = (float)gload_px(prev_spp, tap_ls[i]);
         }
         // The tests are evaluated for every tap and the sums selected rather
         // than branched over (the same values, in the reference's order):
@@ -143,10 +143,10 @@ __device__ __forceinline__ Stage1Pixel stage1_pixel(const StageArgs &a, const St
     // bmfr.cl:478-484 (and the spp of bmfr.cl:442, which mirrored
     // work-items would only rewrite with the same value)
     if (pwx >= 0 && pwx < W && pwy >= 0 && pwy < H) {
-        store3(acc_noisy, lp, nc);
-        gstore(out_prev_pixel, lp, make_float2(ppf_x, ppf_y));
-        gstore(accept_bools, lp, (uint8_t)store_accept);
-        gstore(cur_spp, lp, (uint8_t)new_spp);
+        store3_px(acc_noisy, lp, nc);
+        gstore_px(out_prev_pixel, lp, make_float2(ppf_x, ppf_y));
+        gstore_px(accept_bools, lp, (uint8_t)store_accept);
+        gstore_px(cur_spp, lp, (uint8_t)new_spp);
     }
     return Stage1Pixel{wp, nrm, nc};
 }

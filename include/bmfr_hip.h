@@ -285,11 +285,10 @@ int32_t bmfr_end_frame(bmfr_stream stream);
  * separate entry points always run the five separate kernels. */
 typedef enum bmfr_fuse_level {
     BMFR_FUSE_NONE = 0,  /* weighted_sum, accumulate_filtered_data, taa */
-    BMFR_FUSE_34 = 1,    /* weighted_sum inside accumulate_filtered_data, then taa
-                            (default of a stream) */
+    BMFR_FUSE_34 = 1,    /* weighted_sum inside accumulate_filtered_data, then taa */
     BMFR_FUSE_345 = 2    /* all three in one kernel per 32x32 block, with the
-                            one-pixel TAA ring recomputed (default of batches
-                            and server steps) */
+                            one-pixel TAA ring recomputed (the default of
+                            streams, batches and server steps) */
 } bmfr_fuse_level;
 int32_t bmfr_set_fused(bmfr_stream stream, int32_t level);
 /* All five stages in order, then bmfr_end_frame (opencl/bmfr.cpp:429-484). */

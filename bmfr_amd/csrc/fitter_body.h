@@ -662,13 +662,14 @@ __device__ __forceinline__ void fast_pivot(FastRows<RPL, TT, HALF_ROUND, F + 3> 
     const float uc = xc - norm;
     const float uu = sigma + uc * uc;
     norms[COL] = norm;
-    const float inv_uu = __builtin_amdgcn_rcpf(uu);
+    // (2 d) / uu as d * (2 / uu): scaling by 2 is exact, so the same bits
+    const float inv2_uu = 2.f * __builtin_amdgcn_rcpf(uu);
     const float u_kc = t == COL / V ? uc : x[KC];
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
         const int j = COL + 1 + i;
         const float d = v[1 + i] + uc * e[1 + i];
-        const float fct = (2.f * d) * inv_uu;
+        const float fct = d * inv2_uu;
 #pragma unroll
         for (int k = 0; k < RPL; ++k) {
             const float u = k == KC ? u_kc : x[k];
@@ -745,16 +746,18 @@ __device__ __forceinline__ void fitter_block_fast(const TT *__restrict__ src, fl
         // Markstein's correctly rounded quotient: y = RN(1/d) once per
         // feature, then q = RN(a*y), r = RN(a - q*d) (exact),
         // q' = RN(q + r*y) = RN(a/d).  Without the division
-        // (|max - min| <= 1) the same three operations with d = y = 1 return
-        // the numerator exactly.
+        // (|max - min| <= 1, the same for the whole block) the numerator.
 #pragma unroll
         for (int i = 0; i < kScaled; ++i) {
             const float dd = bmax[i] - bmin[i];
-            const bool divide = fabsf(dd) > 1.0f;
-            const float d = divide ? dd : 1.f;
-            const float y = divide ? 1.0f / dd : 1.f;
+            if (fabsf(dd) > 1.0f) {
+                const float y = 1.0f / dd;
 #pragma unroll
-            for (int k = 0; k < RPL; ++k) m.put(kNotScaled + i, k, div_by(m.a[kNotScaled + i][k] - bmin[i], d, y));
+                for (int k = 0; k < RPL; ++k) m.put(kNotScaled + i, k, div_by(m.a[kNotScaled + i][k] - bmin[i], dd, y));
+            } else {
+#pragma unroll
+                for (int k = 0; k < RPL; ++k) m.put(kNotScaled + i, k, m.a[kNotScaled + i][k] - bmin[i]);
+            }
         }
     } else {
         // every feature's bounds are reduced (one round); those of the scaled

@@ -275,4 +275,14 @@ __device__ __forceinline__ float round_half(float v) {
     return __half2float(__float2half_rn(v));
 }
 
+// round_half of a value rounded to float first: a float result stored with
+// vstore_half.  Without the empty asm the compiler may fold the float
+// operation that produced v into its binary16 conversion (v_fma_mixlo_f16
+// and the like), which rounds the exact result to binary16 once: a different
+// binary16 value for about 1 in 16,000 FMAs.
+__device__ __forceinline__ float round_half_of_float(float v) {
+    asm("" : "+v"(v));
+    return round_half(v);
+}
+
 }  // namespace bmfr

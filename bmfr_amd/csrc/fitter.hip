@@ -73,7 +73,7 @@ __device__ __forceinline__ void refactor_block(const TT *__restrict__ src, const
                     const float r = __builtin_fmaf(-q, d, sc);
                     sc = __builtin_fmaf(r, y, q);
                 }
-                v = HALF_ROUND ? round_half(sc) : sc;
+                v = HALF_ROUND ? round_half_of_float(sc) : sc;
             }
             if (f >= 1 && f < kFeatures)
                 v = (float)((double)v + noise.term((double)noise.r[(f - 1) * kBlockPixels + row]));

@@ -368,7 +368,7 @@ __device__ __forceinline__ void fitter_block(const TT *__restrict__ src, float *
                 const float r = __builtin_fmaf(-q, d, num);
                 sc = __builtin_fmaf(r, y, q);
             }
-            a[kNotScaled + i][k] = HALF_ROUND ? round_half(sc) : sc;
+            a[kNotScaled + i][k] = HALF_ROUND ? round_half_of_float(sc) : sc;
         }
     }
     // ---- feature noise of the first pass, bmfr.cl:623-627: value + noise in
@@ -610,7 +610,12 @@ template <int RPL, typename TT, bool HALF_ROUND, int C = kBufferCount>
 struct FastRows {
     using L = FastLayout<RPL, TT>;
     float a[C][RPL];
+    // A Householder update (the compiler may round its FMA to binary16 once).
     __device__ __forceinline__ void put(int f, int k, float v) { a[f][k] = HALF_ROUND ? round_half(v) : v; }
+    // A scaled feature: the float quotient, then binary16 (bmfr.cl:540).
+    __device__ __forceinline__ void put_scaled(int f, int k, float v) {
+        a[f][k] = HALF_ROUND ? round_half_of_float(v) : v;
+    }
     // Row r (< 64 V) of column f, wave-uniform; valid in wave 0.
     __device__ __forceinline__ float pivot_row(int f, int r) const { return readlane(a[f][r % L::V], r / L::V); }
 };
@@ -753,10 +758,11 @@ __device__ __forceinline__ void fitter_block_fast(const TT *__restrict__ src, fl
             if (fabsf(dd) > 1.0f) {
                 const float y = 1.0f / dd;
 #pragma unroll
-                for (int k = 0; k < RPL; ++k) m.put(kNotScaled + i, k, div_by(m.a[kNotScaled + i][k] - bmin[i], dd, y));
+                for (int k = 0; k < RPL; ++k)
+                    m.put_scaled(kNotScaled + i, k, div_by(m.a[kNotScaled + i][k] - bmin[i], dd, y));
             } else {
 #pragma unroll
-                for (int k = 0; k < RPL; ++k) m.put(kNotScaled + i, k, m.a[kNotScaled + i][k] - bmin[i]);
+                for (int k = 0; k < RPL; ++k) m.put_scaled(kNotScaled + i, k, m.a[kNotScaled + i][k] - bmin[i]);
             }
         }
     } else {
@@ -784,11 +790,14 @@ __device__ __forceinline__ void fitter_block_fast(const TT *__restrict__ src, fl
                 mins_maxs[2 * (f - not_scaled) + 1] = bmax;
             }
             const float dd = bmax - bmin;
-            const bool divide = fabsf(dd) > 1.0f;
-            const float d = divide ? dd : 1.f;
-            const float y = divide ? 1.0f / dd : 1.f;
+            if (fabsf(dd) > 1.0f) {
+                const float y = 1.0f / dd;
 #pragma unroll
-            for (int k = 0; k < RPL; ++k) m.put(f, k, div_by(m.a[f][k] - bmin, d, y));
+                for (int k = 0; k < RPL; ++k) m.put_scaled(f, k, div_by(m.a[f][k] - bmin, dd, y));
+            } else {
+#pragma unroll
+                for (int k = 0; k < RPL; ++k) m.put_scaled(f, k, m.a[f][k] - bmin);
+            }
         }
     }
     // ---- feature noise of the first pass (bmfr.cl:623-627): the value plus

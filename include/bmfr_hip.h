@@ -176,10 +176,11 @@ int32_t bmfr_stream_reset(bmfr_stream stream);
 
 /* Fitter launch shape: rows_per_lane 4, 8 or 16 (a 32x32 block spread over
  * 4, 2 or 1 wavefronts) and blocks_per_group 1, 2 or 4 (supported pairs:
- * 4x1, 4x2, 8x1, 8x2, 16x1, 16x2, 16x4; default 16x2, one wavefront per
- * block and two blocks per work-group, so that a frame with fewer blocks
- * than SIMDs gets one block per SIMD).  A stream with a configured feature
- * set always runs 16x2.  No counterpart in the reference, whose fitter is
+ * 4x1, 4x2, 8x1, 8x2, 16x1, 16x2, 16x4; default 16x4, one wavefront per
+ * block and four blocks per work-group, so that a frame with fewer blocks
+ * than SIMDs gets one block per SIMD).  Shapes with the same rows_per_lane
+ * give the same bits.  A stream with a configured feature set always runs
+ * 16x2.  No counterpart in the reference, whose fitter is
  * always one 256-wide work-group per block (opencl/bmfr.cpp:117-118, 249). */
 int32_t bmfr_set_fitter_config(bmfr_stream stream, int32_t rows_per_lane, int32_t blocks_per_group);
 

@@ -145,18 +145,40 @@ template <typename T>
 __device__ __forceinline__ void gstore_px(T *p, uint32_t px, T v) {
     gstore(px_ptr(p, px * (uint32_t)sizeof(T)), 0, v);
 }
+// p * 3 as one full-rate v_lshl_add_u32 ((p << 1) + p).  Written as asm
+// because the compiler turns the shift-add back into a multiply, and a
+// 32-bit v_mul_lo_u32 issues at a quarter of the rate.
+__device__ __forceinline__ uint32_t times3(uint32_t p) {
+    uint32_t r;
+    asm("v_lshl_add_u32 %0, %1, 1, %1" : "=v"(r) : "v"(p));
+    return r;
+}
+// Byte offset of pixel `pixel` in a three-element-per-pixel buffer of T.
 template <typename T>
-__device__ __forceinline__ float3 load3_px(const T *p, uint32_t pixel) {
-    const T *q = px_ptr(p, pixel * (uint32_t)(3 * sizeof(T)));
+__device__ __forceinline__ uint32_t px3_bytes(uint32_t pixel) {
+    return times3(pixel) * (uint32_t)sizeof(T);
+}
+template <typename T>
+__device__ __forceinline__ float3 load3_at(const T *p, uint32_t byte_offset) {
+    const T *q = px_ptr(p, byte_offset);
     return make_float3(Store<T>::ld(q, 0), Store<T>::ld(q, 1), Store<T>::ld(q, 2));
 }
 template <typename T>
-__device__ __forceinline__ void store3_px(T *p, uint32_t pixel, float3 v) {
-    T *q = px_ptr(p, pixel * (uint32_t)(3 * sizeof(T)));
+__device__ __forceinline__ void store3_at(T *p, uint32_t byte_offset, float3 v) {
+    T *q = px_ptr(p, byte_offset);
     Store<T>::st(q, 0, v.x);
     Store<T>::st(q, 1, v.y);
     Store<T>::st(q, 2, v.z);
 }
+template <typename T>
+__device__ __forceinline__ float3 load3_px(const T *p, uint32_t pixel) {
+    return load3_at(p, px3_bytes<T>(pixel));
+}
+template <typename T>
+__device__ __forceinline__ void store3_px(T *p, uint32_t pixel, float3 v) {
+    store3_at(p, px3_bytes<T>(pixel), v);
+}
+
 
 // convert_int_rtn (opencl/bmfr.cl:356, 794, 879) with an explicitly
 // saturating conversion, NaN -> 0 (what v_cvt_i32_f32 does); written out so
@@ -175,6 +197,33 @@ __device__ __forceinline__ int wrap_add(int a, int b) {
 }
 __device__ __forceinline__ int wrap_mad(int a, int b, int c) {
     return (int)((unsigned)a * (unsigned)b + (unsigned)c);
+}
+
+// The 2x2 bilinear footprint at a reprojected position pf (bmfr.cl:356-360,
+// 794-798, 879-890): top-left texel (ix, iy) = floor(pf) and, per texel i
+// (ix + (i & 1), iy + (i >> 1)), whether it lies in the W x H image and its
+// pixel index (meaningful only then).  The row product runs on the 24-bit
+// multiplier: ix and iy are clamped to [-2, W] x [-2, H] first, which moves
+// no texel into or out of the image and bounds |iy * W| by 2^28.
+struct Footprint {
+    int ix, iy;          // floor(pf), saturated (cvt_rtn)
+    uint32_t idx0;       // pixel index of texel 0 (wrapping when outside)
+    uint32_t in_image;   // bit i: texel i in the image
+};
+__device__ __forceinline__ Footprint footprint(float2 pf, int W, int H) {
+    Footprint f;
+    f.ix = cvt_rtn(pf.x);
+    f.iy = cvt_rtn(pf.y);
+    const int cx = min(max(f.ix, -2), W), cy = min(max(f.iy, -2), H);
+    f.idx0 = (uint32_t)__mul24(cy, W) + (uint32_t)cx;
+    const bool x0 = cx >= 0 && cx < W, x1 = cx + 1 >= 0 && cx + 1 < W;
+    const bool y0 = cy >= 0 && cy < H, y1 = cy + 1 >= 0 && cy + 1 < H;
+    f.in_image = (x0 && y0 ? 1u : 0u) | (x1 && y0 ? 2u : 0u) | (x0 && y1 ? 4u : 0u) | (x1 && y1 ? 8u : 0u);
+    return f;
+}
+// Pixel index of texel i of a footprint.
+__device__ __forceinline__ uint32_t footprint_idx(const Footprint &f, int i, int W) {
+    return f.idx0 + (uint32_t)(i & 1) + ((i >> 1) ? (uint32_t)W : 0u);
 }
 
 // opencl/bmfr.cl:162-171.  The hash argument of the feature noise

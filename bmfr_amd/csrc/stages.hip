@@ -297,43 +297,51 @@ __device__ __forceinline__ Stage4Inputs stage4_inputs(const StageArgs &a, const 
     return in;
 }
 
+// Byte offsets of the four texels of footprint `fp` in a three-element
+// buffer of T, each replaced by the pixel's own offset `own` where `use`
+// has no bit (so that no load sits behind a branch).
+template <typename T>
+__device__ __forceinline__ void footprint_offsets(const Footprint &fp, uint32_t use, uint32_t own, int W,
+                                                  uint32_t (&off)[4]) {
+    const uint32_t t0 = px3_bytes<T>(fp.idx0);
+    const uint32_t dx = 3u * (uint32_t)sizeof(T), dy = (uint32_t)W * dx;
+    const uint32_t t[4] = {t0, t0 + dx, t0 + dy, t0 + dy + dx};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) off[i] = (use >> i) & 1u ? t[i] : own;
+}
+
 // The history taps of pixel lp (the second round trip, addressed by the
 // inputs).
 template <typename GT>
 __device__ __forceinline__ Stage4Gather<GT> stage4_taps(const StageArgs &a, const StageBuffers &bufs, int64_t lp,
-                                                        const Stage4Inputs &in) {
+                                                        const Stage4Inputs &in, const Footprint &fp) {
     const GT *__restrict__ acc_prev = (const GT *)bufs.out_prev;
-    const int W = a.width;
     Stage4Gather<GT> g;
-    const uint32_t acc = in.acc;
     const float2 pf = in.pf;
     g.pf = pf;
     g.spp = in.spp;
     g.albedo = in.albedo;
-    const int ix = cvt_rtn(pf.x), iy = cvt_rtn(pf.y);
-    const float fx = pf.x - (float)ix, fy = pf.y - (float)iy;
+    const float fx = pf.x - (float)fp.ix, fy = pf.y - (float)fp.iy;
     const float omx = 1.f - fx, omy = 1.f - fy;
     g.wts[0] = omx * omy;
     g.wts[1] = fx * omy;
     g.wts[2] = omx * fy;
     g.wts[3] = fx * fy;
-    g.use = 0;
+    // Stage 1 sets a bit only for an in-image tap at the same floor();
+    // requiring the texel to be in the image keeps inconsistent inputs (e.g.
+    // written through the debug API) from reading outside.
+    g.use = in.acc & fp.in_image;
+    uint32_t off[4];
+    footprint_offsets<GT>(fp, g.use, px3_bytes<GT>((uint32_t)lp), a.width, off);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int64_t ls = (int64_t)wrap_add(wrap_mad(wrap_add(iy, i >> 1), W, ix), i & 1);
-        // Stage 1 sets a bit only for an in-image tap at the same floor();
-        // the range check keeps inconsistent inputs (e.g. written through
-        // the debug API) from reading outside.
-        const bool use = (acc & (1u << i)) && ls >= 0 && ls < (int64_t)W * a.height;
-        g.use |= use ? 1u << i : 0u;
-        g.tap[i] = load3_px(acc_prev, (uint32_t)(use ? ls : lp));
-    }
+    for (int i = 0; i < 4; ++i) g.tap[i] = load3_at(acc_prev, off[i]);
     return g;
 }
 
 template <typename GT>
 __device__ __forceinline__ Stage4Gather<GT> stage4_gather(const StageArgs &a, const StageBuffers &bufs, int64_t lp) {
-    return stage4_taps<GT>(a, bufs, lp, stage4_inputs(a, bufs, lp));
+    const Stage4Inputs in = stage4_inputs(a, bufs, lp);
+    return stage4_taps<GT>(a, bufs, lp, in, footprint(in.pf, a.width, a.height));
 }
 
 // The accumulated colour of a pixel from its filtered colour `fc` and its
@@ -460,22 +468,21 @@ struct TaaHistory {
 
 template <typename GT>
 __device__ __forceinline__ TaaHistory<GT> taa_history(const StageArgs &a, const StageBuffers &bufs, int64_t lp,
-                                                      float2 pf) {
+                                                      const Footprint &fp) {
     const GT *__restrict__ prev_result = (const GT *)bufs.result_prev;
-    const int W = a.width, H = a.height;
     TaaHistory<GT> h;
-    h.ix = cvt_rtn(pf.x);
-    h.iy = cvt_rtn(pf.y);
-    const int ix = h.ix, iy = h.iy;
-    h.history = a.frame != 0 && ix >= -1 && iy >= -1 && ix < W && iy < H;
-    const int64_t r0 = (int64_t)iy * W, r1 = ((int64_t)iy + 1) * W;  // 64-bit: iy may saturate
-    h.use[0] = h.history && iy >= 0 && ix >= 0;
-    h.use[1] = h.history && iy >= 0 && ix < W - 1;
-    h.use[2] = h.history && iy < H - 1 && ix >= 0;
-    h.use[3] = h.history && iy < H - 1 && ix < W - 1;
-    const int64_t tap_ls[4] = {r0 + ix, r0 + ix + 1, r1 + ix, r1 + ix + 1};
+    h.ix = fp.ix;
+    h.iy = fp.iy;
+    // floor(prev_pixel) in [-1, W) x [-1, H) exactly when a texel of the
+    // footprint is in the image; each tap is used when its texel is
+    const uint32_t use = a.frame != 0 ? fp.in_image : 0u;
+    h.history = use != 0;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) h.tap[i] = load3_px(prev_result, (uint32_t)(h.use[i] ? tap_ls[i] : lp));
+    for (int i = 0; i < 4; ++i) h.use[i] = (use >> i) & 1u;
+    uint32_t off[4];
+    footprint_offsets<GT>(fp, use, px3_bytes<GT>((uint32_t)lp), a.width, off);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) h.tap[i] = load3_at(prev_result, off[i]);
     return h;
 }
 
@@ -600,7 +607,7 @@ __device__ __forceinline__ void stage5_body(const StageArgs &a, const StageBuffe
     const int64_t lp = inside ? (int64_t)y * W + x : 0;
     const float2 pf = gload_px(prev_pixel, (uint32_t)lp);
     const float3 mine = load3_px(new_frame, (uint32_t)lp);
-    const TaaHistory<GT> h = taa_history<GT>(a, bufs, lp, pf);
+    const TaaHistory<GT> h = taa_history<GT>(a, bufs, lp, footprint(pf, W, H));
     for (int e = threadIdx.x; e < kTaaTileW * kTaaTileH; e += 256) {
         const int ry = e / kTaaTileW, rx = e - ry * kTaaTileW;
         const int gx = x0 - 1 + rx, gy = y0 - 1 + ry;
@@ -681,7 +688,7 @@ __device__ __forceinline__ void stage345_body(const StageArgs &a, const StageBuf
     for (int r = 0; r < 4; ++r) {
         const int x = x0 + tx, y = y0 + ty + 8 * r;
         inside[r] = x >= 0 && y >= 0 && x < W && y < H;
-        lp[r] = (uint32_t)(min(max(y, 0), H - 1) * W + min(max(x, 0), W - 1));
+        lp[r] = (uint32_t)__umul24((unsigned)min(max(y, 0), H - 1), (unsigned)W) + (uint32_t)min(max(x, 0), W - 1);
     }
     // The one-pixel ring around the block (pixels of the neighbouring blocks,
     // whose weights and bounds are vector loads), one texel per thread of the
@@ -698,7 +705,7 @@ __device__ __forceinline__ void stage345_body(const StageArgs &a, const StageBuf
     const int rgx = x0 + rx, rgy = y0 + ry;
     const bool ring_inside = ring && rgx >= 0 && rgy >= 0 && rgx < W && rgy < H;
     const int rcx = min(max(rgx, 0), W - 1), rcy = min(max(rgy, 0), H - 1);
-    const uint32_t ring_lp = (uint32_t)(rcy * W + rcx);
+    const uint32_t ring_lp = (uint32_t)__umul24((unsigned)rcy, (unsigned)W) + (uint32_t)rcx;
 
     float3 mine[4], hist[4];
     uint32_t has_history = 0;   // bit r: pixel r has a TAA history (bmfr.cl:884-890)
@@ -709,8 +716,9 @@ __device__ __forceinline__ void stage345_body(const StageArgs &a, const StageBuf
     float3 ring_wp, ring_n;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        const Stage4Gather<GT> gth = stage4_taps<GT>(a, bufs, lp[r], in);
-        const TaaHistory<GT> h = taa_history<GT>(a, bufs, lp[r], in.pf);
+        const Footprint fp = footprint(in.pf, W, H);
+        const Stage4Gather<GT> gth = stage4_taps<GT>(a, bufs, lp[r], in, fp);
+        const TaaHistory<GT> h = taa_history<GT>(a, bufs, lp[r], fp);
         Stage4Inputs nin;
         float3 nwp, nn;
         if (r < 3) {
@@ -743,7 +751,7 @@ __device__ __forceinline__ void stage345_body(const StageArgs &a, const StageBuf
         // so that it stays a block of the frame
         const int64_t g = (int64_t)((rcy + kBlock / 2 - off.y) / kBlock) * a.blocks_x +
                           (rcx + kBlock / 2 - off.x) / kBlock;
-        const Stage4Gather<GT> gth = stage4_taps<GT>(a, bufs, ring_lp, ring_in);
+        const Stage4Gather<GT> gth = stage4_taps<GT>(a, bufs, ring_lp, ring_in, footprint(ring_in.pf, W, H));
         const float3 ac = stage4_blend<GT>(a, stage3_color_of<GT, CUSTOM>(a, bufs, ring_wp, ring_n, g), gth);
         if (ring_inside) t = ycocg_texel(tone_map(gth.albedo, ac, tt));
         tile[ry + 1][rx + 1] = t;

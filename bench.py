@@ -28,9 +28,9 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # 256 CUs x 4 SIMDs at 2.4 GHz, one wave64 VALU instruction per SIMD every 4 cycles
 VALU_ISSUE_PEAK = 1024 * 2.4e9 / 4 / 1e9
-# SQ_INSTS_VALU / SQ_WAVES of the fitter (k_fitter<16, 2, __half, true, false>),
+# SQ_INSTS_VALU / SQ_WAVES of the fitter (k_fitter<16, 4, __half, true, false>),
 # profiles/fitter_720p_counters.md
-FITTER_VALU_PER_WAVE = 6134
+FITTER_VALU_PER_WAVE = 5962
 
 
 def reference_fitter_lds_bytes():
@@ -479,9 +479,9 @@ def main():
             "frac_back_to_back": round(sb[dom] / (fitter_b2b_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     # The fitter's own limit is instruction issue: its code is straight-line,
     # FITTER_VALU_PER_WAVE instructions per wave (one wave per block, the
-    # launch padded to whole work-groups per XCD), against one wave64 VALU
-    # instruction per 4 cycles per SIMD.
-    waves = 8 * (((nB + 7) // 8 + 1) // 2) * 2
+    # launch padded to whole four-block work-groups per XCD), against one
+    # wave64 VALU instruction per 4 cycles per SIMD.
+    waves = 8 * (((nB + 7) // 8 + 3) // 4) * 4
     issue = waves * FITTER_VALU_PER_WAVE / (stage_ms["fitter"] * 1e-3) / 1e9
     roof["issue"] = {"bound": "valu", "achieved": round(issue, 1), "peak": VALU_ISSUE_PEAK,
                      "unit": "G wave-instructions/s", "frac": round(issue / VALU_ISSUE_PEAK, 4),

@@ -200,30 +200,34 @@ __device__ __forceinline__ int wrap_mad(int a, int b, int c) {
 }
 
 // The 2x2 bilinear footprint at a reprojected position pf (bmfr.cl:356-360,
-// 794-798, 879-890): top-left texel (ix, iy) = floor(pf) and, per texel i
-// (ix + (i & 1), iy + (i >> 1)), whether it lies in the W x H image and its
-// pixel index (meaningful only then).  The row product runs on the 24-bit
-// multiplier: ix and iy are clamped to [-2, W] x [-2, H] first, which moves
-// no texel into or out of the image and bounds |iy * W| by 2^28.
+// 794-798, 879-890): top-left texel (ix, iy) = floor(pf); per texel i
+// (ix + (i & 1), iy + (i >> 1)) whether it lies in the W x H image, and the
+// pixel index of that texel clamped into the image, which is the texel
+// itself when it lies inside and otherwise some valid address whose value
+// is not used (so that no load needs a select or a branch).  Row products
+// run on the 24-bit multiplier (clamped rows < H <= 16384, W <= 16384).
 struct Footprint {
     int ix, iy;          // floor(pf), saturated (cvt_rtn)
-    uint32_t idx0;       // pixel index of texel 0 (wrapping when outside)
+    uint32_t idx[4];     // pixel index of texel i clamped into the image
     uint32_t in_image;   // bit i: texel i in the image
 };
 __device__ __forceinline__ Footprint footprint(float2 pf, int W, int H) {
     Footprint f;
     f.ix = cvt_rtn(pf.x);
     f.iy = cvt_rtn(pf.y);
-    const int cx = min(max(f.ix, -2), W), cy = min(max(f.iy, -2), H);
-    f.idx0 = (uint32_t)__mul24(cy, W) + (uint32_t)cx;
-    const bool x0 = cx >= 0 && cx < W, x1 = cx + 1 >= 0 && cx + 1 < W;
-    const bool y0 = cy >= 0 && cy < H, y1 = cy + 1 >= 0 && cy + 1 < H;
-    f.in_image = (x0 && y0 ? 1u : 0u) | (x1 && y0 ? 2u : 0u) | (x0 && y1 ? 4u : 0u) | (x1 && y1 ? 8u : 0u);
+    const int ix1 = wrap_add(f.ix, 1), iy1 = wrap_add(f.iy, 1);
+    const uint32_t cx0 = (uint32_t)min(max(f.ix, 0), W - 1), cx1 = (uint32_t)min(max(ix1, 0), W - 1);
+    const uint32_t r0 = __umul24((unsigned)min(max(f.iy, 0), H - 1), (unsigned)W);
+    const uint32_t r1 = __umul24((unsigned)min(max(iy1, 0), H - 1), (unsigned)W);
+    f.idx[0] = r0 + cx0;
+    f.idx[1] = r0 + cx1;
+    f.idx[2] = r1 + cx0;
+    f.idx[3] = r1 + cx1;
+    // bits 0 and 2 are column ix, 1 and 3 column ix + 1; bits 0, 1 row iy
+    const uint32_t xm = ((uint32_t)f.ix < (uint32_t)W ? 5u : 0u) | ((uint32_t)ix1 < (uint32_t)W ? 10u : 0u);
+    const uint32_t ym = ((uint32_t)f.iy < (uint32_t)H ? 3u : 0u) | ((uint32_t)iy1 < (uint32_t)H ? 12u : 0u);
+    f.in_image = xm & ym;
     return f;
-}
-// Pixel index of texel i of a footprint.
-__device__ __forceinline__ uint32_t footprint_idx(const Footprint &f, int i, int W) {
-    return f.idx0 + (uint32_t)(i & 1) + ((i >> 1) ? (uint32_t)W : 0u);
 }
 
 // opencl/bmfr.cl:162-171.  The hash argument of the feature noise

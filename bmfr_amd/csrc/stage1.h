@@ -36,7 +36,7 @@ __device__ __forceinline__ Stage1Pixel stage1_pixel(const StageArgs &a, const St
     const int pwx = gx - kBlock / 2 + off.x;
     const int pwy = gy - kBlock / 2 + off.y;
     const int px = mirror(pwx, W), py = mirror(pwy, H);
-    const uint32_t lp = (uint32_t)(py * W + px);
+    const uint32_t lp = __umul24((unsigned)py, (unsigned)W) + (unsigned)px;
 
     const float3 wp = load3_px(cur_positions, lp);
     const float3 nrm = load3_px(cur_normals, lp);
@@ -64,23 +64,22 @@ __device__ __forceinline__ Stage1Pixel stage1_pixel(const StageArgs &a, const St
         ppf_y = uvy * (float)H;
         ppf_x = ppf_x - a.pixel_offset_x;
         ppf_y = ppf_y - (1.f - a.pixel_offset_y);
-        const int ipx = cvt_rtn(ppf_x), ipy = cvt_rtn(ppf_y);
+        const Footprint fp = footprint(make_float2(ppf_x, ppf_y), W, H);
+        const int ipx = fp.ix, ipy = fp.iy;
         const float fx = ppf_x - (float)ipx, fy = ppf_y - (float)ipy;
         const float omx = 1.f - fx, omy = 1.f - fy;
         const float wts[4] = {omx * omy, fx * omy, omx * fy, fx * fy};
         // All four taps' history is loaded up front (one memory round trip
         // instead of three dependent ones: position, then normal, then
-        // colour and spp); a tap outside the image reads the pixel's own
-        // texel and is never used.  The tests and sums below keep the
-        // reference's order (bmfr.cl:374-419).
+        // colour and spp); a tap outside the image reads the texel clamped
+        // into the image and is never used.  The tests and sums below keep
+        // the reference's order (bmfr.cl:374-419).
         bool tap_in[4];
         uint32_t tap_ls[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int sx = wrap_add(ipx, i & 1);
-            const int sy = wrap_add(ipy, i >> 1);
-            tap_in[i] = sx >= 0 && sy >= 0 && sx < W && sy < H;
-            tap_ls[i] = tap_in[i] ? (uint32_t)wrap_mad(sy, W, sx) : lp;
+            tap_in[i] = (fp.in_image >> i) & 1u;
+            tap_ls[i] = fp.idx[i];
         }
         float3 tpp[4], tpn[4], tpc[4];
         float tsp[4];

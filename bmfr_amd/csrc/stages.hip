@@ -247,7 +247,7 @@ __device__ __forceinline__ void stage3_body(const StageArgs &a, const StageBuffe
     ScaleRecips rec;
     if constexpr (!CUSTOM) rec = scale_recips(bufs, g);   // all lanes active; g is the work-group's
     if (x < 0 || y < 0 || x >= a.width || y >= a.height) return;
-    const int64_t lp = (int64_t)y * a.width + x;
+    const int64_t lp = __umul24((unsigned)y, (unsigned)a.width) + (unsigned)x;
     const float3 c = stage3_color<GT, CUSTOM>(a, bufs, lp, g, CUSTOM ? nullptr : &rec);
     float *__restrict__ output = (float *)bufs.filtered;
     store3_px(output, (uint32_t)lp, c);
@@ -298,16 +298,11 @@ __device__ __forceinline__ Stage4Inputs stage4_inputs(const StageArgs &a, const 
 }
 
 // Byte offsets of the four texels of footprint `fp` in a three-element
-// buffer of T, each replaced by the pixel's own offset `own` where `use`
-// has no bit (so that no load sits behind a branch).
+// buffer of T.
 template <typename T>
-__device__ __forceinline__ void footprint_offsets(const Footprint &fp, uint32_t use, uint32_t own, int W,
-                                                  uint32_t (&off)[4]) {
-    const uint32_t t0 = px3_bytes<T>(fp.idx0);
-    const uint32_t dx = 3u * (uint32_t)sizeof(T), dy = (uint32_t)W * dx;
-    const uint32_t t[4] = {t0, t0 + dx, t0 + dy, t0 + dy + dx};
+__device__ __forceinline__ void footprint_offsets(const Footprint &fp, uint32_t (&off)[4]) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) off[i] = (use >> i) & 1u ? t[i] : own;
+    for (int i = 0; i < 4; ++i) off[i] = px3_bytes<T>(fp.idx[i]);
 }
 
 // The history taps of pixel lp (the second round trip, addressed by the
@@ -332,7 +327,7 @@ __device__ __forceinline__ Stage4Gather<GT> stage4_taps(const StageArgs &a, cons
     // written through the debug API) from reading outside.
     g.use = in.acc & fp.in_image;
     uint32_t off[4];
-    footprint_offsets<GT>(fp, g.use, px3_bytes<GT>((uint32_t)lp), a.width, off);
+    footprint_offsets<GT>(fp, off);
 #pragma unroll
     for (int i = 0; i < 4; ++i) g.tap[i] = load3_at(acc_prev, off[i]);
     return g;
@@ -397,7 +392,7 @@ __device__ __forceinline__ void stage4_finish(const StageArgs &a, const StageBuf
 template <typename GT>
 __device__ __forceinline__ void stage4_body(const StageArgs &a, const StageBuffers &bufs, int x, int y) {
     if (x >= a.width || y >= a.height) return;
-    const int64_t lp = (int64_t)y * a.width + x;
+    const int64_t lp = __umul24((unsigned)y, (unsigned)a.width) + (unsigned)x;
     const float *__restrict__ filtered = (const float *)bufs.filtered;
     const Stage4Gather<GT> gth = stage4_gather<GT>(a, bufs, lp);
     stage4_finish<GT>(a, bufs, lp,
@@ -416,7 +411,7 @@ __device__ __forceinline__ void stage34_body(const StageArgs &a, const StageBuff
     ScaleRecips rec;
     if constexpr (!CUSTOM) rec = scale_recips(bufs, g);   // all lanes active; g is the work-group's
     if (x < 0 || y < 0 || x >= a.width || y >= a.height) return;
-    const int64_t lp = (int64_t)y * a.width + x;
+    const int64_t lp = __umul24((unsigned)y, (unsigned)a.width) + (unsigned)x;
     const Stage4Gather<GT> gth = stage4_gather<GT>(a, bufs, lp);
     stage4_finish<GT>(a, bufs, lp, stage3_color<GT, CUSTOM>(a, bufs, lp, g, CUSTOM ? nullptr : &rec), gth);
 }
@@ -480,7 +475,7 @@ __device__ __forceinline__ TaaHistory<GT> taa_history(const StageArgs &a, const 
 #pragma unroll
     for (int i = 0; i < 4; ++i) h.use[i] = (use >> i) & 1u;
     uint32_t off[4];
-    footprint_offsets<GT>(fp, use, px3_bytes<GT>((uint32_t)lp), a.width, off);
+    footprint_offsets<GT>(fp, off);
 #pragma unroll
     for (int i = 0; i < 4; ++i) h.tap[i] = load3_at(prev_result, off[i]);
     return h;
@@ -604,7 +599,7 @@ __device__ __forceinline__ void stage5_body(const StageArgs &a, const StageBuffe
     // the pixel's own loads and its history taps are issued before the
     // tile's, so that they are in flight during the tile load and the barrier
     const bool inside = x < W && y < H;
-    const int64_t lp = inside ? (int64_t)y * W + x : 0;
+    const int64_t lp = inside ? __umul24((unsigned)y, (unsigned)W) + (unsigned)x : 0;
     const float2 pf = gload_px(prev_pixel, (uint32_t)lp);
     const float3 mine = load3_px(new_frame, (uint32_t)lp);
     const TaaHistory<GT> h = taa_history<GT>(a, bufs, lp, footprint(pf, W, H));
@@ -613,7 +608,7 @@ __device__ __forceinline__ void stage5_body(const StageArgs &a, const StageBuffe
         const int gx = x0 - 1 + rx, gy = y0 - 1 + ry;
         float4 t = make_float4(__builtin_nanf(""), __builtin_nanf(""), __builtin_nanf(""), 0.f);
         if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
-            const int64_t ls = (int64_t)gy * W + gx;
+            const int64_t ls = __umul24((unsigned)gy, (unsigned)W) + (unsigned)gx;
             t = ycocg_texel(load3_px(new_frame, (uint32_t)ls));
         }
         tile[ry][rx] = t;

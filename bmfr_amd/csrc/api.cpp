@@ -1022,7 +1022,7 @@ namespace {
 struct BatchGroup {
     bool g16 = false, t16 = false;
     int qr = 0;
-    int lane = 0;                        // software-pipeline lane (0 or 1)
+    int lane = 0;                        // software-pipeline lane (0 .. lanes - 1)
     std::vector<int> members;            // indices into bmfr_batch_s::streams
     bmfr::BatchStream *d_streams = nullptr;
     bmfr::BatchStep *d_steps = nullptr;
@@ -1052,9 +1052,12 @@ struct bmfr_batch_s {
     bmfr::FitterConfig fitter{16, 4};
     int grid_cap = 0;
     int fuse = BMFR_FUSE_345;
-    int lanes = 2;                        // 2: lane 1 runs one stage behind lane 0 on stream2
-    hipStream_t stream2 = nullptr;
-    hipEvent_t ev_s1 = nullptr, ev_join = nullptr;
+    // software-pipeline lanes: lane l runs on lane_stream[l] (lane 0 on
+    // `stream`), its stage 1 starting when lane l - 1's stage 1 is done
+    static constexpr int kMaxLanes = 4;
+    int lanes = 2;
+    hipStream_t lane_stream[kMaxLanes]{};
+    hipEvent_t ev_s1[kMaxLanes]{}, ev_join[kMaxLanes]{};
     bool use_graph = false;
     hipGraphExec_t graph = nullptr;
     // pinned staging of the per-step parameters, one slot per group and ring entry
@@ -1102,11 +1105,11 @@ int32_t batch_enqueue_stage(bmfr_batch b, int st, int lane, hipStream_t stream) 
 }
 
 // The five stages of a step.  Timed per stage: stage by stage over every
-// group on the batch's stream.  Otherwise, with two lanes, a two-deep
-// software pipeline: lane 0 runs on the batch's stream, lane 1 on stream2
-// starting when lane 0's stage 1 is done, so that each lane's fitter
-// (VALU-bound) overlaps the other lane's HBM-streaming stages; lane 1 joins
-// the batch's stream at the end.  The lanes' streams are disjoint.
+// group on the batch's stream.  Otherwise, with L > 1 lanes, an L-deep
+// software pipeline: lane 0 runs on the batch's stream, lane l on its own
+// HIP stream starting when lane l - 1's stage 1 is done, so that each lane's
+// fitter (VALU-bound) overlaps other lanes' HBM-streaming stages; the lanes
+// join the batch's stream at the end.  The lanes' streams are disjoint.
 int32_t batch_enqueue_stages(bmfr_batch b, bool timed) {
     if (timed || b->lanes == 1) {
         for (int st = 0; st < kStages; ++st) {
@@ -1116,17 +1119,97 @@ int32_t batch_enqueue_stages(bmfr_batch b, bool timed) {
         }
         return BMFR_OK;
     }
-    if (int32_t r = batch_enqueue_stage(b, 0, 0, b->stream)) return r;
-    HIP_TRY(hipEventRecord(b->ev_s1, b->stream));
-    HIP_TRY(hipStreamWaitEvent(b->stream2, b->ev_s1, 0));
-    if (int32_t r = batch_enqueue_stage(b, 0, 1, b->stream2)) return r;
-    for (int st = 1; st < kStages; ++st)
-        if (int32_t r = batch_enqueue_stage(b, st, 0, b->stream)) return r;
-    for (int st = 1; st < kStages; ++st)
-        if (int32_t r = batch_enqueue_stage(b, st, 1, b->stream2)) return r;
-    HIP_TRY(hipEventRecord(b->ev_join, b->stream2));
-    HIP_TRY(hipStreamWaitEvent(b->stream, b->ev_join, 0));
+    const int L = b->lanes;
+    for (int l = 0; l < L; ++l) {
+        if (l > 0) HIP_TRY(hipStreamWaitEvent(b->lane_stream[l], b->ev_s1[l - 1], 0));
+        if (int32_t r = batch_enqueue_stage(b, 0, l, b->lane_stream[l])) return r;
+        if (l + 1 < L) HIP_TRY(hipEventRecord(b->ev_s1[l], b->lane_stream[l]));
+    }
+    for (int l = 0; l < L; ++l)
+        for (int st = 1; st < kStages; ++st)
+            if (int32_t r = batch_enqueue_stage(b, st, l, b->lane_stream[l])) return r;
+    for (int l = 1; l < L; ++l) {
+        HIP_TRY(hipEventRecord(b->ev_join[l], b->lane_stream[l]));
+        HIP_TRY(hipStreamWaitEvent(b->stream, b->ev_join[l], 0));
+    }
     return BMFR_OK;
+}
+
+// Frees the groups' device tables and the pinned step staging.
+void free_groups(bmfr_batch b) {
+    for (auto &g : b->groups) {
+        if (g.d_streams) (void)hipFree(g.d_streams);
+        if (g.d_steps) (void)hipFree(g.d_steps);
+        if (g.d_starts) (void)hipFree(g.d_starts);
+    }
+    b->groups.clear();
+    if (b->h_steps) (void)hipHostFree(b->h_steps);
+    b->h_steps = nullptr;
+}
+
+// Splits the members into `lanes` pipeline lanes of about equal pixel count
+// (largest streams first, each to the lane with the fewest pixels) and into
+// launch groups by (storage, tmp storage, QR mode, lane), with each group's
+// device tables and the pinned staging of the per-step parameters.
+hipError_t build_groups(bmfr_batch b, int lanes) {
+    free_groups(b);
+    const int count = (int)b->streams.size();
+    std::vector<int> lane_of(count, 0), order(count);
+    for (int i = 0; i < count; ++i) order[i] = i;
+    auto px = [&](int i) { return (int64_t)b->streams[i]->p.width * b->streams[i]->p.height; };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return px(x) > px(y); });
+    int64_t lane_px[bmfr_batch_s::kMaxLanes] = {};
+    for (int i : order) {
+        int l = 0;
+        for (int k = 1; k < lanes; ++k)
+            if (lane_px[k] < lane_px[l]) l = k;
+        lane_of[i] = l;
+        lane_px[l] += px(i);
+    }
+    for (int i = 0; i < count; ++i) {
+        const bmfr_stream s = b->streams[i];
+        const bool g16 = s->p.storage == BMFR_STORAGE_F16;
+        const bool t16 = s->p.tmp_storage == BMFR_STORAGE_F16;
+        const int qr = s->p.qr_mode;
+        BatchGroup *grp = nullptr;
+        for (auto &g : b->groups)
+            if (g.g16 == g16 && g.t16 == t16 && g.qr == qr && g.lane == lane_of[i]) grp = &g;
+        if (!grp) {
+            b->groups.emplace_back();
+            grp = &b->groups.back();
+            grp->g16 = g16;
+            grp->t16 = t16;
+            grp->qr = qr;
+            grp->lane = lane_of[i];
+        }
+        grp->members.push_back(i);
+    }
+    hipError_t e = hipHostMalloc((void **)&b->h_steps, sizeof(bmfr::BatchStep) * bmfr_batch_s::kRing *
+                                                             b->groups.size() * bmfr::kMaxBatch);
+    for (auto &g : b->groups) {
+        if (e != hipSuccess) break;
+        const int n = (int)g.members.size();
+        std::vector<bmfr::BatchStream> hs(n);
+        for (int k = 0; k < 3; ++k) g.starts[k][0] = 0;
+        for (int m = 0; m < n; ++m) {
+            bmfr_stream s = b->streams[g.members[m]];
+            s->batch_stream(hs[m]);
+            const bmfr_sizes &z = s->sz;
+            g.starts[bmfr::kStartsPadded][m + 1] =
+                g.starts[bmfr::kStartsPadded][m] + (z.margin_width / 32) * (z.margin_height / 8);
+            g.starts[bmfr::kStartsImage][m + 1] =
+                g.starts[bmfr::kStartsImage][m] + ((s->p.width + 63) / 64) * ((s->p.height + 3) / 4);
+            g.starts[bmfr::kStartsBlocks][m + 1] = g.starts[bmfr::kStartsBlocks][m] + z.blocks;
+        }
+        e = hipMalloc((void **)&g.d_streams, sizeof(bmfr::BatchStream) * bmfr::kMaxBatch);
+        if (e == hipSuccess) e = hipMalloc((void **)&g.d_steps, sizeof(bmfr::BatchStep) * bmfr::kMaxBatch);
+        if (e == hipSuccess) e = hipMalloc((void **)&g.d_starts, sizeof(g.starts));
+        if (e == hipSuccess)
+            e = hipMemcpy(g.d_streams, hs.data(), sizeof(bmfr::BatchStream) * n, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(g.d_starts, g.starts, sizeof(g.starts), hipMemcpyHostToDevice);
+    }
+    b->lanes = lanes;
+    return e;
 }
 
 }  // namespace
@@ -1156,72 +1239,23 @@ int32_t bmfr_batch_create(const bmfr_stream *streams, int32_t count, bmfr_batch 
     b->streams.assign(streams, streams + count);
     for (bmfr_stream s : b->streams) ++s->batch_refs;
     b->device = streams[0]->device;
-    // pipeline lanes: largest streams first, each to the lane with fewer pixels
-    std::vector<int> lane_of(count, 0), order(count);
-    for (int i = 0; i < count; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-        return (int64_t)streams[x]->p.width * streams[x]->p.height > (int64_t)streams[y]->p.width * streams[y]->p.height;
-    });
-    int64_t lane_px[2] = {0, 0};
-    for (int i : order) {
-        const int l = lane_px[1] < lane_px[0] ? 1 : 0;
-        lane_of[i] = l;
-        lane_px[l] += (int64_t)streams[i]->p.width * streams[i]->p.height;
-    }
-    for (int i = 0; i < count; ++i) {
-        const bool g16 = streams[i]->p.storage == BMFR_STORAGE_F16;
-        const bool t16 = streams[i]->p.tmp_storage == BMFR_STORAGE_F16;
-        const int qr = streams[i]->p.qr_mode;
-        BatchGroup *grp = nullptr;
-        for (auto &g : b->groups)
-            if (g.g16 == g16 && g.t16 == t16 && g.qr == qr && g.lane == lane_of[i]) grp = &g;
-        if (!grp) {
-            b->groups.emplace_back();
-            grp = &b->groups.back();
-            grp->g16 = g16;
-            grp->t16 = t16;
-            grp->qr = qr;
-            grp->lane = lane_of[i];
-        }
-        grp->members.push_back(i);
-    }
     hipError_t e = hipSetDevice(b->device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream2, hipStreamNonBlocking);
+    b->lane_stream[0] = b->stream;
+    for (int l = 1; l < bmfr_batch_s::kMaxLanes && e == hipSuccess; ++l)
+        e = hipStreamCreateWithFlags(&b->lane_stream[l], hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_s1, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming);
+    for (int l = 0; l < bmfr_batch_s::kMaxLanes && e == hipSuccess; ++l) {
+        e = hipEventCreateWithFlags(&b->ev_s1[l], hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&b->ev_join[l], hipEventDisableTiming);
+    }
     for (int i = 0; i < kStages && e == hipSuccess; ++i) {
         e = hipEventCreate(&b->ev_start[i]);
         if (e == hipSuccess) e = hipEventCreate(&b->ev_end[i]);
     }
     for (int i = 0; i < bmfr_batch_s::kRing && e == hipSuccess; ++i)
         e = hipEventCreateWithFlags(&b->ring_ev[i], hipEventDisableTiming);
-    if (e == hipSuccess)
-        e = hipHostMalloc((void **)&b->h_steps, sizeof(bmfr::BatchStep) * bmfr_batch_s::kRing *
-                                                    b->groups.size() * bmfr::kMaxBatch);
-    for (auto &g : b->groups) {
-        if (e != hipSuccess) break;
-        const int n = (int)g.members.size();
-        std::vector<bmfr::BatchStream> hs(n);
-        for (int k = 0; k < 3; ++k) g.starts[k][0] = 0;
-        for (int m = 0; m < n; ++m) {
-            bmfr_stream s = b->streams[g.members[m]];
-            s->batch_stream(hs[m]);
-            const bmfr_sizes &z = s->sz;
-            g.starts[bmfr::kStartsPadded][m + 1] =
-                g.starts[bmfr::kStartsPadded][m] + (z.margin_width / 32) * (z.margin_height / 8);
-            g.starts[bmfr::kStartsImage][m + 1] =
-                g.starts[bmfr::kStartsImage][m] + ((s->p.width + 63) / 64) * ((s->p.height + 3) / 4);
-            g.starts[bmfr::kStartsBlocks][m + 1] = g.starts[bmfr::kStartsBlocks][m] + z.blocks;
-        }
-        e = hipMalloc((void **)&g.d_streams, sizeof(bmfr::BatchStream) * bmfr::kMaxBatch);
-        if (e == hipSuccess) e = hipMalloc((void **)&g.d_steps, sizeof(bmfr::BatchStep) * bmfr::kMaxBatch);
-        if (e == hipSuccess) e = hipMalloc((void **)&g.d_starts, sizeof(g.starts));
-        if (e == hipSuccess)
-            e = hipMemcpy(g.d_streams, hs.data(), sizeof(bmfr::BatchStream) * n, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(g.d_starts, g.starts, sizeof(g.starts), hipMemcpyHostToDevice);
-    }
+    if (e == hipSuccess) e = build_groups(b, std::min(2, count));
     if (e != hipSuccess) {
         bmfr_batch_destroy(b);
         return fail(BMFR_ERR_HIP, "batch set-up failed: %s", hipGetErrorString(e));
@@ -1234,14 +1268,11 @@ int32_t bmfr_batch_destroy(bmfr_batch b) {
     if (!b) return BMFR_OK;
     (void)hipSetDevice(b->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
-    if (b->stream2) (void)hipStreamSynchronize(b->stream2);
+    for (int l = 1; l < bmfr_batch_s::kMaxLanes; ++l)
+        if (b->lane_stream[l]) (void)hipStreamSynchronize(b->lane_stream[l]);
     for (bmfr_stream s : b->streams) --s->batch_refs;
     if (b->graph) (void)hipGraphExecDestroy(b->graph);
-    for (auto &g : b->groups) {
-        if (g.d_streams) (void)hipFree(g.d_streams);
-        if (g.d_steps) (void)hipFree(g.d_steps);
-        if (g.d_starts) (void)hipFree(g.d_starts);
-    }
+    free_groups(b);
     for (int i = 0; i < kStages; ++i) {
         if (b->ev_start[i]) (void)hipEventDestroy(b->ev_start[i]);
         if (b->ev_end[i]) (void)hipEventDestroy(b->ev_end[i]);
@@ -1249,11 +1280,13 @@ int32_t bmfr_batch_destroy(bmfr_batch b) {
     for (auto &ev : b->ring_ev)
         if (ev) (void)hipEventDestroy(ev);
     if (b->done) (void)hipEventDestroy(b->done);
-    if (b->ev_s1) (void)hipEventDestroy(b->ev_s1);
-    if (b->ev_join) (void)hipEventDestroy(b->ev_join);
-    if (b->h_steps) (void)hipHostFree(b->h_steps);
+    for (int l = 0; l < bmfr_batch_s::kMaxLanes; ++l) {
+        if (b->ev_s1[l]) (void)hipEventDestroy(b->ev_s1[l]);
+        if (b->ev_join[l]) (void)hipEventDestroy(b->ev_join[l]);
+    }
     if (b->stream) (void)hipStreamDestroy(b->stream);
-    if (b->stream2) (void)hipStreamDestroy(b->stream2);
+    for (int l = 1; l < bmfr_batch_s::kMaxLanes; ++l)
+        if (b->lane_stream[l]) (void)hipStreamDestroy(b->lane_stream[l]);
     delete b;
     return BMFR_OK;
 }
@@ -1272,12 +1305,19 @@ int32_t bmfr_batch_set_fused(bmfr_batch b, int32_t level) {
 
 int32_t bmfr_batch_set_lanes(bmfr_batch b, int32_t lanes) {
     if (!b) return fail(BMFR_ERR_INVALID_ARGUMENT, "batch is null");
-    if (lanes != 1 && lanes != 2) return fail(BMFR_ERR_INVALID_ARGUMENT, "lanes %d is not 1 or 2", lanes);
-    b->lanes = lanes;
+    if (lanes < 1 || lanes > bmfr_batch_s::kMaxLanes)
+        return fail(BMFR_ERR_INVALID_ARGUMENT, "lanes %d outside 1..%d", lanes, bmfr_batch_s::kMaxLanes);
+    lanes = std::min(lanes, (int)b->streams.size());
+    if (lanes == b->lanes) return BMFR_OK;
+    HIP_TRY(hipSetDevice(b->device));
+    // the groups' tables and staging are in use until the last step ends
+    HIP_TRY(hipStreamSynchronize(b->stream));
     if (b->graph) {
         (void)hipGraphExecDestroy(b->graph);
         b->graph = nullptr;
     }
+    hipError_t e = build_groups(b, lanes);
+    if (e != hipSuccess) return fail(BMFR_ERR_HIP, "batch regrouping failed: %s", hipGetErrorString(e));
     return BMFR_OK;
 }
 

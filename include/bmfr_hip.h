@@ -350,11 +350,13 @@ int32_t bmfr_batch_synchronize(bmfr_batch batch);
 int32_t bmfr_batch_set_graph(bmfr_batch batch, int32_t enable);
 /* BMFR_FUSE_34 or BMFR_FUSE_345 (default) for the batch's stages 3-5. */
 int32_t bmfr_batch_set_fused(bmfr_batch batch, int32_t level);
-/* 2 (default): the members are split into two lanes of about equal pixel
- * count and an untimed (or graph-replayed) step runs them as a two-deep
- * software pipeline on two HIP streams, each lane's fitter overlapping the
- * other lane's streaming stages; 1: stage by stage over all members.  A step
- * timed per stage always runs stage by stage. */
+/* 2 (default) to 4: the members are split into that many lanes of about
+ * equal pixel count and an untimed (or graph-replayed) step runs them as a
+ * software pipeline on one HIP stream per lane, lane l's stage 1 starting
+ * when lane l - 1's is done, so that each lane's fitter overlaps other
+ * lanes' streaming stages; 1: stage by stage over all members.  At most one
+ * lane per member.  A step timed per stage always runs stage by stage.
+ * Waits for the batch's last step when the lane count changes. */
 int32_t bmfr_batch_set_lanes(bmfr_batch batch, int32_t lanes);
 /* Fitter shape for the batch; grid_cap > 0 bounds the persistent grid. */
 int32_t bmfr_batch_set_fitter_config(bmfr_batch batch, int32_t rows_per_lane,

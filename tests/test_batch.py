@@ -44,13 +44,14 @@ def _buffers(st, fuse):
     return d
 
 
-def _run(mix, frames, batched, graph=False, eager_between=False, fuse=bmfr_amd.FUSE_345):
+def _run(mix, frames, batched, graph=False, eager_between=False, fuse=bmfr_amd.FUSE_345, lanes=2):
     sts, scs = _streams(mix), _scenes(mix)
     for st in sts:
         st.set_fused(fuse)
     b = bmfr_amd.Batch(sts) if batched else None
     if b:
         b.set_fused(fuse)
+        b.set_lanes(lanes)
     if graph:
         b.set_graph(True)
     try:
@@ -104,6 +105,44 @@ def test_batch_graph_replay_equals_eager(gpu):
     graph = _run(MIX[:3] + MIX[4:], 6, batched=True, graph=True, eager_between=True)
     for k, (a, m) in enumerate(zip(eager, graph)):
         for name in a:
+            assert _same(a[name], m[name]), (k, name)
+
+
+@pytest.mark.parametrize("lanes", [1, 3, 4])
+def test_batch_pipeline_lanes_equal_each_stream_alone(gpu, lanes):
+    """The software pipeline's lane count changes only the launch order:
+    graph-replayed steps with 1, 3 or 4 lanes give each stream's alone bits."""
+    alone = _run(MIX, 4, batched=False)
+    mixed = _run(MIX, 4, batched=True, graph=True, eager_between=True, lanes=lanes)
+    for k, (a, m) in enumerate(zip(alone, mixed)):
+        for name in a:
+            assert _same(a[name], m[name]), (lanes, k, name)
+
+
+def test_batch_lanes_change_between_steps(gpu):
+    """Regrouping the members (set_lanes between steps, after graph replays)
+    keeps every stream's bits."""
+    sts, scs = _streams(MIX), _scenes(MIX)
+    b = bmfr_amd.Batch(sts)
+    try:
+        b.set_graph(True)
+        for fr, lanes in enumerate([2, 4, 4, 1, 3, 2]):
+            b.set_lanes(lanes)
+            for st, sc in zip(sts, scs):
+                sc.render_to_stream(fr, st)
+            args = [sc.frame_args(fr) for sc in scs]
+            b.run([fr] * len(sts), np.concatenate([a[0] for a in args]), np.concatenate([a[1] for a in args]))
+        b.synchronize()
+        got = [_buffers(st, bmfr_amd.FUSE_345) for st in sts]
+        with pytest.raises(bmfr_amd.BmfrError):
+            b.set_lanes(5)
+    finally:
+        b.close()
+        for st in sts:
+            st.close()
+    alone = _run(MIX, 6, batched=False)
+    for k, (a, m) in enumerate(zip(alone, got)):
+        for name in m:
             assert _same(a[name], m[name]), (k, name)
 
 

@@ -326,9 +326,9 @@ def run_serve_sweep(counts, duration):
                            "--budget-gib", budget)
         out.append({k: r[k] for k in ("streams", "frames_arrived", "frames", "dropped", "aggregate_frames_per_s",
                                       "mpixel_per_s", "latency_ms_p50", "latency_ms_p99", "latency_ms_max",
-                                      "mean_streams_per_step", "step_gpu_ms_mean", "gpu_busy")})
+                                      "mean_streams_per_step", "step_gpu_ms_mean", "step_time_per_s")})
         progress(f"serve sweep: {n} streams, p99 {r['latency_ms_p99']} ms, {r['mean_streams_per_step']} "
-                 f"streams/step, {r['dropped']} of {r['frames_arrived']} frames dropped, busy {r['gpu_busy']}")
+                 f"streams/step, {r['dropped']} of {r['frames_arrived']} frames dropped, step time per s {r['step_time_per_s']}")
     return out
 
 

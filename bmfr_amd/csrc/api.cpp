@@ -218,6 +218,15 @@ struct bmfr_stream_s {
     bool closing = false;
     int64_t last_step_id = 0;   // the last server step the stream took part in
     int last_lane = 0;          // and the server lane it ran on
+    // Ordering between server steps and the stream's own HIP stream, kept
+    // lazily: ordered_step is the last step the HIP stream waits for (an API
+    // call on the stream, or closing it, adds the wait for later steps);
+    // user_work: work may have been enqueued on the HIP stream since the
+    // last step that was ordered after it; external: the caller holds the
+    // HIP stream (bmfr_stream_hip_stream), so every step is ordered both ways.
+    int64_t ordered_step = 0;
+    bool user_work = true;
+    bool external = false;
 
     int cur(int a_slot) const { return swapped ? a_slot : a_slot + 1; }
     int prev(int a_slot) const { return swapped ? a_slot + 1 : a_slot; }
@@ -366,8 +375,14 @@ struct bmfr_scene_s {
 
 namespace {
 
+int32_t order_after_server_steps(bmfr_stream s, bool external);
+
+// Every stream entry point starts here.  A server-owned stream's HIP stream
+// is first ordered after the server steps it took part in, and the next step
+// is ordered after what the call enqueues.
 int32_t check_stream(bmfr_stream s) {
     if (!s) return fail(BMFR_ERR_INVALID_ARGUMENT, "stream is null");
+    if (s->owner) return order_after_server_steps(s, false);
     return BMFR_OK;
 }
 
@@ -772,6 +787,9 @@ int32_t bmfr_bind_output_surface(bmfr_stream s, void *surface, int32_t format, i
 int32_t bmfr_stream_hip_stream(bmfr_stream s, void **hip_stream) {
     if (int32_t st = check_stream(s)) return st;
     if (!hip_stream) return fail(BMFR_ERR_INVALID_ARGUMENT, "hip_stream is null");
+    if (s->owner) {
+        if (int32_t st = order_after_server_steps(s, true)) return st;
+    }
     *hip_stream = (void *)s->stream;
     return BMFR_OK;
 }
@@ -1581,15 +1599,15 @@ struct bmfr_server_s {
     // and events; a slot is reused once its previous step has ended
     static constexpr int kGroups = 12;
     static constexpr int kRing = 8;
-    bmfr::BatchStream *d_streams = nullptr;  // [kRing][kGroups][kMaxBatch]
-    bmfr::BatchStep *d_steps = nullptr;      // [kRing][kGroups][kMaxBatch]
-    int32_t *d_starts = nullptr;             // [kRing][kGroups][kStartKinds][kMaxBatch+1]
-    struct Staging {
-        bmfr::BatchStream streams[kGroups][bmfr::kMaxBatch];
-        bmfr::BatchStep steps[kGroups][bmfr::kMaxBatch];
-        int32_t starts[kGroups][bmfr::kStartKinds][bmfr::kMaxBatch + 1];
+    // one launch group's tables, as staged and as the kernels read them; a
+    // step's groups are packed at the front of its slot (one copy a step)
+    struct GroupTables {
+        bmfr::BatchStream streams[bmfr::kMaxBatch];
+        bmfr::BatchStep steps[bmfr::kMaxBatch];
+        int32_t starts[bmfr::kStartKinds][bmfr::kMaxBatch + 1];
     };
-    Staging *h_stage = nullptr;              // [kRing], pinned
+    GroupTables *d_tables = nullptr;         // [kRing][kGroups]
+    GroupTables *h_stage = nullptr;          // [kRing][kGroups], pinned
     hipEvent_t step_start[kRing]{}, step_end[kRing]{};
     int64_t ring_id[kRing]{};                // step id each slot's record belongs to (0: none)
     int64_t next_step = 1;
@@ -1629,7 +1647,6 @@ int32_t launch_step_locked(bmfr_server sv, const std::vector<StepMember> &m, int
     const int slot = (int)(id % bmfr_server_s::kRing);
     HIP_TRY(hipEventSynchronize(sv->step_end[slot]));  // the slot's previous step has ended
     sv->ring_id[slot] = id;
-    bmfr_server_s::Staging &hs = sv->h_stage[slot];
     // the lane: where most members' recent steps ran, else the other one
     int score[2] = {0, 0};
     for (const StepMember &mb : m)
@@ -1637,54 +1654,62 @@ int32_t launch_step_locked(bmfr_server sv, const std::vector<StepMember> &m, int
     const int ln = score[0] || score[1] ? (score[1] > score[0] ? 1 : 0) : 1 - sv->last_lane;
     sv->last_lane = ln;
     const hipStream_t ls = sv->lane[ln];
-    const size_t G = bmfr_server_s::kGroups, M = bmfr::kMaxBatch;
-    bmfr::BatchStream *d_streams = sv->d_streams + (size_t)slot * G * M;
-    bmfr::BatchStep *d_steps = sv->d_steps + (size_t)slot * G * M;
-    int32_t *d_starts = sv->d_starts + (size_t)slot * G * bmfr::kStartKinds * (M + 1);
-    int gcount[bmfr_server_s::kGroups] = {};
+    const int G = bmfr_server_s::kGroups;
+    // launch groups in use, packed in group order at the front of the slot
+    int gcount[bmfr_server_s::kGroups] = {}, gindex[bmfr_server_s::kGroups], gid[bmfr_server_s::kGroups];
+    for (const StepMember &mb : m) ++gcount[server_group(mb.s)];
+    int used = 0;
+    for (int g = 0; g < G; ++g) {
+        gindex[g] = gcount[g] ? used : -1;
+        if (gcount[g]) gid[used++] = g;
+        gcount[g] = 0;
+    }
+    bmfr_server_s::GroupTables *hs = sv->h_stage + (size_t)slot * G;
+    bmfr_server_s::GroupTables *dt = sv->d_tables + (size_t)slot * G;
     for (const StepMember &mb : m) {
         bmfr_stream s = mb.s;
         const int g = server_group(s);
+        bmfr_server_s::GroupTables &t = hs[gindex[g]];
         const int k = gcount[g]++;
-        s->batch_stream(hs.streams[g][k]);
-        s->batch_step(mb.frame, mb.matrix, mb.offset, mb.surface, mb.surface_format, mb.surface_pitch,
-                      hs.steps[g][k]);
-        int32_t(&sts)[bmfr::kStartKinds][bmfr::kMaxBatch + 1] = hs.starts[g];
-        if (k == 0) sts[0][0] = sts[1][0] = sts[2][0] = 0;
-        sts[bmfr::kStartsPadded][k + 1] = sts[bmfr::kStartsPadded][k] + (s->sz.margin_width / 32) * (s->sz.margin_height / 8);
-        sts[bmfr::kStartsImage][k + 1] = sts[bmfr::kStartsImage][k] + ((s->p.width + 63) / 64) * ((s->p.height + 3) / 4);
-        sts[bmfr::kStartsBlocks][k + 1] = sts[bmfr::kStartsBlocks][k] + s->sz.blocks;
-        // order after this stream's pending work and its producer's
-        HIP_TRY(hipEventRecord(s->sync_ev, s->stream));
-        HIP_TRY(hipStreamWaitEvent(ls, s->sync_ev, 0));
+        s->batch_stream(t.streams[k]);
+        s->batch_step(mb.frame, mb.matrix, mb.offset, mb.surface, mb.surface_format, mb.surface_pitch, t.steps[k]);
+        if (k == 0) t.starts[0][0] = t.starts[1][0] = t.starts[2][0] = 0;
+        t.starts[bmfr::kStartsPadded][k + 1] = t.starts[bmfr::kStartsPadded][k] + (s->sz.margin_width / 32) * (s->sz.margin_height / 8);
+        t.starts[bmfr::kStartsImage][k + 1] = t.starts[bmfr::kStartsImage][k] + ((s->p.width + 63) / 64) * ((s->p.height + 3) / 4);
+        t.starts[bmfr::kStartsBlocks][k + 1] = t.starts[bmfr::kStartsBlocks][k] + s->sz.blocks;
+        // Order after this stream's earlier work: what was enqueued on its
+        // HIP stream (which an API call has ordered after the stream's last
+        // step), or else its last step itself when that ran on the other
+        // lane (on this lane it is ordered already; a step whose slot has
+        // been reused has ended).  And after the request's producer.
+        if (s->user_work || s->external || s->last_step_id == 0) {
+            HIP_TRY(hipEventRecord(s->sync_ev, s->stream));
+            HIP_TRY(hipStreamWaitEvent(ls, s->sync_ev, 0));
+            s->user_work = false;
+        } else if (s->last_lane != ln) {
+            const int ps = (int)(s->last_step_id % bmfr_server_s::kRing);
+            if (sv->ring_id[ps] == s->last_step_id) HIP_TRY(hipStreamWaitEvent(ls, sv->step_end[ps], 0));
+        }
         if (mb.producer) HIP_TRY(hipStreamWaitEvent(ls, mb.producer, 0));
     }
-    for (int g = 0; g < bmfr_server_s::kGroups; ++g) {
-        if (!gcount[g]) continue;
-        HIP_TRY(hipMemcpyAsync(d_streams + g * M, hs.streams[g], sizeof(bmfr::BatchStream) * gcount[g],
-                               hipMemcpyHostToDevice, ls));
-        HIP_TRY(hipMemcpyAsync(d_steps + g * M, hs.steps[g], sizeof(bmfr::BatchStep) * gcount[g],
-                               hipMemcpyHostToDevice, ls));
-        HIP_TRY(hipMemcpyAsync(d_starts + g * bmfr::kStartKinds * (M + 1), hs.starts[g], sizeof(hs.starts[g]),
-                               hipMemcpyHostToDevice, ls));
-    }
+    HIP_TRY(hipMemcpyAsync(dt, hs, sizeof(bmfr_server_s::GroupTables) * used, hipMemcpyHostToDevice, ls));
     HIP_TRY(hipEventRecord(sv->step_start[slot], ls));
     for (int st = 0; st < kStages; ++st) {
-        for (int g = 0; g < bmfr_server_s::kGroups; ++g) {
-            if (!gcount[g]) continue;
+        for (int ci = 0; ci < used; ++ci) {
+            const int g = gid[ci];
             bmfr::BatchLaunch L;
-            L.streams = d_streams + g * M;
-            L.steps = d_steps + g * M;
-            L.starts = d_starts + g * bmfr::kStartKinds * (M + 1);
+            L.streams = dt[ci].streams;
+            L.steps = dt[ci].steps;
+            L.starts = &dt[ci].starts[0][0];
             L.n = gcount[g];
             const int sg = g / 3;
             const bool g16 = sg >= 2, t16 = sg & 1;
             const int qr = g % 3;
-            const int32_t(&sts)[bmfr::kStartKinds][bmfr::kMaxBatch + 1] = hs.starts[g];
+            const int32_t(&sts)[bmfr::kStartKinds][bmfr::kMaxBatch + 1] = hs[ci].starts;
             hipError_t e;
             switch (st) {
-                case 0: e = bmfr::launch_batch_accumulate_noisy_data(L, sts[0][L.n], g16, t16, ls); break;
-                case 1: e = bmfr::launch_batch_fitter(L, sts[2][L.n], t16, sv->fitter, sv->grid_cap, qr, ls); break;
+                case 0: e = bmfr::launch_batch_accumulate_noisy_data(L, sts[bmfr::kStartsPadded][L.n], g16, t16, ls); break;
+                case 1: e = bmfr::launch_batch_fitter(L, sts[bmfr::kStartsBlocks][L.n], t16, sv->fitter, sv->grid_cap, qr, ls); break;
                 case 2: e = hipSuccess; break;  // fused into stage 4
                 case 3:
                     e = sv->fuse == BMFR_FUSE_345
@@ -1702,7 +1727,12 @@ int32_t launch_step_locked(bmfr_server sv, const std::vector<StepMember> &m, int
     HIP_TRY(hipEventRecord(sv->step_end[slot], ls));
     for (const StepMember &mb : m) {
         bmfr_stream s = mb.s;
-        HIP_TRY(hipStreamWaitEvent(s->stream, sv->step_end[slot], 0));
+        // a HIP stream the caller holds is ordered after the step now; the
+        // others when an API call on the stream next needs it
+        if (s->external) {
+            HIP_TRY(hipStreamWaitEvent(s->stream, sv->step_end[slot], 0));
+            s->ordered_step = id;
+        }
         s->last_result = s->cur(S_RESULT_A);
         s->noise_frame = mb.frame;
         s->last_step_id = id;
@@ -1711,6 +1741,29 @@ int32_t launch_step_locked(bmfr_server sv, const std::vector<StepMember> &m, int
     }
     if (step_id) *step_id = id;
     return BMFR_OK;
+}
+
+// Orders stream s's own HIP stream after the last server step s took part
+// in (a step whose ring slot has been reused has ended); sv->mu is held.
+int32_t order_stream_locked(bmfr_server sv, bmfr_stream s) {
+    if (s->last_step_id > s->ordered_step) {
+        const int ps = (int)(s->last_step_id % bmfr_server_s::kRing);
+        if (sv->ring_id[ps] == s->last_step_id) {
+            HIP_TRY(hipSetDevice(sv->device));
+            HIP_TRY(hipStreamWaitEvent(s->stream, sv->step_end[ps], 0));
+        }
+        s->ordered_step = s->last_step_id;
+    }
+    return BMFR_OK;
+}
+
+// Declared before check_stream: a stream entry point on a server stream.
+int32_t order_after_server_steps(bmfr_stream s, bool external) {
+    bmfr_server sv = s->owner;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    s->user_work = true;
+    if (external) s->external = true;
+    return order_stream_locked(sv, s);
 }
 
 hipEvent_t pool_event_locked(bmfr_server sv) {
@@ -1897,12 +1950,9 @@ int32_t bmfr_server_create(int64_t hbm_budget_bytes, bmfr_server *out) {
     sv->budget = hbm_budget_bytes;
     hipError_t e = hipGetDevice(&sv->device);
     for (int l = 0; l < 2 && e == hipSuccess; ++l) e = hipStreamCreateWithFlags(&sv->lane[l], hipStreamNonBlocking);
-    const size_t G = bmfr_server_s::kGroups, M = bmfr::kMaxBatch, R = bmfr_server_s::kRing;
-    if (e == hipSuccess) e = hipMalloc((void **)&sv->d_streams, sizeof(bmfr::BatchStream) * R * G * M);
-    if (e == hipSuccess) e = hipMalloc((void **)&sv->d_steps, sizeof(bmfr::BatchStep) * R * G * M);
-    if (e == hipSuccess)
-        e = hipMalloc((void **)&sv->d_starts, sizeof(int32_t) * R * G * bmfr::kStartKinds * (M + 1));
-    if (e == hipSuccess) e = hipHostMalloc((void **)&sv->h_stage, sizeof(bmfr_server_s::Staging) * R);
+    const size_t G = bmfr_server_s::kGroups, R = bmfr_server_s::kRing;
+    if (e == hipSuccess) e = hipMalloc((void **)&sv->d_tables, sizeof(bmfr_server_s::GroupTables) * R * G);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&sv->h_stage, sizeof(bmfr_server_s::GroupTables) * R * G);
     for (int i = 0; i < bmfr_server_s::kRing && e == hipSuccess; ++i) {
         e = hipEventCreate(&sv->step_start[i]);
         if (e == hipSuccess) e = hipEventCreate(&sv->step_end[i]);
@@ -1939,9 +1989,7 @@ int32_t bmfr_server_destroy(bmfr_server sv) {
         if (sv->step_end[i]) (void)hipEventDestroy(sv->step_end[i]);
     }
     if (sv->h_stage) (void)hipHostFree(sv->h_stage);
-    if (sv->d_streams) (void)hipFree(sv->d_streams);
-    if (sv->d_steps) (void)hipFree(sv->d_steps);
-    if (sv->d_starts) (void)hipFree(sv->d_starts);
+    if (sv->d_tables) (void)hipFree(sv->d_tables);
     for (hipStream_t l : sv->lane)
         if (l) (void)hipStreamDestroy(l);
     delete sv;
@@ -2036,9 +2084,10 @@ int32_t bmfr_server_close_stream(bmfr_server sv, bmfr_stream s) {
         sv->cv_drained.wait(lk, [&] { return s->queue.empty() && s->frames_in_flight == 0; });
         sv->open.erase(std::find(sv->open.begin(), sv->open.end(), s));
         sv->used -= s->sz.device_bytes;
+        if (int32_t st = order_stream_locked(sv, s)) return st;
     }
-    // destroy_stream synchronises the stream's own HIP stream, which waits for
-    // every server step the stream took part in.
+    // destroy_stream synchronises the stream's own HIP stream, now ordered
+    // after every server step the stream took part in.
     destroy_stream(s);
     admit_waiting(sv);
     return BMFR_OK;

@@ -228,8 +228,11 @@ int main(int argc, char **argv) {
     std::vector<double> lat;
     std::map<int64_t, std::pair<int, float>> steps;   // step id -> (streams, gpu ms)
     int64_t frames_done = 0, pixels_done = 0, dropped = 0, arrived = 0, failed = 0;
-    // throughput and GPU busy over the arrival window [0, seconds] (not the
-    // drain and the closes after it): frames done and steps completed in it
+    // throughput and step time over the arrival window [0, seconds] (not the
+    // drain and the closes after it): frames done and steps completed in it.
+    // step_time_per_s sums the steps' GPU time (first start to last end) per
+    // second of the window; steps on the server's two lanes overlap, so it
+    // can exceed 1.
     int64_t frames_in_window = 0, pixels_in_window = 0;
     double gpu_ms_in_window = 0;
     int admitted = admitted_before, admitted_after_waiting = 0, never_admitted = 0;
@@ -379,7 +382,7 @@ int main(int argc, char **argv) {
                 "\"aggregate_frames_per_s\": %.1f, \"mpixel_per_s\": %.1f, "
                 "\"latency_ms_p50\": %.3f, \"latency_ms_p99\": %.3f, \"latency_ms_max\": %.3f, "
                 "\"steps\": %lld, \"mean_streams_per_step\": %.2f, \"step_gpu_ms_mean\": %.4f, "
-                "\"gpu_busy\": %.3f, \"max_queue_per_stream\": %d, \"max_in_flight\": %d}\n",
+                "\"step_time_per_s\": %.3f, \"max_queue_per_stream\": %d, \"max_in_flight\": %d}\n",
                 a.mode.c_str(), a.streams, wall, a.budget_gib, admitted, admitted_after_waiting, never_admitted,
                 (long long)arrived, (long long)frames_done, (long long)dropped, (long long)failed,
                 (double)frames_in_window / a.seconds, (double)pixels_in_window / a.seconds / 1e6, percentile(lat, 50),

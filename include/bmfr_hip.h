@@ -411,7 +411,12 @@ int32_t bmfr_server_cancel_request(bmfr_server server, uint64_t tag, int32_t *fo
 int32_t bmfr_server_close_stream(bmfr_server server, bmfr_stream stream);
 /* One step now over `count` distinct streams of the server (none of which may
  * have frames queued through bmfr_server_submit); frames[i], matrices[16*i..]
- * and offsets[2*i..] are stream i's bmfr_accumulate_noisy_data arguments. */
+ * and offsets[2*i..] are stream i's bmfr_accumulate_noisy_data arguments.
+ * Ordering: a step runs after each member's earlier steps and after the work
+ * that calls on the member enqueued; any call on a server stream runs after
+ * the steps the stream took part in.  Work enqueued directly on a stream's
+ * HIP stream is ordered the same way once bmfr_stream_hip_stream has been
+ * called for that stream. */
 int32_t bmfr_server_step(bmfr_server server, int32_t count, const bmfr_stream *streams,
                          const int32_t *frames, const float *matrices, const float *offsets,
                          int64_t *step_id);

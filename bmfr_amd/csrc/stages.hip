@@ -194,9 +194,11 @@ __device__ __forceinline__ float3 stage3_color_fit(float3 wp, float3 n, const Bl
             const int i = f - kNotScaled;
             v = rec ? scale_feature_r(v, fit.mn[i], fit.mx[i], rec->s[i]) : scale_feature(v, fit.mn[i], fit.mx[i]);
         }
-        cx = cx + fit.w[f * 3 + 0] * v;
-        cy = cy + fit.w[f * 3 + 1] * v;
-        cz = cz + fit.w[f * 3 + 2] * v;
+        // color += weight * feature, one FMA a term: the reference's
+        // expression (bmfr.cl:745) under OpenCL's default FP_CONTRACT ON
+        cx = __builtin_fmaf(fit.w[f * 3 + 0], v, cx);
+        cy = __builtin_fmaf(fit.w[f * 3 + 1], v, cy);
+        cz = __builtin_fmaf(fit.w[f * 3 + 2], v, cz);
     }
     return make_float3(cx < 0.f ? 0.f : cx, cy < 0.f ? 0.f : cy, cz < 0.f ? 0.f : cz);
 }
@@ -222,9 +224,9 @@ __device__ __forceinline__ float3 stage3_color_of(const StageArgs &a, const Stag
                 const int i = f - fs.not_scaled;
                 v = scale_feature(v, gload(mm, i * 2), gload(mm, i * 2 + 1));
             }
-            cx = cx + gload(w, f * 3 + 0) * v;
-            cy = cy + gload(w, f * 3 + 1) * v;
-            cz = cz + gload(w, f * 3 + 2) * v;
+            cx = __builtin_fmaf(gload(w, f * 3 + 0), v, cx);
+            cy = __builtin_fmaf(gload(w, f * 3 + 1), v, cy);
+            cz = __builtin_fmaf(gload(w, f * 3 + 2), v, cz);
         }
         return make_float3(cx < 0.f ? 0.f : cx, cy < 0.f ? 0.f : cy, cz < 0.f ? 0.f : cz);
     } else {

@@ -2,7 +2,9 @@
  * CPU oracle for the BMFR five-stage frame chain.  TEST INFRASTRUCTURE ONLY
  * (the checker, never the product): see bmfr_oracle.h for scope and the
  * semantic decisions taken where the reference is racy or implementation
- * defined.  Build with -O2 -ffp-contract=off (oracle/Makefile).
+ * defined.  Build with -O2 -ffp-contract=off (oracle/Makefile): every float
+ * operation rounded separately, except the FMAs written out as fmaf (stage
+ * 3's sum, which the reference's OpenCL build contracts).
  *
  * Pinning status (what checks each function; tests/test_oracle_*.py):
  *   orc_hash/orc_random      known-answer table computed independently in the tests
@@ -664,10 +666,13 @@ void orc_weighted_sum(const orc_params *p, const float *weights, const float *mi
                     const int64_t mi = (g * (NF - NNS) + f - NNS) * 2;
                     v = orc_scale(v, mins_maxs[mi + 0], mins_maxs[mi + 1]);
                 }
+                /* color += weight * feature (bmfr.cl:745): one expression,
+                 * which OpenCL's default FP_CONTRACT ON fuses, as the
+                 * reference's compiled kernel does (one FMA a term) */
                 const float *wt = weights + (g * NF + f) * 3;
-                col[0] = col[0] + wt[0] * v;
-                col[1] = col[1] + wt[1] * v;
-                col[2] = col[2] + wt[2] * v;
+                col[0] = fmaf(wt[0], v, col[0]);
+                col[1] = fmaf(wt[1], v, col[1]);
+                col[2] = fmaf(wt[2], v, col[2]);
             }
             for (int c = 0; c < 3; ++c) output[lp * 3 + c] = col[c] < 0.f ? 0.f : col[c];
         }

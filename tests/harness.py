@@ -17,6 +17,29 @@ def f16_round(a):
     return np.asarray(a, np.float32).astype(np.float16).astype(np.float32)
 
 
+def fma32(a, b, c):
+    """Elementwise float32 fused multiply-add, RN(a*b + c) with one rounding.
+    a*b is exact in float64 (24 + 24 bits); s = a*b + c rounds in float64,
+    and TwoSum gives that rounding's exact error e.  RN32(s) is RN32 of the
+    exact value unless s lies exactly halfway between two float32 values,
+    where the sign of e decides the side."""
+    a, b, c = (np.asarray(v, np.float32).astype(np.float64) for v in (a, b, c))
+    p = a * b
+    s = p + c
+    bb = s - p
+    e = (p - (s - bb)) + (c - bb)
+    r = s.astype(np.float32)
+    r64 = r.astype(np.float64)
+    up = np.nextafter(r, np.float32(np.inf)).astype(np.float64)
+    dn = np.nextafter(r, np.float32(-np.inf)).astype(np.float64)
+    with np.errstate(invalid="ignore", over="ignore"):
+        at_up_mid = (s == (r64 + up) / 2) & (e > 0)
+        at_dn_mid = (s == (r64 + dn) / 2) & (e < 0)
+    r = np.where(at_up_mid, up.astype(np.float32), r)
+    r = np.where(at_dn_mid, dn.astype(np.float32), r)
+    return r.astype(np.float32)
+
+
 def gpu_params(w, h, tmp_f16=True, storage_f16=False, **kw):
     return bmfr_amd.default_params(
         w, h, tmp_storage=bmfr_amd.STORAGE_F16 if tmp_f16 else bmfr_amd.STORAGE_F32,

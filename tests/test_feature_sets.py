@@ -19,7 +19,7 @@ import pytest
 import bmfr_amd
 from bmfr_amd import _lib
 from oracle import bind
-from tests.harness import REFERENCE_SET, copy_features, set_factor_pairs
+from tests.harness import REFERENCE_SET, copy_features, fma32, set_factor_pairs
 
 lib = bind.load()
 
@@ -276,7 +276,8 @@ def test_configured_fitter_exact_fit_recovers_generating_weights(name):
 @pytest.mark.parametrize("name", ["one_scaled", "cross5", "quad16"])
 def test_configured_weighted_sum_closed_form(name):
     """filtered = max(0, sum_f w_f * scale(feature_f)) in float32, features
-    in order, scaled ones with their block's bounds (bmfr.cl:727-750)."""
+    in order, each term one FMA, scaled ones with their block's bounds
+    (bmfr.cl:727-750)."""
     w, h = 70, 48
     pairs, ns = SETS[name]
     n = len(pairs)
@@ -306,7 +307,7 @@ def test_configured_weighted_sum_closed_form(name):
             lo, hi = mm[g, i - ns, 0], mm[g, i - ns, 1]
             v = np.where(np.abs(hi - lo) > 1, (v - lo) / (hi - lo), v - lo).astype(np.float32)
         for c in range(3):
-            col[..., c] = col[..., c] + Wt[g, i, c] * v
+            col[..., c] = fma32(Wt[g, i, c], v, col[..., c])   # bmfr.cl:745, contracted
     want = np.where(col < 0, np.float32(0), col)
     assert np.array_equal(o.filtered.reshape(h, w, 3), want)
 

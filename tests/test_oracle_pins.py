@@ -13,6 +13,8 @@ import ctypes
 import numpy as np
 import pytest
 
+from tests.harness import fma32
+
 from oracle import bind
 
 lib = bind.load()
@@ -338,6 +340,56 @@ def test_fitter_householder_sign_is_reference_choice():
 
 
 # ---- stages 3-5 (bmfr.cl:703-974) -----------------------------------------------
+def _rn32_exact(q):
+    """Round a Fraction to the nearest float32, ties to even (normal range)."""
+    from fractions import Fraction
+    import math
+    if q == 0:
+        return np.float32(0)
+    sign = -1 if q < 0 else 1
+    q = abs(q)
+    e = math.floor(math.log2(q))
+    while Fraction(2) ** e > q:
+        e -= 1
+    while Fraction(2) ** (e + 1) <= q:
+        e += 1
+    ulp = Fraction(2) ** (e - 23)
+    t = q / ulp
+    n = t.numerator // t.denominator
+    rem = t - n
+    if rem > Fraction(1, 2) or (rem == Fraction(1, 2) and n % 2 == 1):
+        n += 1
+    return np.float32(sign * float(n * ulp))
+
+
+def test_fma32_is_the_single_rounding():
+    """The numpy fused multiply-add the tests use against exact rational
+    arithmetic: random operands over several exponent ranges, and operands
+    built so that a*b + c lies a hair off a float32 rounding midpoint (where
+    rounding twice, through float64, goes the wrong way)."""
+    from fractions import Fraction
+    rng = np.random.default_rng(5)
+    a = rng.uniform(-2, 2, 4000).astype(np.float32)
+    b = (rng.uniform(-2, 2, 4000) * 10.0 ** rng.integers(-6, 6, 4000)).astype(np.float32)
+    c = (rng.uniform(-2, 2, 4000) * 10.0 ** rng.integers(-6, 6, 4000)).astype(np.float32)
+    # midpoint cases: c in [1, 2) (float32 spacing 2^-23), a*b = +-(2^-24 - 2^-70),
+    # so a*b + c lies 2^-70 inside a float32 midpoint and rounds to c, while
+    # float64 rounds it onto the midpoint
+    k = rng.integers(1, 4000, 200).astype(np.float64)
+    c_mid = (1.0 + k * 2.0 ** -23).astype(np.float32)
+    a_mid = np.full(200, np.float32(1.0 + 2.0 ** -23))
+    h = np.float32(2.0 ** -24 - 2.0 ** -47)
+    b_mid = np.where(rng.random(200) < 0.5, h, -h).astype(np.float32)
+    a = np.concatenate([a, a_mid]).astype(np.float32)
+    b = np.concatenate([b, b_mid]).astype(np.float32)
+    c = np.concatenate([c, c_mid]).astype(np.float32)
+    got = fma32(a, b, c)
+    for i in range(len(a)):
+        q = Fraction(float(a[i])) * Fraction(float(b[i])) + Fraction(float(c[i]))
+        want = _rn32_exact(q)
+        assert got[i] == want, (i, a[i], b[i], c[i], got[i], want)
+
+
 def test_weighted_sum_unit_weights_pick_block_and_feature():
     w, h = 70, 48
     sc = _scene(w, h)

@@ -130,3 +130,33 @@ def _fitter_sample(g, fr):
     assert singular <= len(sample) // 4
     print(f"max size: {len(worse)} blocks ({singular} singular skipped), fitted-colour error vs FP64 "
           f"relative to the oracle's: median {np.median(worse):.2f}, max {np.max(worse):.2f}", flush=True)
+
+
+def test_max_size_batched(gpu):
+    """The batched kernels (one launch over the concatenated tiles and
+    blocks of the batch's members) at the largest frame, beside a small
+    member: every member's results bit-identical to the stream run alone."""
+    sizes = [(W, H), (96, 64)]
+    scs = [scene(w, h, spheres=24, static=(w == W)) for (w, h) in sizes]
+    alone = [bmfr_amd.Stream(gpu_params(w, h)) for (w, h) in sizes]
+    member = [bmfr_amd.Stream(gpu_params(w, h)) for (w, h) in sizes]
+    b = bmfr_amd.Batch(member)
+    try:
+        for fr in range(2):
+            print(f"max size batched: frame {fr}", flush=True)
+            args = [sc.frame_args(fr) for sc in scs]
+            for k, sc in enumerate(scs):
+                sc.render_to_stream(fr, alone[k])
+                sc.render_to_stream(fr, member[k])
+                alone[k].run_frame(fr, *args[k])
+            b.run([fr] * len(sizes), np.concatenate([a[0] for a in args]), np.concatenate([a[1] for a in args]))
+            b.synchronize()
+            for k in range(len(sizes)):
+                assert np.array_equal(alone[k].download_result(), member[k].download_result()), (fr, k)
+                assert np.array_equal(alone[k].read_buffer("weights"), member[k].read_buffer("weights")), (fr, k)
+    finally:
+        b.close()
+        for s in alone + member:
+            s.close()
+        for sc in scs:
+            sc.close()

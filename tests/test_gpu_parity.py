@@ -344,3 +344,70 @@ def test_fused_chain_equals_separate_stages(gpu, storage_f16, w, h, level):
     finally:
         a.close()
         b.close()
+
+
+def _same(got, want):
+    """Bit-exact for integers; for floats equal values, or NaN on both sides
+    (a NaN's payload and sign are not compared: the CPU's default NaN has
+    the sign bit set, the GPU's not)."""
+    if got.dtype.kind != "f":
+        return np.array_equal(got, want)
+    return bool(((got == want) | (np.isnan(got) & np.isnan(want))).all())
+
+
+def test_nonfinite_colours_bit_exact(gpu):
+    """Renderer output with NaN, +-inf and -0 in the noisy colour and the
+    albedo (an overflowed firefly, an unwritten texel): stage 1 bit-exact
+    against the oracle (NaN and inf go through the blend; the block matrix
+    stores NaN as 0 and clamps the rest to +-65504 in binary16,
+    bmfr.cl:466-476), and stages 3-5 bit-exact given the GPU's inputs (the
+    tone curve maps NaN and negatives to 0 and +inf to 1, bmfr.cl:851-856;
+    the TAA blends what it gets)."""
+    w, h = 70, 48
+    P = w * h
+    ls = Lockstep(w, h, tmp_f16=True)
+    rng = np.random.default_rng(5)
+    try:
+        for fr in range(4):
+            f = ls.sc.render_host(fr)
+            noisy, albedo = f["noisy"].copy(), f["albedo"].copy()
+            i = rng.choice(3 * P, 40, replace=False)
+            noisy[i[:10]], noisy[i[10:20]], noisy[i[20:30]], noisy[i[30:]] = np.nan, np.inf, -np.inf, -0.0
+            i = rng.choice(3 * P, 30, replace=False)
+            albedo[i[:10]], albedo[i[10:20]], albedo[i[20:]] = np.nan, np.inf, -0.0
+            ls.g.upload_frame(albedo, f["normals"], f["positions"], noisy)
+            ls.o.upload(albedo, f["normals"], f["positions"], noisy)
+            vp, off = ls.sc.frame_args(fr)
+            ls.g.accumulate_noisy_data(fr, vp, off)
+            ls.o.accumulate_noisy_data(fr, vp, off)
+            o = ls.o
+            c, pv = o.cur(), o.prev()
+            for name, want in [("noisy_acc_cur", o.noisy_acc[c]), ("prev_pixel", o.prev_pixel),
+                               ("accept", o.accept), ("spp_cur", o.spp[c]), ("tmp_data", o.tmp)]:
+                got = ls.read(name)
+                assert _same(got, want), (fr, name)
+            assert np.isfinite(ls.read("tmp_data")).all()
+            ls.g.fitter(fr)
+            o.weights[:] = ls.read("weights")
+            o.mins_maxs[:] = ls.read("mins_maxs")
+            ls.g.weighted_sum(fr)
+            o.weighted_sum(fr)
+            assert _same(ls.read("filtered"), o.filtered), fr
+            o.out[pv][:] = ls.read("out_prev")
+            ls.g.accumulate_filtered_data(fr)
+            o.accumulate_filtered_data(fr)
+            assert _same(ls.read("out_cur"), o.out[c]), fr
+            tone = ls.read("tone_mapped")
+            assert np.isfinite(tone).all() and tone.min() >= 0.0 and tone.max() <= 1.0
+            fin = np.isfinite(o.tone)
+            ulp = np.abs(tone[fin].view(np.int32).astype(np.int64) - o.tone[fin].view(np.int32).astype(np.int64))
+            assert fin.all() and ulp.max() <= 1, (fr, int(ulp.max()))
+            o.tone[:] = tone
+            o.result[pv][:] = ls.read("result_prev")
+            ls.g.taa(fr)
+            o.taa(fr)
+            assert _same(ls.read("result_cur"), o.result[c]), fr
+            ls.g.end_frame()
+            o.end_frame()
+    finally:
+        ls.close()

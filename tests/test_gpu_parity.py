@@ -355,10 +355,13 @@ def _same(got, want):
     return bool(((got == want) | (np.isnan(got) & np.isnan(want))).all())
 
 
-def test_nonfinite_colours_bit_exact(gpu):
+@pytest.mark.parametrize("geometry", [False, True])
+def test_nonfinite_inputs_bit_exact(gpu, geometry):
     """Renderer output with NaN, +-inf and -0 in the noisy colour and the
-    albedo (an overflowed firefly, an unwritten texel): stage 1 bit-exact
-    against the oracle (NaN and inf go through the blend; the block matrix
+    albedo (an overflowed firefly, an unwritten texel) and, with `geometry`,
+    in world positions and normals too: stage 1 bit-exact against the oracle
+    (NaN and inf go through the reprojection and the blend; floor() of NaN
+    is 0 and of +-inf saturates, as AMD's conversion does; the block matrix
     stores NaN as 0 and clamps the rest to +-65504 in binary16,
     bmfr.cl:466-476), and stages 3-5 bit-exact given the GPU's inputs (the
     tone curve maps NaN and negatives to 0 and +inf to 1, bmfr.cl:851-856;
@@ -375,8 +378,17 @@ def test_nonfinite_colours_bit_exact(gpu):
             noisy[i[:10]], noisy[i[10:20]], noisy[i[20:30]], noisy[i[30:]] = np.nan, np.inf, -np.inf, -0.0
             i = rng.choice(3 * P, 30, replace=False)
             albedo[i[:10]], albedo[i[10:20]], albedo[i[20:]] = np.nan, np.inf, -0.0
-            ls.g.upload_frame(albedo, f["normals"], f["positions"], noisy)
-            ls.o.upload(albedo, f["normals"], f["positions"], noisy)
+            normals, positions = f["normals"].copy(), f["positions"].copy()
+            if geometry:
+                # whole pixels and single components
+                px = rng.choice(P, 12, replace=False)
+                for k, v in enumerate([np.nan, np.inf, -np.inf]):
+                    positions.reshape(P, 3)[px[2 * k]] = v
+                    normals.reshape(P, 3)[px[2 * k + 1]] = v
+                i = rng.choice(3 * P, 18, replace=False)
+                positions[i[:6]], normals[i[6:12]], positions[i[12:]] = np.nan, np.inf, -np.inf
+            ls.g.upload_frame(albedo, normals, positions, noisy)
+            ls.o.upload(albedo, normals, positions, noisy)
             vp, off = ls.sc.frame_args(fr)
             ls.g.accumulate_noisy_data(fr, vp, off)
             ls.o.accumulate_noisy_data(fr, vp, off)

@@ -138,8 +138,9 @@ BATCH_MIX = [(640, 360)] * 4 + [(1280, 720)] * 6 + [(1920, 1080)] * 4 + [(2560, 
 
 def run_batched(steps, warmup, lanes=2):
     """16 concurrent streams of mixed resolutions, one frame each per step:
-    every stage is one launch over all of them (the fitter a persistent
-    launch over all 22,988 blocks), replayed from a captured hipGraph.  The
+    every stage is one launch over all of them (the fitter one launch over
+    all 22,988 blocks, its shape chosen by the autotuner), replayed from a
+    captured hipGraph.  The
     renderer stand-in writes each stream's G-buffers before a step (not
     timed); the step is timed with HIP events on the batch's HIP stream."""
     import bmfr_amd

@@ -30,7 +30,7 @@ HBM_PEAK_GBS = 8000.0
 VALU_ISSUE_PEAK = 1024 * 2.4e9 / 4 / 1e9
 # SQ_INSTS_VALU / SQ_WAVES of the fitter (k_fitter<16, 4, __half, true, false>),
 # profiles/fitter_720p_counters.md
-FITTER_VALU_PER_WAVE = 5962
+FITTER_VALU_PER_WAVE = 5275
 
 
 def reference_fitter_lds_bytes():

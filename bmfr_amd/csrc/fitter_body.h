@@ -692,6 +692,170 @@ __device__ __forceinline__ void fast_pivots(FastRows<RPL, TT, HALF_ROUND, F + 3>
     }
 }
 
+// ---- Binary16 pairs after the first pivot (one wave per block, binary16
+// tmp_data) ----
+// Every element the reference stores is a binary16 value, and after the
+// first pivot's update (bmfr.cl:651) so is every element the later pivots
+// read.  A lane's 16 rows of a column are then held as 8 pairs of binary16
+// in 8 registers (element pair (2i, 2i+1), the layout the 16-byte loads
+// deliver), the dot products are v_dot2_f32_f16 (two exact products and a
+// sum a step) and the updates v_fma_mixlo_f16 / v_fma_mixhi_f16 (the FMA of
+// the binary16 element and its binary16 u with the FP32 factor, rounded to
+// binary16 once: the same single rounding as the unpacked update), written
+// in asm because the compiler keeps each half in a register of its own and
+// repacks the pairs for every dot product.
+typedef _Float16 Half2 __attribute__((ext_vector_type(2)));
+
+template <int C>
+struct PackedRows {
+    Half2 p[C][8];
+    // Row r (< 512) of column f, wave-uniform: lane r / 8, element r % 8.
+    __device__ __forceinline__ float pivot_row(int f, int r) const {
+        const uint32_t s = (uint32_t)__builtin_amdgcn_readlane(__builtin_bit_cast(int, p[f][(r % 8) / 2]), r / 8);
+        return (float)__builtin_bit_cast(_Float16, (uint16_t)((r & 1) ? s >> 16 : s));
+    }
+};
+
+// a = RN16(a - u * f) in both halves, u binary16 pairs.  Each half is an
+// asm statement of its own, so that the scheduler can put other work
+// between the two writes of the register.
+__device__ __forceinline__ void pair_update(Half2 &a, Half2 u, float f) {
+    asm("v_fma_mixlo_f16 %0, -%1, %2, %0 op_sel_hi:[1,0,1]" : "+v"(a) : "v"(u), "v"(f));
+    asm("v_fma_mixhi_f16 %0, -%1, %2, %0 op_sel:[1,0,1] op_sel_hi:[1,0,1]" : "+v"(a) : "v"(u), "v"(f));
+}
+// The same with u in FP32 (the pair holding the pivot row: u_c is not a
+// binary16 value).
+__device__ __forceinline__ void pair_update_f32u(Half2 &a, float ulo, float uhi, float f) {
+    asm("v_fma_mixlo_f16 %0, -%1, %2, %0 op_sel_hi:[0,0,1]" : "+v"(a) : "v"(ulo), "v"(f));
+    asm("v_fma_mixhi_f16 %0, -%1, %2, %0 op_sel:[0,0,1] op_sel_hi:[0,0,1]" : "+v"(a) : "v"(uhi), "v"(f));
+}
+// RN16(alo - ulo * f), RN16(ahi - uhi * f) from FP32 operands (the first
+// pivot, whose noised features are FP32), as a pair in alo's register.
+__device__ __forceinline__ Half2 pair_from_f32(float ulo, float uhi, float f, float alo, float ahi) {
+    asm("v_fma_mixlo_f16 %0, -%1, %2, %0 op_sel_hi:[0,0,0]" : "+v"(alo) : "v"(ulo), "v"(f));
+    asm("v_fma_mixhi_f16 %0, -%1, %2, %3 op_sel_hi:[0,0,0]" : "+v"(alo) : "v"(uhi), "v"(f), "v"(ahi));
+    return __builtin_bit_cast(Half2, alo);
+}
+
+// A lane's share of sum_i x_i y_i over its 16 rows as pairs: two chains of
+// v_dot2_f32_f16 (four measured slower: their accumulators start as moves).
+__device__ __forceinline__ float pair_dot(const Half2 (&x)[8], const Half2 (&y)[8]) {
+    float s[2] = {0.f, 0.f};
+#pragma unroll
+    for (int i = 0; i < 8; ++i) s[i % 2] = __builtin_amdgcn_fdot2(x[i], y[i], s[i % 2], false);
+    return s[0] + s[1];
+}
+
+// The first pivot: fast_pivot<0>'s arithmetic with the F feature columns in
+// FP32 (the noised features are not binary16 values) and the three colour
+// columns as binary16 pairs from the load; every updated column is written
+// as pairs into `pk`.
+template <int RPL, typename TT, int F>
+__device__ __forceinline__ void packed_pivot0(FastRows<RPL, TT, true, F + 3> &m, PackedRows<F + 3> &pk,
+                                              float (&norms)[F], int t) {
+    static_assert(RPL == 16, "one wave per block");
+    constexpr int NV = F + 3 - 1;
+    float x[RPL];
+#pragma unroll
+    for (int k = 0; k < RPL; ++k) x[k] = (k == 0 && t == 0) ? 0.f : m.a[0][k];   // row 0 is the pivot
+    Half2 xp[8];   // the same values as pairs (column 0 holds binary16 values)
+#pragma unroll
+    for (int q = 0; q < 8; ++q) xp[q] = Half2{(_Float16)x[2 * q], (_Float16)x[2 * q + 1]};
+    float v[NV + 1];
+    v[0] = lane_dot<RPL>(x, [&](int k) { return x[k]; });
+#pragma unroll
+    for (int i = 0; i < F - 1; ++i) v[1 + i] = lane_dot<RPL>(x, [&](int k) { return m.a[1 + i][k]; });
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[F + c] = pair_dot(xp, pk.p[F + c]);
+    wave_reduce_n<OpAdd, NV + 1>(v);
+    float e[NV + 1];
+    e[0] = m.pivot_row(0, 0);
+#pragma unroll
+    for (int i = 0; i < F - 1; ++i) e[1 + i] = m.pivot_row(1 + i, 0);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) e[F + c] = pk.pivot_row(F + c, 0);
+    const float sigma = v[0], xc = e[0];
+    const float norm = __builtin_amdgcn_sqrtf(sigma + xc * xc);
+    const float uc = xc - norm;
+    const float uu = sigma + uc * uc;
+    norms[0] = norm;
+    const float inv2_uu = 2.f * __builtin_amdgcn_rcpf(uu);
+    const float u0 = t == 0 ? uc : x[0];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        const int j = 1 + i;
+        const float fct = (v[1 + i] + uc * e[1 + i]) * inv2_uu;
+        if (j < F) {
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                pk.p[j][q] = pair_from_f32(q == 0 ? u0 : x[2 * q], x[2 * q + 1], fct, m.a[j][2 * q],
+                                           m.a[j][2 * q + 1]);
+        } else {
+            pair_update_f32u(pk.p[j][0], u0, x[1], fct);
+#pragma unroll
+            for (int q = 1; q < 8; ++q) pair_update(pk.p[j][q], xp[q], fct);
+        }
+    }
+}
+
+// Pivot COL >= 1 on binary16 pairs: fast_pivot's arithmetic (the dot
+// products summed in another order).
+template <int COL, int F>
+__device__ __forceinline__ void packed_pivot(PackedRows<F + 3> &pk, float (&norms)[F], int t) {
+    constexpr int NV = F + 3 - 1 - COL;
+    constexpr int KC = COL % 8;   // element of lane COL / 8 holding the pivot row
+    // x: the pivot column below the pivot row (rows <= COL zeroed; they sit
+    // in elements 0..7 of lanes 0 and 1)
+    Half2 x[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        if (i < 4 && 2 * i <= COL) {
+            const int rlo = t * 8 + 2 * i;
+            const uint32_t mask = (rlo > COL ? 0xFFFFu : 0u) | (rlo + 1 > COL ? 0xFFFF0000u : 0u);
+            x[i] = __builtin_bit_cast(Half2, __builtin_bit_cast(uint32_t, pk.p[COL][i]) & mask);
+        } else {
+            x[i] = pk.p[COL][i];
+        }
+    }
+    float v[NV + 1];
+    v[0] = pair_dot(x, x);
+#pragma unroll
+    for (int i = 0; i < NV; ++i) v[1 + i] = pair_dot(x, pk.p[COL + 1 + i]);
+    wave_reduce_n<OpAdd, NV + 1>(v);
+    float e[NV + 1];
+    e[0] = pk.pivot_row(COL, COL);
+#pragma unroll
+    for (int i = 0; i < NV; ++i) e[1 + i] = pk.pivot_row(COL + 1 + i, COL);
+    const float sigma = v[0], xc = e[0];
+    const float norm = __builtin_amdgcn_sqrtf(sigma + xc * xc);
+    const float uc = xc - norm;
+    const float uu = sigma + uc * uc;
+    norms[COL] = norm;
+    const float inv2_uu = 2.f * __builtin_amdgcn_rcpf(uu);
+    // the pair holding the pivot row, with u_c in the pivot lane, in FP32
+    const bool pl = t == COL / 8;
+    const float ulo = (pl && (KC & 1) == 0) ? uc : (float)x[KC / 2].x;
+    const float uhi = (pl && (KC & 1) == 1) ? uc : (float)x[KC / 2].y;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        const int j = COL + 1 + i;
+        const float fct = (v[1 + i] + uc * e[1 + i]) * inv2_uu;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            if (q == KC / 2) pair_update_f32u(pk.p[j][q], ulo, uhi, fct);
+            else pair_update(pk.p[j][q], x[q], fct);
+        }
+    }
+}
+
+template <int COL, int F>
+__device__ __forceinline__ void packed_pivots(PackedRows<F + 3> &pk, float (&norms)[F], int t) {
+    if constexpr (COL < F) {
+        packed_pivot<COL, F>(pk, norms, t);
+        packed_pivots<COL + 1, F>(pk, norms, t);
+    }
+}
+
 // One 32x32 block by the W waves of work-group slice `sub`; `active` false
 // recomputes a valid block (keeping barriers uniform) without storing.
 // NF: the feature set (SetShape); a configured set (NF > 0) scales its
@@ -708,11 +872,27 @@ __device__ __forceinline__ void fitter_block_fast(const TT *__restrict__ src, fl
     const int sub = threadIdx.x / T;
     const int lane = threadIdx.x & 63;
     const int wave = t >> 6;
+    // binary16 pairs from the first pivot on (one wave per block, binary16
+    // tmp_data): the colour columns stay pairs from the load
+    constexpr bool kPairs = HALF_ROUND && RPL == 16;
     FastRows<RPL, TT, HALF_ROUND, C> m;
+    PackedRows<C> pk;
 #pragma unroll
-    for (int f = 0; f < C; ++f)
+    for (int f = 0; f < C; ++f) {
 #pragma unroll
-        for (int q = 0; q < L::Q; ++q) load_rows<TT, V>(src + f * kBlockPixels + q * (T * V) + t * V, &m.a[f][q * V]);
+        for (int q = 0; q < L::Q; ++q) {
+            const TT *p = src + f * kBlockPixels + q * (T * V) + t * V;
+            if (kPairs && f >= F) {
+                const uint4 w = *reinterpret_cast<const uint4 *>(p);
+                pk.p[f][4 * q + 0] = __builtin_bit_cast(Half2, w.x);
+                pk.p[f][4 * q + 1] = __builtin_bit_cast(Half2, w.y);
+                pk.p[f][4 * q + 2] = __builtin_bit_cast(Half2, w.z);
+                pk.p[f][4 * q + 3] = __builtin_bit_cast(Half2, w.w);
+            } else {
+                load_rows<TT, V>(p, &m.a[f][q * V]);
+            }
+        }
+    }
 
     // ---- scaling, bmfr.cl:511-542 (max taken as -min(-x): exact) ----
     if constexpr (NF == 0) {
@@ -821,11 +1001,16 @@ __device__ __forceinline__ void fitter_block_fast(const TT *__restrict__ src, fl
 
     // ---- Householder QR over the F feature columns, bmfr.cl:549-656 ----
     float norms[F];
-    fast_pivots<0, RPL, BPG, TT, HALF_ROUND, F>(m, norms, red, sub, t, wave, lane);
-
-    // ---- back substitution, bmfr.cl:658-699, in wave 0 ----
-    if (wave == 0)
-        back_substitution<F>([&](int c, int r) { return m.pivot_row(c, r); }, norms, weights, active, lane);
+    if constexpr (kPairs) {
+        packed_pivot0<RPL, TT, F>(m, pk, norms, t);
+        packed_pivots<1, F>(pk, norms, t);
+        back_substitution<F>([&](int c, int r) { return pk.pivot_row(c, r); }, norms, weights, active, lane);
+    } else {
+        fast_pivots<0, RPL, BPG, TT, HALF_ROUND, F>(m, norms, red, sub, t, wave, lane);
+        // ---- back substitution, bmfr.cl:658-699, in wave 0 ----
+        if (wave == 0)
+            back_substitution<F>([&](int c, int r) { return m.pivot_row(c, r); }, norms, weights, active, lane);
+    }
 }
 
 
@@ -891,8 +1076,11 @@ inline hipError_t launch_set(const FitterParams &fp, bool tmp_f16, hipStream_t s
 // Persistent batched fitter: the work-groups stride over the concatenated
 // block list of every stream in the batch (edge blocks included).  The trip
 // count depends only on blockIdx, so barriers stay uniform.
+// The fast binary16-pair body is held to two waves per SIMD (256 registers;
+// the batch loop's own registers would take it just past that).
 template <int RPL, int BPG, typename TT, bool HALF_ROUND, bool COMP>
-__global__ __launch_bounds__(64 * (16 / RPL) * BPG) void kb_fitter(const BatchLaunch L) {
+__global__ __launch_bounds__(64 * (16 / RPL) * BPG)
+__attribute__((amdgpu_waves_per_eu(HALF_ROUND && RPL == 16 && !COMP ? 2 : 1))) void kb_fitter(const BatchLaunch L) {
     __shared__ float red[2][BPG][16 / RPL][kFastSlots];
     const int32_t *starts = L.starts + kStartsBlocks * (kMaxBatch + 1);
     const int total = starts[L.n];

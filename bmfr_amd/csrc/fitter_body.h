@@ -8,7 +8,10 @@
 //
 // Layout: the whole 1024x13 block matrix lives in VGPRs.  A block is
 // handled by W = 16/RPL waves; each lane owns RPL rows (13*RPL registers),
-// so the QR never touches memory after the initial load.  Column sums are
+// so the QR never touches memory after the initial load.  With binary16
+// tmp_data and one wave per block (RPL 16, the default) the columns are
+// binary16 pairs from the first pivot's update on (PackedRows: 8 registers
+// a column, v_dot2_f32_f16 dot products, half updates).  Column sums are
 // 64-lane DPP reductions (quad_perm, row_half_mirror, row_mirror,
 // row_bcast15, row_bcast31, then v_readlane of lane 63); with W > 1 the W
 // wave partials meet in a few LDS words behind one barrier.  All trailing

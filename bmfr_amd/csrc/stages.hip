@@ -140,8 +140,11 @@ __device__ __forceinline__ double lane_value(double y, int l) {
 __device__ __forceinline__ ScaleRecips scale_recips(const StageBuffers &bufs, int64_t g) {
     const float *mm = (const float *)bufs.mins_maxs + g * (kScaled * 2);
     const int l = (int)(threadIdx.x & 63);
+    // the shift-only case (|max - min| <= 1, or NaN) as a division by 1:
+    // quotient(x, 1.0) is x, so the feature needs no branch and no max
     float d = 1.f;
     if (l < kScaled) d = gload(mm, l * 2 + 1) - gload(mm, l * 2);
+    if (!(fabsf(d) > 1.0f)) d = 1.f;
     const double y = quotient_reciprocal(d);
     ScaleRecips r;
 #pragma unroll
@@ -149,11 +152,9 @@ __device__ __forceinline__ ScaleRecips scale_recips(const StageBuffers &bufs, in
     return r;
 }
 
-// opencl/bmfr.cl:200-205 with the divisor's reciprocal: the bits of scale_feature
-__device__ __forceinline__ float scale_feature_r(float v, float mn, float mx, double y) {
-    if (fabsf(mx - mn) > 1.0f) return quotient(v - mn, y);
-    return v - mn;
-}
+// opencl/bmfr.cl:200-205 with the divisor's reciprocal (1 for the shift-only
+// case, scale_recips): the bits of scale_feature
+__device__ __forceinline__ float scale_feature_r(float v, float mn, double y) { return quotient(v - mn, y); }
 
 // The weights and scaling bounds of one block for the reference's feature
 // set, read once: a kernel that stores before it has used them all would
@@ -192,7 +193,7 @@ __device__ __forceinline__ float3 stage3_color_fit(float3 wp, float3 n, const Bl
         float v = feat[f];
         if (f >= kNotScaled) {
             const int i = f - kNotScaled;
-            v = rec ? scale_feature_r(v, fit.mn[i], fit.mx[i], rec->s[i]) : scale_feature(v, fit.mn[i], fit.mx[i]);
+            v = rec ? scale_feature_r(v, fit.mn[i], rec->s[i]) : scale_feature(v, fit.mn[i], fit.mx[i]);
         }
         // color += weight * feature, one FMA a term: the reference's
         // expression (bmfr.cl:745) under OpenCL's default FP_CONTRACT ON

@@ -653,17 +653,6 @@ __device__ __forceinline__ void stage345_body(const StageArgs &a, const StageBuf
     __shared__ double2 tone_log[256];
     __shared__ double tone_pow2[149];
     static_assert(sizeof(kToneLog) == sizeof(tone_log) && sizeof(kTonePow2) == sizeof(tone_pow2), "tables");
-    // 256 threads: one log entry each, the first 149 one power of two each
-    tone_log[threadIdx.x] = make_double2(kToneLog[threadIdx.x][0], kToneLog[threadIdx.x][1]);
-    if (threadIdx.x < 149) tone_pow2[threadIdx.x] = kTonePow2[threadIdx.x];
-    __syncthreads();
-    const ToneTablesLds tt{tone_log, tone_pow2};
-    ScaleRecips rec;
-    BlockFit fit;
-    if constexpr (!CUSTOM) {
-        rec = scale_recips(bufs, blk);   // all lanes active; blk is the work-group's
-        fit = block_fit(bufs, blk);
-    }
     const int W = a.width, H = a.height;
     const int2 off = block_offset(a.frame);
     const int by = blk / a.blocks_x, bx = blk - by * a.blocks_x;
@@ -707,9 +696,24 @@ __device__ __forceinline__ void stage345_body(const StageArgs &a, const StageBuf
 
     float3 mine[4], hist[4];
     uint32_t has_history = 0;   // bit r: pixel r has a TAA history (bmfr.cl:884-890)
+    // The first pixel's inputs, the block's fit and the tone tables are one
+    // memory round trip: issued together, before the tables' barrier.
     Stage4Inputs in = stage4_inputs(a, bufs, lp[0]);
     float3 wp = load3_px((const GT *)bufs.positions_cur, lp[0]);
     float3 nrm = load3_px((const GT *)bufs.normals_cur, lp[0]);
+    // 256 threads: one log entry each, the first 149 one power of two each
+    const double2 tl = make_double2(kToneLog[threadIdx.x][0], kToneLog[threadIdx.x][1]);
+    const double tp = kTonePow2[min((int)threadIdx.x, 148)];
+    ScaleRecips rec;
+    BlockFit fit;
+    if constexpr (!CUSTOM) {
+        rec = scale_recips(bufs, blk);   // all lanes active; blk is the work-group's
+        fit = block_fit(bufs, blk);
+    }
+    tone_log[threadIdx.x] = tl;
+    if (threadIdx.x < 149) tone_pow2[threadIdx.x] = tp;
+    __syncthreads();
+    const ToneTablesLds tt{tone_log, tone_pow2};
     Stage4Inputs ring_in;
     float3 ring_wp, ring_n;
 #pragma unroll

@@ -117,10 +117,36 @@ def run_plain(args):
             totals.append(float(st.stage_times()[5]))
         elif f >= args.warmup + args.steps:
             per_stage.append(st.stage_times())
+    # The fitter's time inside the chain with no event between it and stage 1:
+    # frames alternate a window from stage 1's start to its end (A) and one
+    # from stage 1's start to the fitter's end (B), one event pair each, so
+    # that the events' own cost cancels in mean(B) - mean(A), which keeps the
+    # chain's stage 1 -> fitter dispatch gap.  The rest of such a frame runs
+    # as separate stages, outside both windows.
+    st.synchronize()
+    st.set_timing(0)
+    win = ([], [])
+    for k in range(2 * args.steps):
+        f = n_frames + k
+        sc.render_to_stream(f, st)
+        vp, off = sc.frame_args(f)
+        st.record_marker(0)
+        st.accumulate_noisy_data(f, vp, off)
+        if k % 2 == 0:
+            st.record_marker(1)
+        st.fitter(f)
+        if k % 2 == 1:
+            st.record_marker(1)
+        st.weighted_sum(f)
+        st.accumulate_filtered_data(f)
+        st.taa(f)
+        st.end_frame()
+        win[k % 2].append(st.marker_elapsed(0, 1))
+    fitter_chain_ms = float(np.mean(win[1]) - np.mean(win[0]))
+    n_frames += 2 * args.steps
     # the fitter's kernel time without dispatch gaps: the last frame's fitter
     # launched back to back, events only around the whole run
     st.synchronize()
-    st.set_timing(0)
     reps = 50
     st.record_marker(0)
     for _ in range(reps):
@@ -130,7 +156,7 @@ def run_plain(args):
     s = st.sizes
     st.close()
     sc.close()
-    return np.array(totals), np.array(per_stage, np.float64), wall, s, fitter_b2b_ms
+    return np.array(totals), np.array(per_stage, np.float64), wall, s, fitter_b2b_ms, fitter_chain_ms
 
 
 BATCH_MIX = [(640, 360)] * 4 + [(1280, 720)] * 6 + [(1920, 1080)] * 4 + [(2560, 1440)] * 2
@@ -456,7 +482,7 @@ def main():
               "exchange)", file=sys.stderr)
         sys.exit(2)
 
-    totals, t, wall, s, fitter_b2b_ms = run_plain(args)
+    totals, t, wall, s, fitter_b2b_ms, fitter_chain_ms = run_plain(args)
     ms_frame = float(totals.mean())
     progress(f"plain 1280x720: {ms_frame:.4f} ms/frame over {len(totals)} frames")
     P, Q, nB = s.pixels, s.padded_pixels, s.blocks
@@ -467,9 +493,16 @@ def main():
     # the metric's kernel (and the longest one in the kernel trace); every
     # stage's fraction of the HBM peak is reported beside it
     dom = "fitter"
-    roof = {"kernel": dom, "bound": "hbm", "achieved": round(stage_gbs[dom], 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(stage_gbs[dom] / HBM_PEAK_GBS, 4),
+    # its duration in the chain: the difference of the two event windows
+    # (run_plain), free of the per-stage events' own cost
+    fit_ms = fitter_chain_ms
+    dom_gbs = sb[dom] / (fit_ms * 1e-3) / 1e9
+    roof = {"kernel": dom, "bound": "hbm", "achieved": round(dom_gbs, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(dom), "algorithmic_bytes": int(sb[dom]),
+            "kernel_ms": round(fit_ms, 5),
+            "kernel_ms_method": "in the chain: events from stage 1's start to the fitter's end minus "
+                                "events from stage 1's start to its end, alternate frames",
             "stage_frac": {k: round(v / HBM_PEAK_GBS, 4) for k, v in stage_gbs.items()},
             # What actually binds the fitter (one wave per 32x32 block, ~1 wave
             # per SIMD at 720p): its dependent VALU instruction chain, not HBM.
@@ -483,7 +516,7 @@ def main():
     # launch padded to whole four-block work-groups per XCD), against one
     # wave64 VALU instruction per 4 cycles per SIMD.
     waves = 8 * (((nB + 7) // 8 + 3) // 4) * 4
-    issue = waves * FITTER_VALU_PER_WAVE / (stage_ms["fitter"] * 1e-3) / 1e9
+    issue = waves * FITTER_VALU_PER_WAVE / (fit_ms * 1e-3) / 1e9
     roof["issue"] = {"bound": "valu", "achieved": round(issue, 1), "peak": VALU_ISSUE_PEAK,
                      "unit": "G wave-instructions/s", "frac": round(issue / VALU_ISSUE_PEAK, 4),
                      "valu_per_wave": FITTER_VALU_PER_WAVE, "waves": waves}
@@ -494,7 +527,7 @@ def main():
     import bmfr_amd
     lds_peak = bmfr_amd.lds_read_bandwidth() / 1e9
     lds_b = reference_fitter_lds_bytes()
-    lds_rate = nB * lds_b / (stage_ms["fitter"] * 1e-3) / 1e9
+    lds_rate = nB * lds_b / (fit_ms * 1e-3) / 1e9
     roof["lds"] = {"bound": "lds", "algorithmic_bytes_per_block": lds_b, "achieved": round(lds_rate, 1),
                    "peak": round(lds_peak, 1), "peak_nominal": round(256 * 128 * 2.4, 1), "unit": "GB/s",
                    "frac": round(lds_rate / lds_peak, 4)}

@@ -70,6 +70,41 @@ def test_accumulate_noisy_data_bit_exact_720p(gpu):
         ls.close()
 
 
+@pytest.mark.parametrize("base", [322_638, 2**31 - 8])
+def test_large_frame_numbers(gpu, base):
+    """Frame numbers past the point where the noise seed's
+    frame * 13 * 1024 wraps 32 bits (frame > 322,638; bmfr.cl:179-181) and
+    next to the int32 limit; the block offset is frame % 16 (bmfr.cl:314).
+    The scene renders its frames 0-3 while the library is given 0, base + 1,
+    base + 2, base + 3: stage 1 bit-exact, the fitter's mins_maxs bit-exact
+    and its weights within the FP16 bar of the oracle's on every frame."""
+    ls = Lockstep(70, 48, tmp_f16=True)
+    try:
+        for k in range(4):
+            fr = 0 if k == 0 else base + k
+            ls.upload(k)
+            vp, off = ls.sc.frame_args(k)
+            ls.g.accumulate_noisy_data(fr, vp, off)
+            ls.o.accumulate_noisy_data(fr, vp, off)
+            c = ls.o.cur()
+            for name, want in [("noisy_acc_cur", ls.o.noisy_acc[c]), ("prev_pixel", ls.o.prev_pixel),
+                               ("accept", ls.o.accept), ("spp_cur", ls.o.spp[c]), ("tmp_data", ls.o.tmp)]:
+                assert np.array_equal(ls.read(name), want), (fr, name)
+            tmp0 = ls.o.tmp.copy()
+            ls.g.fitter(fr)
+            ls.o.fitter(fr)
+            s = ls.o.s
+            assert np.array_equal(ls.read("mins_maxs"), ls.o.mins_maxs), fr
+            Wg = ls.read("weights").reshape(s.blocks, 10, 3).astype(np.float64)
+            Wo = ls.o.weights.reshape(s.blocks, 10, 3).astype(np.float64)
+            fe, we = _fitted_error(ls.o.p, tmp0, fr, s.blocks, Wg, Wo)
+            assert np.isfinite(Wg).all() and fe < 2e-3, (fr, fe, we)
+            ls.g.end_frame()
+            ls.o.end_frame()
+    finally:
+        ls.close()
+
+
 def _fitted_error(p, tmp, frame, blocks, W_got, W_want):
     """Max over blocks of |A W_got - A W_want| / max|A W_want| (fitted colour)
     and of the weight difference relative to the block's weight scale."""

@@ -23,6 +23,8 @@ from tests.harness import Lockstep, f16_round, psnr, rel_err, scene
 pytestmark = pytest.mark.gpu
 
 SIZES = [(47, 47), (70, 48), (256, 256)]   # 47: the smallest size the API accepts
+# one block row / one block column of the padded grid (plus its margin)
+THIN = [(1000, 47), (47, 600)]
 
 
 def _stage1_frames(ls, frames, check=True):
@@ -44,7 +46,7 @@ def _stage1_frames(ls, frames, check=True):
             ls.o.end_frame()
 
 
-@pytest.mark.parametrize("w,h", SIZES)
+@pytest.mark.parametrize("w,h", SIZES + THIN)
 @pytest.mark.parametrize("tmp_f16", [True, False])
 def test_accumulate_noisy_data_bit_exact(gpu, w, h, tmp_f16):
     ls = Lockstep(w, h, tmp_f16=tmp_f16)
@@ -201,7 +203,7 @@ def _run_chain(ls, frames):
     return ls.g.download_result()
 
 
-@pytest.mark.parametrize("w,h", SIZES)
+@pytest.mark.parametrize("w,h", SIZES + THIN)
 def test_stages_345_bit_exact_given_same_inputs(gpu, w, h):
     """Feed the oracle's stages 3-5 the GPU's own inputs to each stage."""
     ls = Lockstep(w, h, tmp_f16=True)
@@ -353,7 +355,8 @@ def test_invalid_fitter_config_and_stream_errors(gpu):
 
 
 @pytest.mark.parametrize("level", [bmfr_amd.FUSE_34, bmfr_amd.FUSE_345])
-@pytest.mark.parametrize("storage_f16,w,h", [(False, 130, 70), (True, 130, 70), (False, 96, 160)])
+@pytest.mark.parametrize("storage_f16,w,h", [(False, 130, 70), (True, 130, 70), (False, 96, 160),
+                                               (False, 1000, 47), (True, 47, 600)])
 def test_fused_chain_equals_separate_stages(gpu, storage_f16, w, h, level):
     """bmfr_run_frame's fused kernels (weighted_sum inside
     accumulate_filtered_data; or all of stages 3-5 per 32x32 block with the

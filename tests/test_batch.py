@@ -18,6 +18,8 @@ MIX = [  # (w, h, storage_f16, tmp_f16, seed offset, qr mode)
     (200, 120, False, False, 3, bmfr_amd.QR_FP32),   # FP32 tmp_data: a third group
     (96, 64, False, False, 4, bmfr_amd.QR_KAHAN),    # compensated QR
     (130, 70, True, False, 5, bmfr_amd.QR_KAHAN_FP64),   # + FP64 re-factorisation
+    (1000, 47, False, True, 6, bmfr_amd.QR_FP32),    # one block row of the padded grid
+    (47, 600, True, True, 7, bmfr_amd.QR_FP32),      # one block column
 ]
 # A threshold at which many blocks are re-factorised in FP64 (the default,
 # 1e-6, flags almost none of these scenes' blocks).
@@ -97,7 +99,7 @@ def test_batch_equals_each_stream_alone(gpu, fuse):
         for name in a:
             assert _same(a[name], m[name]), (k, name)
     # the compensated streams: only the FP64 one re-factorises, and it does
-    assert [int(a["refactored"][0]) > 0 for a in alone] == [False] * 5 + [True]
+    assert [int(a["refactored"][0]) > 0 for a in alone] == [m[5] == bmfr_amd.QR_KAHAN_FP64 for m in MIX]
 
 
 def test_batch_graph_replay_equals_eager(gpu):
